@@ -1,0 +1,134 @@
+/*
+ * pfc_hip.h -- C ABI of the MI355X (gfx950) spectral PFC library.
+ *
+ * This is the drop-in boundary: a C++ host (the pfc::fft::IFFT adapter and
+ * the device-resident PFC model in openpfc_amd/host/) or any FFI (ctypes,
+ * cgo, JNI) binds these symbols.  No C++ or HIP types cross it: handles are
+ * opaque pointers, device buffers are plain `void *`, streams are passed as
+ * `void *` (a hipStream_t value; NULL is the default stream).
+ *
+ * Conventions
+ *   - Every function returns 0 on success or a PFC_HIP_E* code; the message
+ *     of the last failure on the calling thread is pfc_hip_last_error().
+ *     No C++ exception crosses this boundary.
+ *   - Transforms are enqueued on `stream` and return without waiting; the
+ *     caller synchronises the stream before reading results on the host.
+ *   - Layout is the reference's one-rank layout (r2c_direction = 0):
+ *     real fields are nx*ny*nz values with x fastest; spectra are
+ *     (nx/2+1)*ny*nz complex values (re, im interleaved), kx fastest.
+ *   - Precision PFC_HIP_F64 uses double / double-complex buffers,
+ *     PFC_HIP_F32 float / float-complex buffers.
+ *
+ * Supported extents: every axis length in {1,2,4,...,1024}; nx even.
+ * Other lengths are rejected with PFC_HIP_EUNSUPPORTED.
+ */
+#ifndef PFC_HIP_H
+#define PFC_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFC_HIP_ABI_VERSION 1
+
+enum {
+  PFC_HIP_OK = 0,
+  PFC_HIP_EINVAL = 1,       /* bad argument (null pointer, bad size, ...) */
+  PFC_HIP_EUNSUPPORTED = 2, /* extent or model not supported */
+  PFC_HIP_EDEVICE = 3,      /* HIP runtime error (launch, copy, ...) */
+  PFC_HIP_ENOMEM = 4,       /* device allocation failed */
+  PFC_HIP_ESTATE = 5        /* call out of order (e.g. step before operators) */
+};
+
+enum { PFC_HIP_F64 = 0, PFC_HIP_F32 = 1 };
+
+enum { PFC_HIP_MODEL_PFC = 0 };
+
+typedef struct pfc_hip_plan_s *pfc_hip_plan_t;
+
+/* ABI version of the loaded library (PFC_HIP_ABI_VERSION). */
+int pfc_hip_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *pfc_hip_last_error(void);
+
+/*
+ * Create a transform plan for an nx*ny*nz grid on the current device.
+ * Replaces the construction of heffte::fft3d_r2c<rocfft> in
+ *   src/openpfc/runtime/hip/fft_hip.cpp:19-37 (create_hip) and
+ *   src/openpfc/runtime/cpu/fft.cpp:114-121,131-191 (create / create_with_backend)
+ * at one rank: inbox [0,nx-1]x[0,ny-1]x[0,nz-1], outbox [0,nx/2]x[0,ny-1]x[0,nz-1].
+ * flags: reserved, pass 0.
+ */
+int pfc_hip_plan_create(int nx, int ny, int nz, int precision, int flags,
+                        pfc_hip_plan_t *plan);
+
+/* Release a plan and every device buffer it owns. NULL is accepted. */
+int pfc_hip_plan_destroy(pfc_hip_plan_t plan);
+
+/*
+ * Element counts of the real box, the complex box and the transform
+ * workspace (complex elements).  Replaces IFFT::size_inbox/size_outbox/
+ * size_workspace (include/openpfc/kernel/fft/fft_interface.hpp:66-71).
+ */
+int pfc_hip_sizes(pfc_hip_plan_t plan, size_t *inbox, size_t *outbox,
+                  size_t *workspace);
+
+/* Device bytes currently owned by the plan (tables, workspace, step state).
+ * Replaces IFFT::get_allocated_memory_bytes (fft_interface.hpp:73). */
+int pfc_hip_allocated_bytes(pfc_hip_plan_t plan, size_t *bytes);
+
+/*
+ * Forward R2C, unnormalised: out[k] = sum_x in[x] exp(-2 pi i k.x/n).
+ * in: inbox real values; out: outbox complex values (device pointers).
+ * Replaces FFT_Impl<rocfft>::forward(DataBuffer, DataBuffer)
+ * (include/openpfc/kernel/fft/detail/fft_heffte_backend.hpp:146-170).
+ */
+int pfc_hip_r2c(pfc_hip_plan_t plan, const void *in, void *out, void *stream);
+
+/*
+ * Backward C2R scaled by 1/(nx*ny*nz) (heffte::scale::full).  `in` is not
+ * modified; the plan's workspace holds the intermediate spectrum.
+ * Replaces FFT_Impl<rocfft>::backward(DataBuffer, DataBuffer)
+ * (fft_heffte_backend.hpp:200-227).
+ */
+int pfc_hip_c2r(pfc_hip_plan_t plan, const void *in, void *out, void *stream);
+
+/*
+ * Build the ETD1 operators of a spectral model on the device and allocate
+ * the step state.  Replaces the operator precomputation of the model's
+ * initialize(dt):
+ *   model_kind PFC_HIP_MODEL_PFC, params = {dx, dy, dz, eps} (nparams 4):
+ *     opL = exp(L dt), opN = (opL-1)/L * (-k^2) (dt*(-k^2) when |L| <= 1e-14),
+ *     L = -k^2 (eps + (1-k^2)^2)
+ *     (docs/api/examples/10_complete_pfc_model.cpp:166-213).
+ */
+int pfc_hip_pfc_set_operators(pfc_hip_plan_t plan, int model_kind,
+                              const double *params, int nparams, double dt);
+
+/*
+ * Advance psi (device, inbox real values, updated in place) by nsteps ETD1
+ * steps: psi_hat <- opL*F(psi) + opN*F(psi^3); psi <- F^-1(psi_hat).
+ * Replaces PFCModel::step (10_complete_pfc_model.cpp:121-144) called
+ * nsteps times.  Requires pfc_hip_pfc_set_operators.
+ */
+int pfc_hip_pfc_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *stream);
+
+/*
+ * Measurement variant of pfc_hip_pfc_step: same work, with HIP events
+ * around every kernel of the step.  On return (the stream is synchronised)
+ * kernel_ms[i] holds the mean duration in milliseconds of pass i over the
+ * nsteps steps, pass order: 0 forward-x (psi, psi^3), 1 forward-y (both
+ * spectra), 2 z (forward, combine, inverse), 3 inverse-y, 4 inverse-x.
+ * nkernels must be >= 5.  No reference counterpart (profiling aid).
+ */
+int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,
+                           void *stream, double *kernel_ms, int nkernels);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFC_HIP_H */

@@ -1,0 +1,167 @@
+"""openpfc_amd -- MI355X-native spectral PFC time step (HIP, gfx950).
+
+Python access to the C ABI (``include/pfc_hip.h``), for tests and the
+benchmark.  The reference-shaped host API (``pfc::fft::IFFT``, ``Model``,
+``Simulator``) lives in C++ under ``openpfc_amd/host/``.
+
+``HipFft`` mirrors the reference's ``FFT_Impl<rocfft>`` DataBuffer overloads
+(include/openpfc/kernel/fft/detail/fft_heffte_backend.hpp:146-227): sizes are
+checked and a mismatch raises ``ValueError`` (the reference throws
+std::invalid_argument) before anything is enqueued.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+from . import _lib
+from ._lib import F32, F64, MODEL_PFC, PfcHipError, check, load
+
+__all__ = ["HipFft", "PfcStepper", "PfcHipError", "F32", "F64", "load"]
+
+
+def _stream_ptr(stream) -> ctypes.c_void_p:
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(int(getattr(stream, "cuda_stream", stream)))
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(int(t.data_ptr()))
+
+
+class HipFft:
+    """3-D R2C/C2R on one GPU with the reference's one-rank layout.
+
+    Real boxes are (nz, ny, nx) tensors (x fastest), spectra are
+    (nz, ny, nx//2+1) complex tensors.  ``precision`` is F64 or F32.
+    """
+
+    def __init__(self, nx: int, ny: int, nz: int, precision: int = F64):
+        self._lib = load()
+        h = ctypes.c_void_p()
+        check(self._lib.pfc_hip_plan_create(nx, ny, nz, precision, 0, ctypes.byref(h)))
+        self._h = h
+        self.n = (nx, ny, nz)
+        self.precision = precision
+        self._fft_time = 0.0
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.pfc_hip_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    # -- sizes (IFFT::size_inbox/size_outbox/size_workspace) -----------------
+    def _sizes(self):
+        a, b, c = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t()
+        check(self._lib.pfc_hip_sizes(self._h, ctypes.byref(a), ctypes.byref(b),
+                                      ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def size_inbox(self) -> int:
+        return self._sizes()[0]
+
+    def size_outbox(self) -> int:
+        return self._sizes()[1]
+
+    def size_workspace(self) -> int:
+        return self._sizes()[2]
+
+    def get_allocated_memory_bytes(self) -> int:
+        b = ctypes.c_size_t()
+        check(self._lib.pfc_hip_allocated_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    def reset_fft_time(self) -> None:
+        self._fft_time = 0.0
+
+    def get_fft_time(self) -> float:
+        return self._fft_time
+
+    # -- transforms -----------------------------------------------------------
+    def _check(self, real, cplx, what: str):
+        import torch
+        rdt = torch.float64 if self.precision == F64 else torch.float32
+        cdt = torch.complex128 if self.precision == F64 else torch.complex64
+        if real.numel() != self.size_inbox():
+            raise ValueError(f"HipFft::{what}: real buffer size {real.numel()} != "
+                             f"size_inbox() {self.size_inbox()}")
+        if cplx.numel() != self.size_outbox():
+            raise ValueError(f"HipFft::{what}: complex buffer size {cplx.numel()} != "
+                             f"size_outbox() {self.size_outbox()}")
+        if real.dtype != rdt or cplx.dtype != cdt:
+            raise ValueError(f"HipFft::{what}: dtypes {real.dtype}/{cplx.dtype} do not "
+                             f"match the plan precision")
+        if not (real.is_cuda and cplx.is_cuda):
+            raise ValueError(f"HipFft::{what}: buffers must be device tensors")
+        if not (real.is_contiguous() and cplx.is_contiguous()):
+            raise ValueError(f"HipFft::{what}: buffers must be contiguous")
+
+    def forward(self, real_in, cplx_out, stream=None):
+        self._check(real_in, cplx_out, "forward")
+        t0 = time.perf_counter()
+        check(self._lib.pfc_hip_r2c(self._h, _ptr(real_in), _ptr(cplx_out),
+                                    _stream_ptr(stream)))
+        self._fft_time += time.perf_counter() - t0
+
+    def backward(self, cplx_in, real_out, stream=None):
+        self._check(real_out, cplx_in, "backward")
+        t0 = time.perf_counter()
+        check(self._lib.pfc_hip_c2r(self._h, _ptr(cplx_in), _ptr(real_out),
+                                    _stream_ptr(stream)))
+        self._fft_time += time.perf_counter() - t0
+
+
+class PfcStepper:
+    """Device-resident plain conserved PFC ETD1 stepper over a HipFft plan.
+
+    Mirrors PFCModel::initialize/step (docs/api/examples/10_complete_pfc_model.cpp:
+    94-144,166-213): ``initialize(dt)`` builds opL/opN on the device,
+    ``step(psi, n)`` advances ``psi`` in place by n steps.
+    """
+
+    def __init__(self, fft: HipFft, spacing, epsilon: float):
+        self.fft = fft
+        self.spacing = tuple(float(s) for s in spacing)
+        self.epsilon = float(epsilon)
+        self.dt = None
+
+    def initialize(self, dt: float) -> None:
+        params = (ctypes.c_double * 4)(*self.spacing, self.epsilon)
+        check(self.fft._lib.pfc_hip_pfc_set_operators(self.fft.handle, MODEL_PFC, params,
+                                                      4, float(dt)))
+        self.dt = float(dt)
+
+    def _check_psi(self, psi):
+        import torch
+        rdt = torch.float64 if self.fft.precision == F64 else torch.float32
+        if psi.numel() != self.fft.size_inbox() or psi.dtype != rdt or not psi.is_cuda \
+                or not psi.is_contiguous():
+            raise ValueError("PfcStepper.step: psi must be a contiguous device tensor of "
+                             f"size_inbox() {self.fft.size_inbox()} elements of {rdt}")
+
+    def step(self, psi, nsteps: int = 1, stream=None) -> None:
+        self._check_psi(psi)
+        check(self.fft._lib.pfc_hip_pfc_step(self.fft.handle, _ptr(psi), int(nsteps),
+                                             _stream_ptr(stream)))
+
+    def step_timed(self, psi, nsteps: int, stream=None):
+        """Run nsteps with HIP events around each of the 5 passes; returns
+        the mean duration of each pass in ms (stream synchronised)."""
+        self._check_psi(psi)
+        out = (ctypes.c_double * 5)()
+        check(self.fft._lib.pfc_hip_pfc_step_timed(self.fft.handle, _ptr(psi), int(nsteps),
+                                                   _stream_ptr(stream), out, 5))
+        return list(out)

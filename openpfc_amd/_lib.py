@@ -1,0 +1,79 @@
+"""ctypes binding of the C ABI declared in ``include/pfc_hip.h``.
+
+The shared library ``openpfc_amd/libopenpfc_hip.so`` is built in-tree by
+``openpfc_amd.build`` (hipcc, gfx950).  Loading fails loudly when it is
+missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libopenpfc_hip.so")
+
+PFC_HIP_OK = 0
+PFC_HIP_EINVAL = 1
+PFC_HIP_EUNSUPPORTED = 2
+PFC_HIP_EDEVICE = 3
+PFC_HIP_ENOMEM = 4
+PFC_HIP_ESTATE = 5
+
+F64 = 0
+F32 = 1
+MODEL_PFC = 0
+
+# Every symbol include/pfc_hip.h declares, with its ctypes signature.
+_vp = ctypes.c_void_p
+_sz = ctypes.POINTER(ctypes.c_size_t)
+SIGNATURES = {
+    "pfc_hip_abi_version": (ctypes.c_int, []),
+    "pfc_hip_last_error": (ctypes.c_char_p, []),
+    "pfc_hip_plan_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_void_p)]),
+    "pfc_hip_plan_destroy": (ctypes.c_int, [_vp]),
+    "pfc_hip_sizes": (ctypes.c_int, [_vp, _sz, _sz, _sz]),
+    "pfc_hip_allocated_bytes": (ctypes.c_int, [_vp, _sz]),
+    "pfc_hip_r2c": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "pfc_hip_c2r": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "pfc_hip_pfc_set_operators": (ctypes.c_int, [_vp, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.c_int, ctypes.c_double]),
+    "pfc_hip_pfc_step": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
+    "pfc_hip_pfc_step_timed": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp,
+                                              ctypes.POINTER(ctypes.c_double),
+                                              ctypes.c_int]),
+}
+
+_lib = None
+
+
+class PfcHipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[pfc_hip error {code}] {msg}")
+        self.code = code
+
+
+def load(path: str = LIB_PATH):
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+            " (there is no CPU fallback)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != PFC_HIP_OK:
+        msg = load().pfc_hip_last_error().decode(errors="replace")
+        raise PfcHipError(rc, msg)

@@ -1,0 +1,56 @@
+"""In-tree build of the HIP library (gfx950) and the C++ host tests.
+
+    python -m openpfc_amd.build          # builds openpfc_amd/libopenpfc_hip.so
+
+All HIP code is compiled with the packed-FP32 instructions disabled
+(-Xclang -target-feature -Xclang -packed-fp32-ops).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libopenpfc_hip.so")
+ARCH = os.environ.get("PFC_HIP_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++20", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-pass-failed",
+]
+
+
+def hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd, quiet=True):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    out = "\n".join(l for l in (r.stdout + r.stderr).splitlines()
+                    if "packed-fp32-ops' is not a recognized feature" not in l
+                    and "-Wunknown-warning" not in l)
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed ({' '.join(cmd)}):\n{out}")
+    if out.strip() and not quiet:
+        print(out)
+
+
+def build_library(verbose: bool = False) -> str:
+    """Compile every HIP source into openpfc_amd/libopenpfc_hip.so."""
+    srcs = [os.path.join(CSRC, "pfc_hip.hip")]
+    tmp = LIB + ".tmp"
+    _run([hipcc(), *HIP_FLAGS, "-shared", "-o", tmp, *srcs], quiet=not verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build_library(verbose="-v" in sys.argv))

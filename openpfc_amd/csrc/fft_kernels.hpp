@@ -1,0 +1,346 @@
+// fft_kernels.hpp -- HBM-streaming passes of the 3-D R2C/C2R transform and
+// of the fused PFC step, built on the line FFTs of fft_device.hpp.
+//
+// Layout (the reference's r2c_direction = 0 contract, fft.cpp:58-70):
+//   real field     psi[z][y][x], x fastest, nx*ny*nz values
+//   half spectrum  F[z][y][kx], kx in [0, nx/2], row pitch P >= nx/2+1
+//                  (P = nx/2+1 for caller buffers, padded to a 128-byte
+//                  multiple for the plan's own step buffers)
+//
+// Passes
+//   k_x_r2c     one row per line: the real row is read as nx/2 complex
+//               values, transformed, and split into the half spectrum.
+//   k_x_c2r     the inverse: half spectrum row -> nx/2-point inverse ->
+//               interleaved real row (optionally scaled).
+//   k_col       complex transform of L adjacent columns along y or z; the
+//               lanes of a wave cover (column, position) pairs so each load
+//               and store moves L*sizeof(complex) contiguous bytes.
+//   k_pfc_x     fused step, forward x: psi row -> (psi, psi^3) packed as one
+//               complex row -> nx-point transform -> both half spectra.
+//   k_pfc_z     fused step, z: forward z of both spectra, ETD combine with
+//               opL/opN, inverse z, written back in place.
+#pragma once
+
+#include "fft_device.hpp"
+
+namespace pfc_amd {
+
+// ----------------------------------------------------------------------------
+// Column transform along y or z.
+//   column (x, outer) starts at outer*s_outer + x; position p at + p*s_pos.
+//   gridDim.x = ntx * n_outer, gridDim.y = number of fields (1 or 2).
+// ----------------------------------------------------------------------------
+template <typename T> struct ColArgs {
+  const cplx<T> *src[2];
+  cplx<T> *dst[2];
+  const cplx<T> *tw;
+  long s_pos;
+  long s_outer;
+  int W;   // active columns in x
+  int ntx; // tiles in x
+};
+
+template <typename T, typename Plan, int DIR, int L, bool SPLIT>
+__global__ void __launch_bounds__(L *Plan::TL)
+    k_col(ColArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N;
+  __shared__ T lds[(SPLIT ? 1 : 2) * L * N];
+  const int f = blockIdx.y;
+  const cplx<T> *__restrict__ src = a.src[f];
+  cplx<T> *__restrict__ dst = a.dst[f];
+  const int c = threadIdx.x % L, t = threadIdx.x / L;
+  const int tx = blockIdx.x % a.ntx;
+  const long outer = blockIdx.x / a.ntx;
+  const int x = tx * L + c;
+  const bool active = x < a.W;
+  const long base = outer * a.s_outer + x;
+
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    v[e] = active ? src[base + (long)(t + TL * e) * a.s_pos] : cplx<T>{0, 0};
+  const LdsLine<T> ll{lds, c, L, L * N};
+  LineFft<T, Plan, DIR, SPLIT, false>::run(v, t, ll, a.tw);
+  if (active) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) dst[base + (long)(t + TL * e) * a.s_pos] = v[e];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Row helpers: fetch the value held at line position `pp(e)` for every owned
+// element, through LDS (row layout with padding).
+// ----------------------------------------------------------------------------
+template <typename T, int E, int TL, typename PartnerPos>
+__device__ __forceinline__ void row_fetch_partner(const cplx<T> (&v)[E],
+                                                  cplx<T> (&p)[E], int t,
+                                                  T *lds_re, T *lds_im,
+                                                  PartnerPos pp) {
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = pad_pos<true>(t + TL * e);
+    lds_re[i] = v[e].x;
+    lds_im[i] = v[e].y;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int i = pad_pos<true>(pp(t + TL * e));
+    p[e] = cplx<T>{lds_re[i], lds_im[i]};
+  }
+  __syncthreads();
+}
+
+// ----------------------------------------------------------------------------
+// x-axis R2C: real rows of length nx = 2M -> half spectrum rows (M+1 values).
+// Plan is the M-point complex transform; LPB rows per block.
+// ----------------------------------------------------------------------------
+template <typename T> struct XArgs {
+  const T *in;       // real rows (r2c) / spectrum rows (c2r)
+  T *out;            // spectrum rows (r2c) / real rows (c2r)
+  const cplx<T> *tw; // line twiddles
+  const cplx<T> *wn; // exp(-2 pi i k / nx), k = 0..M
+  long rows;
+  int pitch; // spectrum row pitch (complex elements)
+  T scale;   // c2r output scale
+};
+
+template <typename T, typename Plan, int LPB>
+__global__ void __launch_bounds__(LPB *Plan::TL) k_x_r2c(XArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N;
+  constexpr int RP = row_padded(M);
+  __shared__ T lds[2 * LPB * RP];
+  const int t = threadIdx.x % TL, c = threadIdx.x / TL;
+  const long row = (long)blockIdx.x * LPB + c;
+  const bool active = row < a.rows;
+  const long r = active ? row : 0;
+  const cplx<T> *__restrict__ in =
+      reinterpret_cast<const cplx<T> *>(a.in + r * (2L * M));
+  cplx<T> *__restrict__ out = reinterpret_cast<cplx<T> *>(a.out) + r * a.pitch;
+
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = in[t + TL * e];
+  T *re = lds + c * RP;
+  T *im = lds + LPB * RP + c * RP;
+  const LdsLine<T> ll{lds, c * RP, 1, LPB * RP};
+  LineFft<T, Plan, -1, false, true>::run(v, t, ll, a.tw);
+
+  cplx<T> pz[E];
+  row_fetch_partner<T, E, TL>(v, pz, t, re, im,
+                              [](int k) { return k == 0 ? 0 : M - k; });
+  if (!active) return;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + TL * e;
+    const cplx<T> zk = v[e], zc = c_conj(pz[e]);
+    const cplx<T> ev = c_scale(c_add(zk, zc), (T)0.5);
+    const cplx<T> d = c_sub(zk, zc);
+    const cplx<T> od = {(T)0.5 * d.y, (T)-0.5 * d.x}; // -i/2 * d
+    out[k] = c_add(ev, c_mul(a.wn[k], od));
+  }
+  if (t == 0) {
+    // X[M] = Re Z0 - Im Z0 (v[0] holds Z0 on thread 0)
+    out[M] = cplx<T>{v[0].x - v[0].y, (T)0};
+  }
+}
+
+// ----------------------------------------------------------------------------
+// x-axis C2R: half spectrum rows -> real rows (scaled).  The imaginary parts
+// of X[0] and X[M] are ignored, as a half-complex inverse does.
+// ----------------------------------------------------------------------------
+template <typename T, typename Plan, int LPB>
+__global__ void __launch_bounds__(LPB *Plan::TL) k_x_c2r(XArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N;
+  constexpr int RP = row_padded(M);
+  __shared__ T lds[2 * LPB * RP];
+  const int t = threadIdx.x % TL, c = threadIdx.x / TL;
+  const long row = (long)blockIdx.x * LPB + c;
+  const bool active = row < a.rows;
+  const long r = active ? row : 0;
+  const cplx<T> *__restrict__ in =
+      reinterpret_cast<const cplx<T> *>(a.in) + r * a.pitch;
+  cplx<T> *__restrict__ out = reinterpret_cast<cplx<T> *>(a.out + r * (2L * M));
+
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + TL * e;
+    cplx<T> xa = in[k];
+    cplx<T> xb = c_conj(in[M - k]);
+    if (k == 0) {
+      xa.y = (T)0;
+      xb.y = (T)0;
+    }
+    const cplx<T> ev = c_add(xa, xb);
+    const cplx<T> od = c_mulc(c_sub(xa, xb), a.wn[k]); // * exp(+2 pi i k/nx)
+    v[e] = cplx<T>{ev.x - od.y, ev.y + od.x};          // ev + i*od
+  }
+  const LdsLine<T> ll{lds, c * RP, 1, LPB * RP};
+  LineFft<T, Plan, +1, false, true>::run(v, t, ll, a.tw);
+  if (!active) return;
+#pragma unroll
+  for (int e = 0; e < E; ++e) out[t + TL * e] = c_scale(v[e], a.scale);
+}
+
+// ----------------------------------------------------------------------------
+// Fused PFC step, forward x pass.  One row of psi (nx = N real values) is
+// packed as z = psi + i*psi^3, transformed with an N-point complex FFT and
+// split into Psi[k] = (Z[k] + conj Z[N-k])/2 and Nl[k] = (Z[k] - conj Z[N-k])/(2i)
+// for k = 0..N/2.  The nonlinearity is psi*psi*psi, evaluated in that order
+// (10_complete_pfc_model.cpp:131-133).
+// ----------------------------------------------------------------------------
+template <typename T> struct PfcXArgs {
+  const T *psi;
+  cplx<T> *outA; // Psi half spectrum
+  cplx<T> *outB; // N half spectrum
+  const cplx<T> *tw;
+  long rows;
+  int pitch;
+};
+
+template <typename T, typename Plan, int LPB>
+__global__ void __launch_bounds__(LPB *Plan::TL) k_pfc_x(PfcXArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N;
+  constexpr int RP = row_padded(N);
+  __shared__ T lds[2 * LPB * RP];
+  const int t = threadIdx.x % TL, c = threadIdx.x / TL;
+  const long row = (long)blockIdx.x * LPB + c;
+  const bool active = row < a.rows;
+  const long r = active ? row : 0;
+  const T *__restrict__ in = a.psi + r * (long)N;
+
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const T u = in[t + TL * e];
+    v[e] = cplx<T>{u, u * u * u};
+  }
+  T *re = lds + c * RP;
+  T *im = lds + LPB * RP + c * RP;
+  const LdsLine<T> ll{lds, c * RP, 1, LPB * RP};
+  LineFft<T, Plan, -1, false, true>::run(v, t, ll, a.tw);
+
+  cplx<T> pz[E];
+  row_fetch_partner<T, E, TL>(v, pz, t, re, im,
+                              [](int k) { return k == 0 ? 0 : N - k; });
+  if (!active) return;
+  cplx<T> *__restrict__ oa = a.outA + r * a.pitch;
+  cplx<T> *__restrict__ ob = a.outB + r * a.pitch;
+#pragma unroll
+  for (int e = 0; e < E / 2; ++e) {
+    const int k = t + TL * e;
+    const cplx<T> zk = v[e], zc = c_conj(pz[e]);
+    const cplx<T> s = c_add(zk, zc), d = c_sub(zk, zc);
+    oa[k] = c_scale(s, (T)0.5);
+    ob[k] = cplx<T>{(T)0.5 * d.y, (T)-0.5 * d.x};
+  }
+  if (t == 0) {
+    // k = N/2 is owned by thread 0 as element E/2 (for E == 1, N == 1)
+    constexpr int eh = E / 2;
+    const cplx<T> zk = v[eh], zc = c_conj(pz[eh]);
+    const cplx<T> s = c_add(zk, zc), d = c_sub(zk, zc);
+    oa[N / 2] = c_scale(s, (T)0.5);
+    ob[N / 2] = cplx<T>{(T)0.5 * d.y, (T)-0.5 * d.x};
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Fused PFC step, z pass: for L adjacent kx columns at one ky,
+//   A = Fz(Psi) ; B = Fz(Nl) ; A = opL*A + opN*B ; Psi = Fz^-1(A)
+// opL/opN are tables in the spectrum layout (pitch P), already multiplied by
+// the 1/(nx*ny*nz) normalisation of the backward transform.
+// ----------------------------------------------------------------------------
+template <typename T> struct PfcZArgs {
+  cplx<T> *A;
+  const cplx<T> *B;
+  const T *opL;
+  const T *opN;
+  const cplx<T> *tw;
+  long s_pos;   // P*ny
+  int pitch;    // P
+  int W;        // nx/2+1
+  int ntx;
+};
+
+template <typename T, typename Plan, int L, bool SPLIT>
+__global__ void __launch_bounds__(L *Plan::TL) k_pfc_z(PfcZArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N;
+  __shared__ T lds[(SPLIT ? 1 : 2) * L * N];
+  const int c = threadIdx.x % L, t = threadIdx.x / L;
+  const int tx = blockIdx.x % a.ntx;
+  const long y = blockIdx.x / a.ntx;
+  const int x = tx * L + c;
+  const bool active = x < a.W;
+  const long base = y * a.pitch + x;
+  const LdsLine<T> ll{lds, c, L, L * N};
+
+  cplx<T> acc[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e)
+    acc[e] = active ? a.A[base + (long)(t + TL * e) * a.s_pos] : cplx<T>{0, 0};
+  LineFft<T, Plan, -1, SPLIT, false>::run(acc, t, ll, a.tw);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const T l = active ? a.opL[base + (long)(t + TL * e) * a.s_pos] : (T)0;
+    acc[e] = c_scale(acc[e], l);
+  }
+  {
+    cplx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      v[e] = active ? a.B[base + (long)(t + TL * e) * a.s_pos] : cplx<T>{0, 0};
+    LineFft<T, Plan, -1, SPLIT, false>::run(v, t, ll, a.tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const T n = active ? a.opN[base + (long)(t + TL * e) * a.s_pos] : (T)0;
+      acc[e] = c_add(acc[e], c_scale(v[e], n));
+    }
+  }
+  LineFft<T, Plan, +1, SPLIT, false>::run(acc, t, ll, a.tw);
+  if (active) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) a.A[base + (long)(t + TL * e) * a.s_pos] = acc[e];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// PFC operators (10_complete_pfc_model.cpp:166-213), evaluated per mode on
+// the device in the reference's formula and order, times `scale`.
+// ----------------------------------------------------------------------------
+struct PfcOpArgs {
+  double fx, fy, fz; // 2*pi/(d*n) per axis
+  double eps, dt, scale;
+  int nx, ny, nz, W, pitch;
+};
+
+template <typename T>
+__global__ void k_pfc_operators(PfcOpArgs a, T *opL, T *opN) {
+  const long total = (long)a.W * a.ny * a.nz;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % a.W);
+    const long jk = idx / a.W;
+    const int j = (int)(jk % a.ny);
+    const int k = (int)(jk / a.ny);
+    const double kx = (i <= a.nx / 2) ? i * a.fx : (i - a.nx) * a.fx;
+    const double ky = (j <= a.ny / 2) ? j * a.fy : (j - a.ny) * a.fy;
+    const double kz = (k <= a.nz / 2) ? k * a.fz : (k - a.nz) * a.fz;
+    const double k2 = __dadd_rn(__dadd_rn(__dmul_rn(kx, kx), __dmul_rn(ky, ky)),
+                                __dmul_rn(kz, kz));
+    const double k2_lap = -k2;
+    const double one_m = 1.0 - k2;
+    const double L = __dmul_rn(k2_lap, __dadd_rn(a.eps, __dmul_rn(one_m, one_m)));
+    const double oL = exp(__dmul_rn(L, a.dt));
+    double oN;
+    if (fabs(L) > 1e-14)
+      oN = __dmul_rn((oL - 1.0) / L, k2_lap);
+    else
+      oN = __dmul_rn(a.dt, k2_lap);
+    const long o = jk * a.pitch + i;
+    opL[o] = (T)(oL * a.scale);
+    opN[o] = (T)(oN * a.scale);
+  }
+}
+
+} // namespace pfc_amd
