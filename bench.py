@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Benchmark of the fused spectral PFC ETD1 step on one MI355X.
+
+Workload (default): plain conserved PFC, 1024^3 periodic grid, FP64,
+dx = pi/4, dt = 0.5, r = -0.25; psi0 = -0.2 + U(-0.01, 0.01) (counter-based
+splitmix64 stream, seed 1234) with a one-mode bcc seed (rho = -0.2,
+amplitude 0.1, radius 64*dx, untouched outside) at the centre of a grid whose
+origin is -N*dx/2 per axis.  A step is one ETD1 step:
+psi_hat = opL*F(psi) + opN*F(psi^3), psi = F^-1(psi_hat)
+(docs/api/examples/10_complete_pfc_model.cpp:121-144).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  `value` is FP64 steps/s with the field resident in
+HBM; the same workload in FP32 is reported under "fp32".  `roofline` is the
+dominant kernel's algorithmic bytes per launch over its mean launch duration
+(HIP events on the stream the kernels run on).  `cpu_baseline` times the CPU
+oracle (the repository's C restatement of the reference path, OpenMP) on one
+step of the same 1024^3 workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "steps/s at 1024³: FP32-mixed vs FP64, ns/DOF/step, % of HBM peak"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+# One-mode amplitude of the seed.  The reference demo's 0.3 makes the explicit
+# psi^3 term of ETD1 unstable at dt = 0.5 (the CPU oracle diverges within 8
+# steps as well); 0.1 is stable and near the equilibrium amplitude.
+SEED_AMP = 0.1
+PASS_NAMES = ["x_fwd_pfc", "y_fwd", "z_fused", "y_inv", "x_inv_c2r"]
+KERNEL_NAMES = {  # device kernel symbol stems, for matching rocprof output
+    "x_fwd_pfc": "k_pfc_x", "y_fwd": "k_col", "z_fused": "k_pfc_z",
+    "y_inv": "k_col", "x_inv_c2r": "k_x_c2r"}
+
+
+def pass_bytes(n: int, esize: int):
+    """Algorithmic HBM bytes per launch of each pass: one read and one write of
+    every array the pass streams (opL/opN tables included, as they are read)."""
+    R = esize * n ** 3                     # one real field
+    C = 2 * esize * (n // 2 + 1) * n * n   # one half spectrum
+    T = esize * (n // 2 + 1) * n * n       # one operator table
+    return {"x_fwd_pfc": R + 2 * C, "y_fwd": 4 * C, "z_fused": 3 * C + 2 * T,
+            "y_inv": 2 * C, "x_inv_c2r": C + R}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def setup(P, torch, n, prec, dx, dt, r):
+    fft = P.HipFft(n, n, n, precision=prec)
+    st = P.PfcStepper(fft, (dx, dx, dx), r)
+    st.initialize(dt)
+    dtype = torch.float64 if prec == P.F64 else torch.float32
+    psi = torch.empty(n ** 3, dtype=dtype, device="cuda")
+    origin = (-0.5 * n * dx,) * 3
+    P.ic_constant(fft, psi, -0.2)
+    P.ic_add_noise(fft, psi, 1234, -0.01, 0.01)
+    P.ic_single_seed(fft, psi, origin, (dx, dx, dx), -0.2, SEED_AMP, 64 * dx, keep_outside=True)
+    torch.cuda.synchronize()
+    return fft, st, psi
+
+
+def time_steps(torch, st, psi, steps, warmup):
+    st.step(psi, warmup)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(s)
+    st.step(psi, steps)
+    e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / steps, wall
+
+
+def cpu_baseline(n, dx, dt, r, nsteps):
+    """One (or nsteps) oracle step(s) on the same workload; returns steps/s."""
+    import numpy as np
+    from oracle import oracle as O
+    a = np.full((n, n, n), -0.2)
+    O.add_counter_noise(a, 1234, -0.01, 0.01)
+    O.ic_single_seed_masked(a, (-0.5 * n * dx,) * 3, (dx, dx, dx), -0.2, SEED_AMP, 64 * dx, 1)
+    opL, opN = O.pfc_operators((n, n, n), (dx, dx, dx), r, dt)
+    t0 = time.perf_counter()
+    O.pfc_step(a, opL, opN, nsteps)
+    dt_s = (time.perf_counter() - t0) / nsteps
+    return 1.0 / dt_s, O.num_threads()
+
+
+def copy_calibration(torch, nbytes=4 << 30, reps=5):
+    """Device-to-device copy of nbytes (read nbytes + write nbytes per launch)."""
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return {"bytes_per_launch": 2 * nbytes, "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also time a device-to-device copy of 2 x 4 GiB (the plain "
+                         "copy kernel whose PMC byte counts scale the traffic figure)")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (no sharding or collectives); "
+              "use --gpus 1", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import openpfc_amd as P
+
+    torch.cuda.set_device(0)
+    n, dx, dt, r = args.n, math.pi / 4, 0.5, -0.25
+
+    # ---- FP64 (value) -----------------------------------------------------
+    log(f"fp64 {n}^3: setup")
+    fft, st, psi = setup(P, torch, n, P.F64, dx, dt, r)
+    ms64, wall64 = time_steps(torch, st, psi, args.steps, args.warmup)
+    kms = st.step_timed(psi, 3)
+    finite = bool(torch.isfinite(psi).all().item())
+    mean64 = psi.mean().item()
+    del psi, st, fft
+    torch.cuda.empty_cache()
+
+    pb = pass_bytes(n, 8)
+    dom = max(range(5), key=lambda i: kms[i])
+    dname = PASS_NAMES[dom]
+    achieved = pb[dname] / (kms[dom] * 1e-3)
+    step_bytes = sum(pb.values())
+
+    out = {
+        "metric": METRIC,
+        "value": 1000.0 / ms64,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms64,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: -0.2 + U(-0.01,0.01) noise (splitmix64 seed 1234) + bcc one-mode seed",
+        "config": {"workload": f"plain PFC ETD1 step {n}^3 FP64", "grid": [n, n, n],
+                   "dx": dx, "dt": dt, "r": r, "psi_mean": -0.2, "noise": 0.01,
+                   "seed_radius": 64 * dx, "seed_amplitude": SEED_AMP},
+        "ns_per_dof_step": ms64 * 1e6 / n ** 3,
+        "step_hbm_frac": step_bytes / (ms64 * 1e-3) / HBM_PEAK,
+        "step_alg_bytes": step_bytes,
+        "wall_s_timed": wall64,
+        "kernels_ms": dict(zip(PASS_NAMES, kms)),
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dname], "pass": dname,
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "alg_bytes_per_launch": pb[dname]},
+        "checks": {"finite": finite, "mean_after": mean64},
+    }
+
+    # ---- FP32 (same workload) ---------------------------------------------
+    log(f"fp64 done: {ms64:.3f} ms/step")
+    if not args.no_fp32:
+        log("fp32: setup")
+        fft, st, psi = setup(P, torch, n, P.F32, dx, dt, r)
+        ms32, _ = time_steps(torch, st, psi, args.steps, args.warmup)
+        k32 = st.step_timed(psi, 3)
+        del psi, st, fft
+        torch.cuda.empty_cache()
+        out["fp32"] = {"mode": "fp32 storage and arithmetic (plain)",
+                       "steps_per_s": 1000.0 / ms32, "ms_per_step": ms32,
+                       "ns_per_dof_step": ms32 * 1e6 / n ** 3,
+                       "speedup_vs_fp64": ms64 / ms32,
+                       "step_hbm_frac": sum(pass_bytes(n, 4).values()) / (ms32 * 1e-3) / HBM_PEAK,
+                       "kernels_ms": dict(zip(PASS_NAMES, k32))}
+
+    # ---- CPU baseline (oracle, rank 0, N=1) --------------------------------
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle) start")
+        sps, threads = cpu_baseline(n, dx, dt, r, args.cpu_steps)
+        out["cpu_baseline"] = {
+            "value": sps, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{args.cpu_steps} step(s) of the same {n}^3 FP64 workload, "
+                      f"oracle C restatement (own Stockham FFT, OpenMP {threads} threads)"}
+
+    if args.calibrate_copy:
+        out["copy_calibration"] = copy_calibration(torch)
+
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

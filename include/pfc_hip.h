@@ -127,6 +127,31 @@ int pfc_hip_pfc_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *stream);
 int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,
                            void *stream, double *kernel_ms, int nkernels);
 
+/*
+ * Initial conditions applied on the device to a real field of the plan's
+ * inbox (x = origin + i*spacing, x fastest).  All enqueue on `stream`.
+ *
+ * pfc_hip_ic_constant: psi = value everywhere.
+ *   Replaces Constant::apply
+ *   (include/openpfc/kernel/simulation/initial_conditions/constant.hpp:82-89).
+ * pfc_hip_ic_single_seed: one-mode bcc seed rho + sum_q 2*amp*cos(q.x), q the
+ *   six <110>/sqrt(2) vectors, inside x^2+y^2+z^2 < radius^2.  Outside it
+ *   writes 0.0 when keep_outside == 0 (the reference behaviour, where radius
+ *   is 64.0) or leaves psi unchanged when keep_outside == 1.
+ *   Replaces SingleSeed::apply (single_seed.hpp:51-83).
+ * pfc_hip_ic_add_noise: psi += lo + (hi-lo)*u_i with
+ *   u_i = (splitmix64(splitmix64(seed) + i) >> 11) * 2^-53, i the linear
+ *   index.  No reference counterpart (the reference has no noise IC); the
+ *   stream is counter-based so any host can reproduce it exactly.
+ */
+int pfc_hip_ic_constant(pfc_hip_plan_t plan, void *psi, double value,
+                        void *stream);
+int pfc_hip_ic_single_seed(pfc_hip_plan_t plan, void *psi, const double *origin3,
+                           const double *spacing3, double rho, double amp,
+                           double radius, int keep_outside, void *stream);
+int pfc_hip_ic_add_noise(pfc_hip_plan_t plan, void *psi, unsigned long long seed,
+                         double lo, double hi, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

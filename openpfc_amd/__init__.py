@@ -124,6 +124,43 @@ class HipFft:
         self._fft_time += time.perf_counter() - t0
 
 
+def _check_field(fft: HipFft, psi, what: str):
+    import torch
+    rdt = torch.float64 if fft.precision == F64 else torch.float32
+    if psi.numel() != fft.size_inbox() or psi.dtype != rdt or not psi.is_cuda \
+            or not psi.is_contiguous():
+        raise ValueError(f"{what}: psi must be a contiguous device tensor of "
+                         f"size_inbox() {fft.size_inbox()} elements of {rdt}")
+
+
+def ic_constant(fft: HipFft, psi, value: float, stream=None) -> None:
+    """Constant::apply (initial_conditions/constant.hpp:82-89) on the device."""
+    _check_field(fft, psi, "ic_constant")
+    check(fft._lib.pfc_hip_ic_constant(fft.handle, _ptr(psi), float(value),
+                                       _stream_ptr(stream)))
+
+
+def ic_single_seed(fft: HipFft, psi, origin, spacing, rho: float, amplitude: float,
+                   radius: float = 64.0, keep_outside: bool = False, stream=None) -> None:
+    """SingleSeed::apply (initial_conditions/single_seed.hpp:51-83) on the device.
+
+    keep_outside=False writes 0.0 outside the sphere, as the reference does;
+    True leaves the field untouched there."""
+    _check_field(fft, psi, "ic_single_seed")
+    o = (ctypes.c_double * 3)(*origin)
+    s = (ctypes.c_double * 3)(*spacing)
+    check(fft._lib.pfc_hip_ic_single_seed(fft.handle, _ptr(psi), o, s, float(rho),
+                                          float(amplitude), float(radius),
+                                          int(bool(keep_outside)), _stream_ptr(stream)))
+
+
+def ic_add_noise(fft: HipFft, psi, seed: int, lo: float, hi: float, stream=None) -> None:
+    """psi += lo + (hi-lo)*u_i, u_i from the counter-based splitmix64 stream."""
+    _check_field(fft, psi, "ic_add_noise")
+    check(fft._lib.pfc_hip_ic_add_noise(fft.handle, _ptr(psi), ctypes.c_ulonglong(seed),
+                                        float(lo), float(hi), _stream_ptr(stream)))
+
+
 class PfcStepper:
     """Device-resident plain conserved PFC ETD1 stepper over a HipFft plan.
 

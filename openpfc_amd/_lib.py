@@ -44,6 +44,13 @@ SIGNATURES = {
     "pfc_hip_pfc_step_timed": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp,
                                               ctypes.POINTER(ctypes.c_double),
                                               ctypes.c_int]),
+    "pfc_hip_ic_constant": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _vp]),
+    "pfc_hip_ic_single_seed": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_double),
+                                              ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_int, _vp]),
+    "pfc_hip_ic_add_noise": (ctypes.c_int, [_vp, _vp, ctypes.c_ulonglong, ctypes.c_double,
+                                            ctypes.c_double, _vp]),
 }
 
 _lib = None

@@ -343,4 +343,85 @@ __global__ void k_pfc_operators(PfcOpArgs a, T *opL, T *opN) {
   }
 }
 
+// ----------------------------------------------------------------------------
+// Initial conditions on the device, in the reference's coordinate convention
+// x = origin + i*spacing (include/openpfc/kernel/data/domain.hpp:185-189) and
+// x-fastest linear order (include/openpfc/kernel/field/operations.hpp:81-105).
+// ----------------------------------------------------------------------------
+template <typename T>
+__global__ void k_ic_constant(T *psi, long n, double value) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x)
+    psi[i] = (T)value;
+}
+
+// splitmix64 finaliser (Steele, Lea, Flood 2014; public-domain algorithm).
+__host__ __device__ inline unsigned long long splitmix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+// psi[i] += lo + (hi-lo)*u_i, u_i = (splitmix64(splitmix64(seed) + i) >> 11)*2^-53:
+// a counter-based uniform stream, identical on host and device.
+template <typename T>
+__global__ void k_ic_add_noise(T *psi, long n, unsigned long long seed,
+                               double lo, double hi) {
+  const unsigned long long s0 = splitmix64(seed);
+  const double w = hi - lo;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    const double u =
+        (double)(splitmix64(s0 + (unsigned long long)i) >> 11) * 0x1.0p-53;
+    psi[i] = (T)__dadd_rn((double)psi[i], __dadd_rn(lo, __dmul_rn(w, u)));
+  }
+}
+
+// SingleSeed (include/openpfc/kernel/simulation/initial_conditions/single_seed.hpp:51-83):
+// inside x^2+y^2+z^2 < radius^2 the value is rho + sum_q 2*amp*cos(q.x) over
+// the six <110>/sqrt(2) vectors; outside it writes 0.0 (keep_outside = 0, the
+// reference behaviour) or leaves the field untouched (keep_outside = 1).
+struct SeedArgs {
+  double x0, y0, z0, dx, dy, dz, rho, amp, r2;
+  int nx, ny, nz, keep_outside;
+};
+
+template <typename T> __global__ void k_ic_single_seed(T *psi, SeedArgs a) {
+  const long n = (long)a.nx * a.ny * a.nz;
+  const double s = 1.0 / sqrt(2.0);
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % a.nx);
+    const long jk = idx / a.nx;
+    const int j = (int)(jk % a.ny);
+    const int k = (int)(jk / a.ny);
+    const double x = __dadd_rn(a.x0, __dmul_rn((double)i, a.dx));
+    const double y = __dadd_rn(a.y0, __dmul_rn((double)j, a.dy));
+    const double z = __dadd_rn(a.z0, __dmul_rn((double)k, a.dz));
+    const double rr =
+        __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
+    if (rr >= a.r2) {
+      if (!a.keep_outside) psi[idx] = (T)0;
+      continue;
+    }
+    const double sx = __dmul_rn(s, x), sy = __dmul_rn(s, y), sz = __dmul_rn(s, z);
+    // q.x for q1..q6 = (s,s,0) (s,0,s) (0,s,s) (s,0,-s) (s,-s,0) (0,s,-s),
+    // summed as (qx*x + qy*y) + qz*z with the zero terms kept
+    const double zero = 0.0;
+    const double arg[6] = {
+        __dadd_rn(__dadd_rn(sx, sy), __dmul_rn(zero, z)),
+        __dadd_rn(__dadd_rn(sx, __dmul_rn(zero, y)), sz),
+        __dadd_rn(__dadd_rn(__dmul_rn(zero, x), sy), sz),
+        __dadd_rn(__dadd_rn(sx, __dmul_rn(zero, y)), -sz),
+        __dadd_rn(__dadd_rn(sx, -sy), __dmul_rn(zero, z)),
+        __dadd_rn(__dadd_rn(__dmul_rn(zero, x), sy), -sz)};
+    double u = a.rho;
+    const double two_amp = __dmul_rn(2.0, a.amp);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) u = __dadd_rn(u, __dmul_rn(two_amp, cos(arg[q])));
+    psi[idx] = (T)u;
+  }
+}
+
 } // namespace pfc_amd

@@ -419,6 +419,10 @@ int step_impl(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
   return 0;
 }
 
+unsigned ic_blocks(long n) {
+  return (unsigned)std::min<long>((n + 255) / 256, 1L << 20);
+}
+
 int check_plan(pfc_hip_plan_t p) {
   if (!p) return fail(PFC_HIP_EINVAL, "null plan");
   return 0;
@@ -637,6 +641,73 @@ int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,
   }
   for (auto &e : ev) (void)hipEventDestroy(e);
   return rc;
+}
+
+int pfc_hip_ic_constant(pfc_hip_plan_t plan, void *psi, double value,
+                        void *stream) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!psi) return fail(PFC_HIP_EINVAL, "pfc_hip_ic_constant: null psi");
+  if (int rc = set_device(plan)) return rc;
+  const long n = (long)plan->inbox();
+  hipStream_t s = (hipStream_t)stream;
+  if (plan->prec == PFC_HIP_F64)
+    hipLaunchKernelGGL(k_ic_constant<double>, dim3(ic_blocks(n)), dim3(256),
+                       0, s, (double *)psi, n, value);
+  else
+    hipLaunchKernelGGL(k_ic_constant<float>, dim3(ic_blocks(n)), dim3(256), 0,
+                       s, (float *)psi, n, value);
+  PFC_HIP_TRY(hipGetLastError(), "ic_constant launch");
+  return 0;
+}
+
+int pfc_hip_ic_single_seed(pfc_hip_plan_t plan, void *psi, const double *origin3,
+                           const double *spacing3, double rho, double amp,
+                           double radius, int keep_outside, void *stream) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!psi || !origin3 || !spacing3)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_ic_single_seed: null argument");
+  if (int rc = set_device(plan)) return rc;
+  SeedArgs a{};
+  a.x0 = origin3[0];
+  a.y0 = origin3[1];
+  a.z0 = origin3[2];
+  a.dx = spacing3[0];
+  a.dy = spacing3[1];
+  a.dz = spacing3[2];
+  a.rho = rho;
+  a.amp = amp;
+  a.r2 = radius * radius;
+  a.nx = plan->nx;
+  a.ny = plan->ny;
+  a.nz = plan->nz;
+  a.keep_outside = keep_outside ? 1 : 0;
+  const long n = (long)plan->inbox();
+  hipStream_t s = (hipStream_t)stream;
+  if (plan->prec == PFC_HIP_F64)
+    hipLaunchKernelGGL(k_ic_single_seed<double>, dim3(ic_blocks(n)),
+                       dim3(256), 0, s, (double *)psi, a);
+  else
+    hipLaunchKernelGGL(k_ic_single_seed<float>, dim3(ic_blocks(n)), dim3(256),
+                       0, s, (float *)psi, a);
+  PFC_HIP_TRY(hipGetLastError(), "ic_single_seed launch");
+  return 0;
+}
+
+int pfc_hip_ic_add_noise(pfc_hip_plan_t plan, void *psi, unsigned long long seed,
+                         double lo, double hi, void *stream) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!psi) return fail(PFC_HIP_EINVAL, "pfc_hip_ic_add_noise: null psi");
+  if (int rc = set_device(plan)) return rc;
+  const long n = (long)plan->inbox();
+  hipStream_t s = (hipStream_t)stream;
+  if (plan->prec == PFC_HIP_F64)
+    hipLaunchKernelGGL(k_ic_add_noise<double>, dim3(ic_blocks(n)), dim3(256),
+                       0, s, (double *)psi, n, seed, lo, hi);
+  else
+    hipLaunchKernelGGL(k_ic_add_noise<float>, dim3(ic_blocks(n)), dim3(256), 0,
+                       s, (float *)psi, n, seed, lo, hi);
+  PFC_HIP_TRY(hipGetLastError(), "ic_add_noise launch");
+  return 0;
 }
 
 } // extern "C"

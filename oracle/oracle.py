@@ -55,6 +55,10 @@ def lib():
         L.orc_tungsten_step.argtypes = [ctypes.c_int] * 3 + [_dp] * 6 + [ctypes.c_int]
         L.orc_ic_constant.argtypes = [_dp, ctypes.c_size_t, ctypes.c_double]
         L.orc_ic_single_seed.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 9 + [_dp]
+        L.orc_add_counter_noise.argtypes = [_dp, ctypes.c_size_t, ctypes.c_uint64,
+                                            ctypes.c_double, ctypes.c_double]
+        L.orc_ic_single_seed_masked.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 9
+                                                + [ctypes.c_int, _dp])
         L.orc_sum.restype = ctypes.c_double
         L.orc_sum.argtypes = [_dp, ctypes.c_size_t]
         L.orc_sum_sq.restype = ctypes.c_double
@@ -203,3 +207,17 @@ def sum_sq(a: np.ndarray) -> float:
 
 def num_threads() -> int:
     return int(lib().orc_num_threads())
+
+
+def add_counter_noise(a: np.ndarray, seed: int, lo: float, hi: float) -> np.ndarray:
+    """psi += lo + (hi-lo)*u_i with the counter-based splitmix64 stream."""
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    lib().orc_add_counter_noise(_p(a), a.size, ctypes.c_uint64(seed), lo, hi)
+    return a
+
+
+def ic_single_seed_masked(a: np.ndarray, origin, spacing, rho, amp, radius, keep_outside):
+    nz, ny, nx = a.shape
+    lib().orc_ic_single_seed_masked(nx, ny, nz, *origin, *spacing, rho, amp, radius,
+                                    int(keep_outside), _p(a))
+    return a

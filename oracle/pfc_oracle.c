@@ -466,10 +466,11 @@ void orc_pfc_operators(int nx, int ny, int nz, double dx, double dy, double dz,
   const double fx = two_pi / (dx * nx), fy = two_pi / (dy * ny),
                fz = two_pi / (dz * nz);
   const int w = nx / 2 + 1;
-  size_t idx = 0;
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < w; ++i) {
+        const size_t idx = ((size_t)k * ny + j) * w + i;
         double kx = (i <= nx / 2) ? i * fx : (i - nx) * fx;
         double ky = (j <= ny / 2) ? j * fy : (j - ny) * fy;
         double kz = (k <= nz / 2) ? k * fz : (k - nz) * fz;
@@ -482,7 +483,6 @@ void orc_pfc_operators(int nx, int ny, int nz, double dx, double dy, double dz,
           opN[idx] = (oL - 1.0) / L * k2_lap;
         else
           opN[idx] = dt * k2_lap;
-        ++idx;
       }
 }
 
@@ -651,6 +651,52 @@ void orc_ic_single_seed(int nx, int ny, int nz, double x0, double y0,
                  cos((q[qi][0] * x) + (q[qi][1] * y) + (q[qi][2] * z));
         }
         psi[idx++] = u;
+      }
+}
+
+/* Counter-based noise (no reference counterpart): psi[i] += lo + (hi-lo)*u_i,
+ * u_i = (splitmix64(splitmix64(seed) + i) >> 11) * 2^-53. */
+static uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+void orc_add_counter_noise(double *psi, size_t n, uint64_t seed, double lo,
+                           double hi) {
+  const uint64_t s0 = splitmix64(seed);
+  const double w = hi - lo;
+  for (size_t i = 0; i < n; ++i) {
+    double u = (double)(splitmix64(s0 + (uint64_t)i) >> 11) * 0x1.0p-53;
+    psi[i] = psi[i] + (lo + w * u);
+  }
+}
+
+/* SingleSeed with the outside behaviour selectable: keep_outside = 0 writes
+ * 0.0 outside (single_seed.hpp:73-74), 1 leaves psi unchanged there. */
+void orc_ic_single_seed_masked(int nx, int ny, int nz, double x0, double y0,
+                               double z0, double dx, double dy, double dz,
+                               double rho, double amp, double radius,
+                               int keep_outside, double *psi) {
+  const double s = 1.0 / sqrt(2.0);
+  const double q[6][3] = {{s, s, 0}, {s, 0, s},  {0, s, s},
+                          {s, 0, -s}, {s, -s, 0}, {0, s, -s}};
+  const double r2 = radius * radius;
+  size_t idx = 0;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i, ++idx) {
+        const double x = x0 + i * dx, y = y0 + j * dy, z = z0 + k * dz;
+        if (x * x + y * y + z * z >= r2) {
+          if (!keep_outside) psi[idx] = 0.0;
+          continue;
+        }
+        double u = rho;
+        for (int qi = 0; qi < 6; ++qi)
+          u += 2.0 * amp *
+               cos((q[qi][0] * x) + (q[qi][1] * y) + (q[qi][2] * z));
+        psi[idx] = u;
       }
 }
 
