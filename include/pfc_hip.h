@@ -152,6 +152,51 @@ int pfc_hip_ic_single_seed(pfc_hip_plan_t plan, void *psi, const double *origin3
 int pfc_hip_ic_add_noise(pfc_hip_plan_t plan, void *psi, unsigned long long seed,
                          double lo, double hi, void *stream);
 
+/*
+ * Operator tables of the fused step, written in the outbox layout (W = nx/2+1
+ * pitch) to opL and opN (device, outbox real values each), scaled by
+ * 1/(nx*ny*nz).  The step itself evaluates them per mode on the fly with the
+ * same device code; this entry point exists to inspect and test them.
+ * Counterpart of m_operator_L / m_operator_N
+ * (docs/api/examples/10_complete_pfc_model.cpp:166-213).
+ */
+int pfc_hip_pfc_operators(pfc_hip_plan_t plan, void *opL, void *opN, void *stream);
+
+/*
+ * Launch variants.  Every pass of the transforms and of the step exists in
+ * several compile-time shapes (radix plan, values per thread, column tile
+ * width or rows per block, LDS exchange mode).  Roles:
+ *   0 x forward (plain R2C)  1 x inverse (C2R and step)  2 y columns
+ *   3 z columns (plain transforms)  4 step forward x  5 step z pass
+ * Index 0 is the default of each role.  No reference counterpart: the
+ * reference delegates shapes to rocFFT; its GPU autotuner only times 1-D
+ * block sizes (include/openpfc/runtime/common/gpu_autotune.hpp:295-333).
+ */
+enum {
+  PFC_HIP_ROLE_X_R2C = 0,
+  PFC_HIP_ROLE_X_C2R = 1,
+  PFC_HIP_ROLE_COL_Y = 2,
+  PFC_HIP_ROLE_COL_Z = 3,
+  PFC_HIP_ROLE_PFC_X = 4,
+  PFC_HIP_ROLE_PFC_Z = 5,
+  PFC_HIP_ROLE_COUNT = 6
+};
+int pfc_hip_variant_count(pfc_hip_plan_t plan, int role, int *count);
+/* Name of a variant ("r16x16x4/E16/L8/split"), NULL if out of range. */
+const char *pfc_hip_variant_name(pfc_hip_plan_t plan, int role, int index);
+int pfc_hip_variant_get(pfc_hip_plan_t plan, int role, int *index);
+int pfc_hip_variant_set(pfc_hip_plan_t plan, int role, int index);
+
+/*
+ * Time every variant of the step's roles (forward x, y columns, z pass,
+ * inverse x) on full steps of psi, `reps` timed steps each after one warm-up,
+ * and keep the fastest per role.  psi is restored before return (the stream
+ * is synchronised).  times_ms (may be NULL) receives the mean pass time of
+ * every variant tried, roles in the order above, up to max_times entries.
+ */
+int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
+                         double *times_ms, int max_times);
+
 #ifdef __cplusplus
 }
 #endif

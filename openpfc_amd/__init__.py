@@ -90,6 +90,23 @@ class HipFft:
     def get_fft_time(self) -> float:
         return self._fft_time
 
+    # -- launch variants ------------------------------------------------------
+    ROLES = {"x_r2c": 0, "x_c2r": 1, "col_y": 2, "col_z": 3, "pfc_x": 4, "pfc_z": 5}
+
+    def variants(self, role) -> list:
+        r = self.ROLES.get(role, role)
+        c = ctypes.c_int()
+        check(self._lib.pfc_hip_variant_count(self._h, r, ctypes.byref(c)))
+        return [self._lib.pfc_hip_variant_name(self._h, r, i).decode() for i in range(c.value)]
+
+    def get_variant(self, role) -> int:
+        c = ctypes.c_int()
+        check(self._lib.pfc_hip_variant_get(self._h, self.ROLES.get(role, role), ctypes.byref(c)))
+        return c.value
+
+    def set_variant(self, role, index: int) -> None:
+        check(self._lib.pfc_hip_variant_set(self._h, self.ROLES.get(role, role), int(index)))
+
     # -- transforms -----------------------------------------------------------
     def _check(self, real, cplx, what: str):
         import torch
@@ -193,6 +210,34 @@ class PfcStepper:
         self._check_psi(psi)
         check(self.fft._lib.pfc_hip_pfc_step(self.fft.handle, _ptr(psi), int(nsteps),
                                              _stream_ptr(stream)))
+
+    def operators(self, stream=None):
+        """(opL, opN) as device tensors in the outbox layout, scaled by 1/(nx ny nz)."""
+        import torch
+        dt = torch.float64 if self.fft.precision == F64 else torch.float32
+        nx, ny, nz = self.fft.n
+        opL = torch.empty((nz, ny, nx // 2 + 1), dtype=dt, device="cuda")
+        opN = torch.empty_like(opL)
+        check(self.fft._lib.pfc_hip_pfc_operators(self.fft.handle, _ptr(opL), _ptr(opN),
+                                                  _stream_ptr(stream)))
+        return opL, opN
+
+    def autotune(self, psi, reps: int = 2, stream=None) -> dict:
+        """Time every variant of the step's passes on psi (restored afterwards)
+        and keep the fastest; returns {role: [(name, ms), ...]}."""
+        self._check_psi(psi)
+        roles = ["pfc_x", "col_y", "pfc_z", "x_c2r"]
+        names = {r: self.fft.variants(r) for r in roles}
+        total = sum(len(v) for v in names.values())
+        out = (ctypes.c_double * max(total, 1))()
+        check(self.fft._lib.pfc_hip_pfc_autotune(self.fft.handle, _ptr(psi), int(reps),
+                                                 _stream_ptr(stream), out, total))
+        res, i = {}, 0
+        for r in roles:
+            res[r] = [(n, out[i + k]) for k, n in enumerate(names[r])]
+            i += len(names[r])
+        res["chosen"] = {r: names[r][self.fft.get_variant(r)] for r in roles}
+        return res
 
     def step_timed(self, psi, nsteps: int, stream=None):
         """Run nsteps with HIP events around each of the 5 passes; returns

@@ -43,11 +43,25 @@ def _run(cmd, quiet=True):
         print(out)
 
 
+SOURCES = ["pfc_hip.hip", "variants_common.hip", "variants_f64.hip", "variants_f32.hip"]
+OBJDIR = os.path.join(HERE, "_build")
+
+
 def build_library(verbose: bool = False) -> str:
-    """Compile every HIP source into openpfc_amd/libopenpfc_hip.so."""
-    srcs = [os.path.join(CSRC, "pfc_hip.hip")]
+    """Compile every HIP source (in parallel) and link openpfc_amd/libopenpfc_hip.so."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OBJDIR, exist_ok=True)
+    objs = []
+    jobs = []
+    for src in SOURCES:
+        obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        jobs.append([hipcc(), *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)])
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 4)) as ex:
+        list(ex.map(lambda c: _run(c, quiet=not verbose), jobs))
     tmp = LIB + ".tmp"
-    _run([hipcc(), *HIP_FLAGS, "-shared", "-o", tmp, *srcs], quiet=not verbose)
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs],
+         quiet=not verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -262,8 +262,9 @@ template <typename T> struct LdsLine {
   }
 };
 
-// Row-mode padding: one pad element every 16 positions breaks the
-// power-of-two strides of the Stockham write pattern across LDS banks.
+// LDS padding: one pad position every 16 positions breaks the power-of-two
+// strides of the Stockham write pattern across LDS banks (used by rows and
+// by column tiles, whose L columns are interleaved per position).
 __host__ __device__ constexpr int row_padded(int n) { return n + n / 16; }
 
 template <bool ROWPAD>

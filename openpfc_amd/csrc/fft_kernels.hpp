@@ -44,7 +44,10 @@ template <typename T, typename Plan, int DIR, int L, bool SPLIT>
 __global__ void __launch_bounds__(L *Plan::TL)
     k_col(ColArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N;
-  __shared__ T lds[(SPLIT ? 1 : 2) * L * N];
+  // position-major, column-interleaved LDS with one pad position per 16:
+  // the stride-16 writes of a radix-16 pass then spread over all banks
+  constexpr int NP = row_padded(N);
+  __shared__ T lds[(SPLIT ? 1 : 2) * L * NP];
   const int f = blockIdx.y;
   const cplx<T> *__restrict__ src = a.src[f];
   cplx<T> *__restrict__ dst = a.dst[f];
@@ -59,8 +62,8 @@ __global__ void __launch_bounds__(L *Plan::TL)
 #pragma unroll
   for (int e = 0; e < E; ++e)
     v[e] = active ? src[base + (long)(t + TL * e) * a.s_pos] : cplx<T>{0, 0};
-  const LdsLine<T> ll{lds, c, L, L * N};
-  LineFft<T, Plan, DIR, SPLIT, false>::run(v, t, ll, a.tw);
+  const LdsLine<T> ll{lds, c, L, L * NP};
+  LineFft<T, Plan, DIR, SPLIT, true>::run(v, t, ll, a.tw);
   if (active) {
 #pragma unroll
     for (int e = 0; e < E; ++e) dst[base + (long)(t + TL * e) * a.s_pos] = v[e];
@@ -245,101 +248,128 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_pfc_x(PfcXArgs<T> a) {
   }
 }
 
+// Functions that restate reference arithmetic use `#pragma clang fp
+// contract(off)` and plain operators, so every product and sum rounds as the
+// reference's expression does (the _rn intrinsics are plain operators defined
+// outside the pragma's scope and would still be fused into FMAs).
+// ----------------------------------------------------------------------------
+// PFC operators (docs/api/examples/10_complete_pfc_model.cpp:166-213) for one
+// mode, in the reference's formula and evaluation order:
+//   k2 = kx*kx + ky*ky + kz*kz, L = -k2*(eps + (1-k2)^2),
+//   opL = exp(L*dt), opN = (opL-1)/L*(-k2)  (dt*(-k2) when |L| <= 1e-14),
+// each multiplied by `scale` (the 1/(nx*ny*nz) of the backward transform).
+// kxy = kx*kx + ky*ky is passed in so the sum keeps the reference's order.
+// ----------------------------------------------------------------------------
+struct PfcOp {
+  double fx, fy, fz; // 2*pi/(d*n) per axis
+  double eps, dt, scale;
+  int nx, ny, nz;
+};
+
+__device__ __forceinline__ double pfc_kc(int i, int n, double f) {
+  return (i <= n / 2) ? i * f : (i - n) * f;
+}
+
+__device__ __forceinline__ void pfc_op_mode(const PfcOp &o, double kxy, double kz,
+                                            double &opL, double &opN) {
+#pragma clang fp contract(off)
+  const double k2 = (kxy + (kz * kz));
+  const double k2_lap = -k2;
+  const double one_m = 1.0 - k2;
+  const double L = (k2_lap * (o.eps + (one_m * one_m)));
+  const double oL = exp((L * o.dt));
+  double oN;
+  if (fabs(L) > 1e-14)
+    oN = ((oL - 1.0) / L * k2_lap);
+  else
+    oN = (o.dt * k2_lap);
+  opL = (oL * o.scale);
+  opN = (oN * o.scale);
+}
+
+// kx*kx + ky*ky rounded as written (no contraction into an FMA).
+__device__ __forceinline__ double pfc_kxy(const PfcOp &o, int i, int j) {
+#pragma clang fp contract(off)
+  const double kx = pfc_kc(i, o.nx, o.fx), ky = pfc_kc(j, o.ny, o.fy);
+  return kx * kx + ky * ky;
+}
+
+// Operator tables in the outbox layout (pitch W), for inspection and tests.
+template <typename T>
+__global__ void k_pfc_operators(PfcOp o, int W, T *opL, T *opN) {
+#pragma clang fp contract(off)
+  const long total = (long)W * o.ny * o.nz;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % W);
+    const long jk = idx / W;
+    const int j = (int)(jk % o.ny);
+    const int k = (int)(jk / o.ny);
+    const double kxy = pfc_kxy(o, i, j);
+    double l, n;
+    pfc_op_mode(o, kxy, pfc_kc(k, o.nz, o.fz), l, n);
+    opL[idx] = (T)l;
+    opN[idx] = (T)n;
+  }
+}
+
 // ----------------------------------------------------------------------------
 // Fused PFC step, z pass: for L adjacent kx columns at one ky,
 //   A = Fz(Psi) ; B = Fz(Nl) ; A = opL*A + opN*B ; Psi = Fz^-1(A)
-// opL/opN are tables in the spectrum layout (pitch P), already multiplied by
-// the 1/(nx*ny*nz) normalisation of the backward transform.
+// with opL/opN evaluated per mode on the fly (no operator tables in HBM).
 // ----------------------------------------------------------------------------
 template <typename T> struct PfcZArgs {
   cplx<T> *A;
   const cplx<T> *B;
-  const T *opL;
-  const T *opN;
   const cplx<T> *tw;
-  long s_pos;   // P*ny
-  int pitch;    // P
-  int W;        // nx/2+1
+  PfcOp op;
+  long s_pos; // P*ny
+  int pitch;  // P
+  int W;      // nx/2+1
   int ntx;
 };
 
 template <typename T, typename Plan, int L, bool SPLIT>
 __global__ void __launch_bounds__(L *Plan::TL) k_pfc_z(PfcZArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N;
-  __shared__ T lds[(SPLIT ? 1 : 2) * L * N];
+  constexpr int NP = row_padded(N);
+  __shared__ T lds[(SPLIT ? 1 : 2) * L * NP];
   const int c = threadIdx.x % L, t = threadIdx.x / L;
   const int tx = blockIdx.x % a.ntx;
-  const long y = blockIdx.x / a.ntx;
+  const int y = blockIdx.x / a.ntx;
   const int x = tx * L + c;
   const bool active = x < a.W;
-  const long base = y * a.pitch + x;
-  const LdsLine<T> ll{lds, c, L, L * N};
+  const long base = (long)y * a.pitch + x;
+  const LdsLine<T> ll{lds, c, L, L * NP};
 
   cplx<T> acc[E];
 #pragma unroll
   for (int e = 0; e < E; ++e)
     acc[e] = active ? a.A[base + (long)(t + TL * e) * a.s_pos] : cplx<T>{0, 0};
-  LineFft<T, Plan, -1, SPLIT, false>::run(acc, t, ll, a.tw);
+  LineFft<T, Plan, -1, SPLIT, true>::run(acc, t, ll, a.tw);
+
+  const double kxy = pfc_kxy(a.op, x, y);
+  T opn[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const T l = active ? a.opL[base + (long)(t + TL * e) * a.s_pos] : (T)0;
-    acc[e] = c_scale(acc[e], l);
+    double l, n;
+    pfc_op_mode(a.op, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), l, n);
+    acc[e] = c_scale(acc[e], (T)l);
+    opn[e] = (T)n;
   }
   {
     cplx<T> v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e)
       v[e] = active ? a.B[base + (long)(t + TL * e) * a.s_pos] : cplx<T>{0, 0};
-    LineFft<T, Plan, -1, SPLIT, false>::run(v, t, ll, a.tw);
+    LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const T n = active ? a.opN[base + (long)(t + TL * e) * a.s_pos] : (T)0;
-      acc[e] = c_add(acc[e], c_scale(v[e], n));
-    }
+    for (int e = 0; e < E; ++e) acc[e] = c_add(acc[e], c_scale(v[e], opn[e]));
   }
-  LineFft<T, Plan, +1, SPLIT, false>::run(acc, t, ll, a.tw);
+  LineFft<T, Plan, +1, SPLIT, true>::run(acc, t, ll, a.tw);
   if (active) {
 #pragma unroll
     for (int e = 0; e < E; ++e) a.A[base + (long)(t + TL * e) * a.s_pos] = acc[e];
-  }
-}
-
-// ----------------------------------------------------------------------------
-// PFC operators (10_complete_pfc_model.cpp:166-213), evaluated per mode on
-// the device in the reference's formula and order, times `scale`.
-// ----------------------------------------------------------------------------
-struct PfcOpArgs {
-  double fx, fy, fz; // 2*pi/(d*n) per axis
-  double eps, dt, scale;
-  int nx, ny, nz, W, pitch;
-};
-
-template <typename T>
-__global__ void k_pfc_operators(PfcOpArgs a, T *opL, T *opN) {
-  const long total = (long)a.W * a.ny * a.nz;
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
-       idx += (long)gridDim.x * blockDim.x) {
-    const int i = (int)(idx % a.W);
-    const long jk = idx / a.W;
-    const int j = (int)(jk % a.ny);
-    const int k = (int)(jk / a.ny);
-    const double kx = (i <= a.nx / 2) ? i * a.fx : (i - a.nx) * a.fx;
-    const double ky = (j <= a.ny / 2) ? j * a.fy : (j - a.ny) * a.fy;
-    const double kz = (k <= a.nz / 2) ? k * a.fz : (k - a.nz) * a.fz;
-    const double k2 = __dadd_rn(__dadd_rn(__dmul_rn(kx, kx), __dmul_rn(ky, ky)),
-                                __dmul_rn(kz, kz));
-    const double k2_lap = -k2;
-    const double one_m = 1.0 - k2;
-    const double L = __dmul_rn(k2_lap, __dadd_rn(a.eps, __dmul_rn(one_m, one_m)));
-    const double oL = exp(__dmul_rn(L, a.dt));
-    double oN;
-    if (fabs(L) > 1e-14)
-      oN = __dmul_rn((oL - 1.0) / L, k2_lap);
-    else
-      oN = __dmul_rn(a.dt, k2_lap);
-    const long o = jk * a.pitch + i;
-    opL[o] = (T)(oL * a.scale);
-    opN[o] = (T)(oN * a.scale);
   }
 }
 
@@ -368,13 +398,14 @@ __host__ __device__ inline unsigned long long splitmix64(unsigned long long z) {
 template <typename T>
 __global__ void k_ic_add_noise(T *psi, long n, unsigned long long seed,
                                double lo, double hi) {
+#pragma clang fp contract(off)
   const unsigned long long s0 = splitmix64(seed);
   const double w = hi - lo;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x) {
     const double u =
         (double)(splitmix64(s0 + (unsigned long long)i) >> 11) * 0x1.0p-53;
-    psi[i] = (T)__dadd_rn((double)psi[i], __dadd_rn(lo, __dmul_rn(w, u)));
+    psi[i] = (T)((double)psi[i] + (lo + (w * u)));
   }
 }
 
@@ -388,6 +419,7 @@ struct SeedArgs {
 };
 
 template <typename T> __global__ void k_ic_single_seed(T *psi, SeedArgs a) {
+#pragma clang fp contract(off)
   const long n = (long)a.nx * a.ny * a.nz;
   const double s = 1.0 / sqrt(2.0);
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n;
@@ -396,30 +428,30 @@ template <typename T> __global__ void k_ic_single_seed(T *psi, SeedArgs a) {
     const long jk = idx / a.nx;
     const int j = (int)(jk % a.ny);
     const int k = (int)(jk / a.ny);
-    const double x = __dadd_rn(a.x0, __dmul_rn((double)i, a.dx));
-    const double y = __dadd_rn(a.y0, __dmul_rn((double)j, a.dy));
-    const double z = __dadd_rn(a.z0, __dmul_rn((double)k, a.dz));
+    const double x = (a.x0 + ((double)i * a.dx));
+    const double y = (a.y0 + ((double)j * a.dy));
+    const double z = (a.z0 + ((double)k * a.dz));
     const double rr =
-        __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
+        (((x * x) + (y * y)) + (z * z));
     if (rr >= a.r2) {
       if (!a.keep_outside) psi[idx] = (T)0;
       continue;
     }
-    const double sx = __dmul_rn(s, x), sy = __dmul_rn(s, y), sz = __dmul_rn(s, z);
+    const double sx = (s * x), sy = (s * y), sz = (s * z);
     // q.x for q1..q6 = (s,s,0) (s,0,s) (0,s,s) (s,0,-s) (s,-s,0) (0,s,-s),
     // summed as (qx*x + qy*y) + qz*z with the zero terms kept
     const double zero = 0.0;
     const double arg[6] = {
-        __dadd_rn(__dadd_rn(sx, sy), __dmul_rn(zero, z)),
-        __dadd_rn(__dadd_rn(sx, __dmul_rn(zero, y)), sz),
-        __dadd_rn(__dadd_rn(__dmul_rn(zero, x), sy), sz),
-        __dadd_rn(__dadd_rn(sx, __dmul_rn(zero, y)), -sz),
-        __dadd_rn(__dadd_rn(sx, -sy), __dmul_rn(zero, z)),
-        __dadd_rn(__dadd_rn(__dmul_rn(zero, x), sy), -sz)};
+        ((sx + sy) + (zero * z)),
+        ((sx + (zero * y)) + sz),
+        (((zero * x) + sy) + sz),
+        ((sx + (zero * y)) + -sz),
+        ((sx + -sy) + (zero * z)),
+        (((zero * x) + sy) + -sz)};
     double u = a.rho;
-    const double two_amp = __dmul_rn(2.0, a.amp);
+    const double two_amp = (2.0 * a.amp);
 #pragma unroll
-    for (int q = 0; q < 6; ++q) u = __dadd_rn(u, __dmul_rn(two_amp, cos(arg[q])));
+    for (int q = 0; q < 6; ++q) u = (u + (two_amp * cos(arg[q])));
     psi[idx] = (T)u;
   }
 }

@@ -1,7 +1,7 @@
 // pfc_hip.hip -- C-ABI implementation: plans, twiddle tables, size dispatch
 // and the launch sequences of the R2C/C2R transforms and the fused PFC step.
 #include "../../include/pfc_hip.h"
-#include "fft_kernels.hpp"
+#include "variants.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -10,6 +10,8 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <map>
+#include <utility>
 #include <vector>
 
 using namespace pfc_amd;
@@ -32,71 +34,6 @@ int fail(int code, const std::string &msg) {
   } while (0)
 
 // ---------------------------------------------------------------------------
-// Compile-time plans per line length.
-// ---------------------------------------------------------------------------
-template <int N> struct PlanFor;
-template <> struct PlanFor<1> { using P = FftPlan<1, 1, RadixList<1>>; };
-template <> struct PlanFor<2> { using P = FftPlan<2, 2, RadixList<2>>; };
-template <> struct PlanFor<4> { using P = FftPlan<4, 4, RadixList<4>>; };
-template <> struct PlanFor<8> { using P = FftPlan<8, 8, RadixList<8>>; };
-template <> struct PlanFor<16> { using P = FftPlan<16, 16, RadixList<16>>; };
-template <> struct PlanFor<32> { using P = FftPlan<32, 8, RadixList<8, 4>>; };
-template <> struct PlanFor<64> { using P = FftPlan<64, 8, RadixList<8, 8>>; };
-template <> struct PlanFor<128> { using P = FftPlan<128, 16, RadixList<16, 8>>; };
-template <> struct PlanFor<256> { using P = FftPlan<256, 16, RadixList<16, 16>>; };
-template <> struct PlanFor<512> { using P = FftPlan<512, 8, RadixList<8, 8, 8>>; };
-template <> struct PlanFor<1024> {
-  using P = FftPlan<1024, 16, RadixList<16, 16, 4>>;
-};
-
-// Radices of each plan, for the host twiddle builder.
-template <typename List> struct RadixVec;
-template <int... Rs> struct RadixVec<RadixList<Rs...>> {
-  static std::vector<int> get() { return {Rs...}; }
-};
-
-bool supported_len(int n) {
-  switch (n) {
-  case 1: case 2: case 4: case 8: case 16: case 32: case 64: case 128:
-  case 256: case 512: case 1024:
-    return true;
-  default:
-    return false;
-  }
-}
-
-std::vector<int> radices_of(int n) {
-  switch (n) {
-  case 1: return RadixVec<PlanFor<1>::P::List>::get();
-  case 2: return RadixVec<PlanFor<2>::P::List>::get();
-  case 4: return RadixVec<PlanFor<4>::P::List>::get();
-  case 8: return RadixVec<PlanFor<8>::P::List>::get();
-  case 16: return RadixVec<PlanFor<16>::P::List>::get();
-  case 32: return RadixVec<PlanFor<32>::P::List>::get();
-  case 64: return RadixVec<PlanFor<64>::P::List>::get();
-  case 128: return RadixVec<PlanFor<128>::P::List>::get();
-  case 256: return RadixVec<PlanFor<256>::P::List>::get();
-  case 512: return RadixVec<PlanFor<512>::P::List>::get();
-  case 1024: return RadixVec<PlanFor<1024>::P::List>::get();
-  default: return {};
-  }
-}
-
-// Tile width in columns: 128 contiguous bytes per column-tile row.
-template <typename T, typename Plan> constexpr int col_tile() {
-  constexpr int l = 128 / (2 * (int)sizeof(T));
-  return (l * Plan::TL > 1024) ? 1024 / Plan::TL : l;
-}
-// Exchange one plane at a time when a full exchange would exceed 64 KiB.
-template <typename T, typename Plan> constexpr bool col_split() {
-  return 2 * col_tile<T, Plan>() * Plan::N * (int)sizeof(T) > 65536;
-}
-// Rows per block for row kernels: one wave per block.
-template <typename Plan> constexpr int rows_per_block() {
-  return Plan::TL >= 64 ? 1 : 64 / Plan::TL;
-}
-
-// ---------------------------------------------------------------------------
 // Host twiddle tables (long double, rounded once).
 // ---------------------------------------------------------------------------
 const long double kTwoPi = 6.283185307179586476925286766559005768L;
@@ -114,11 +51,11 @@ void unit_root_ld(long m, long n, long double &c, long double &s) {
   s = sinl(a);
 }
 
-// Line twiddles for the Stockham passes: exp(-2 pi i q k/(Ns R)).
-std::vector<long double> line_twiddles(int n) {
+// Line twiddles for the Stockham passes of plan `pid`: exp(-2 pi i q k/(Ns R)).
+std::vector<long double> line_twiddles(int n, int pid) {
   std::vector<long double> out; // re, im pairs
   int ns = 1;
-  for (int r : radices_of(n)) {
+  for (int r : radices_for(n, pid)) {
     if (ns > 1) {
       for (int k = 0; k < ns; ++k)
         for (int q = 1; q < r; ++q) {
@@ -156,21 +93,31 @@ struct pfc_hip_plan_s {
   int P = 0;        // padded pitch of the plan's own spectrum buffers
   size_t esize = 8; // sizeof(real)
   int device = 0;
-  // device tables: twiddles of the nx-, nx/2-, ny-, nz-point lines and the
-  // real-FFT twiddles of nx (all in the plan's precision)
-  void *tw_xn = nullptr, *tw_xm = nullptr, *tw_y = nullptr, *tw_z = nullptr;
+  // line twiddles per (length, plan id) and real-FFT twiddles of nx
+  std::map<std::pair<int, int>, void *> tw;
   void *wn = nullptr;
   void *work = nullptr; // c2r workspace: W*ny*nz complex
   // fused-step state
-  void *A = nullptr, *B = nullptr, *opL = nullptr, *opN = nullptr;
+  void *A = nullptr, *B = nullptr;
   bool ops_ready = false;
   int model_kind = -1;
   double dt = 0;
+  PfcOp op{};
+  int var[ROLE_COUNT] = {0, 0, 0, 0, 0, 0};
   size_t bytes = 0;
 
   size_t inbox() const { return (size_t)nx * ny * nz; }
   size_t outbox() const { return (size_t)W * ny * nz; }
   size_t padded() const { return (size_t)P * ny * nz; }
+  int role_len(int role) const {
+    switch (role) {
+    case ROLE_X_R2C: case ROLE_X_C2R: return nx / 2;
+    case ROLE_COL_Y: return ny;
+    case ROLE_COL_Z: case ROLE_PFC_Z: return nz;
+    case ROLE_PFC_X: return nx;
+    default: return 0;
+    }
+  }
 };
 
 namespace {
@@ -207,9 +154,28 @@ int upload_table(pfc_hip_plan_s *p, void **dst, const std::vector<long double> &
   return 0;
 }
 
+// Twiddles of every plan of length n.
+int upload_line_tables(pfc_hip_plan_s *p, int n) {
+  for (int pid = 0; pid < plan_count(n); ++pid) {
+    auto key = std::make_pair(n, pid);
+    if (p->tw.count(key)) continue;
+    void *d = nullptr;
+    if (int rc = upload_table(p, &d, line_twiddles(n, pid), "line twiddles")) return rc;
+    p->tw[key] = d;
+  }
+  return 0;
+}
+
+template <typename T> const cplx<T> *twp(pfc_hip_plan_s *p, int n, int pid) {
+  auto it = p->tw.find(std::make_pair(n, pid));
+  return it == p->tw.end() ? nullptr : (const cplx<T> *)it->second;
+}
+
 void free_all(pfc_hip_plan_s *p) {
-  void **bufs[] = {&p->tw_xn, &p->tw_xm, &p->tw_y, &p->tw_z, &p->wn,
-                   &p->work,  &p->A,     &p->B,    &p->opL,  &p->opN};
+  for (auto &kv : p->tw)
+    if (kv.second) (void)hipFree(kv.second);
+  p->tw.clear();
+  void **bufs[] = {&p->wn, &p->work, &p->A, &p->B};
   for (void **b : bufs) {
     if (*b) (void)hipFree(*b);
     *b = nullptr;
@@ -217,62 +183,28 @@ void free_all(pfc_hip_plan_s *p) {
   p->bytes = 0;
 }
 
-// ---------------------------------------------------------------------------
-// Launchers (templated on precision and line length).
-// ---------------------------------------------------------------------------
-template <typename T, int N> struct Launch {
-  using Plan = typename PlanFor<N>::P;
-  static constexpr int L = col_tile<T, Plan>();
-  static constexpr bool SPLIT = col_split<T, Plan>();
-  static constexpr int LPB = rows_per_block<Plan>();
+template <typename T> int variant_count(pfc_hip_plan_s *p, int role) {
+  const int n = p->role_len(role);
+  switch (role) {
+  case ROLE_X_R2C: return (int)xr2c_variants<T>(n).size();
+  case ROLE_X_C2R: return (int)xc2r_variants<T>(n).size();
+  case ROLE_COL_Y: case ROLE_COL_Z: return (int)col_variants<T>(n).size();
+  case ROLE_PFC_X: return (int)pfcx_variants<T>(n).size();
+  case ROLE_PFC_Z: return (int)pfcz_variants<T>(n).size();
+  default: return 0;
+  }
+}
 
-  static hipError_t col(int dir, const ColArgs<T> &a, long n_outer, int nfields,
-                        hipStream_t s) {
-    dim3 grid((unsigned)(a.ntx * n_outer), (unsigned)nfields);
-    dim3 block(L * Plan::TL);
-    if (dir < 0)
-      hipLaunchKernelGGL((k_col<T, Plan, -1, L, SPLIT>), grid, block, 0, s, a);
-    else
-      hipLaunchKernelGGL((k_col<T, Plan, +1, L, SPLIT>), grid, block, 0, s, a);
-    return hipGetLastError();
-  }
-  static hipError_t x_r2c(const XArgs<T> &a, hipStream_t s) {
-    dim3 grid((unsigned)((a.rows + LPB - 1) / LPB)), block(LPB * Plan::TL);
-    hipLaunchKernelGGL((k_x_r2c<T, Plan, LPB>), grid, block, 0, s, a);
-    return hipGetLastError();
-  }
-  static hipError_t x_c2r(const XArgs<T> &a, hipStream_t s) {
-    dim3 grid((unsigned)((a.rows + LPB - 1) / LPB)), block(LPB * Plan::TL);
-    hipLaunchKernelGGL((k_x_c2r<T, Plan, LPB>), grid, block, 0, s, a);
-    return hipGetLastError();
-  }
-  static hipError_t pfc_x(const PfcXArgs<T> &a, hipStream_t s) {
-    dim3 grid((unsigned)((a.rows + LPB - 1) / LPB)), block(LPB * Plan::TL);
-    hipLaunchKernelGGL((k_pfc_x<T, Plan, LPB>), grid, block, 0, s, a);
-    return hipGetLastError();
-  }
-  static hipError_t pfc_z(const PfcZArgs<T> &a, long ny, hipStream_t s) {
-    dim3 grid((unsigned)(a.ntx * ny)), block(L * Plan::TL);
-    hipLaunchKernelGGL((k_pfc_z<T, Plan, L, SPLIT>), grid, block, 0, s, a);
-    return hipGetLastError();
-  }
-  static int tile() { return L; }
-};
-
-template <typename T, typename F> hipError_t with_len(int n, F &&f) {
-  switch (n) {
-  case 1: return f(Launch<T, 1>{});
-  case 2: return f(Launch<T, 2>{});
-  case 4: return f(Launch<T, 4>{});
-  case 8: return f(Launch<T, 8>{});
-  case 16: return f(Launch<T, 16>{});
-  case 32: return f(Launch<T, 32>{});
-  case 64: return f(Launch<T, 64>{});
-  case 128: return f(Launch<T, 128>{});
-  case 256: return f(Launch<T, 256>{});
-  case 512: return f(Launch<T, 512>{});
-  case 1024: return f(Launch<T, 1024>{});
-  default: return hipErrorInvalidValue;
+template <typename T> const std::string *variant_name(pfc_hip_plan_s *p, int role, int v) {
+  const int n = p->role_len(role);
+  if (v < 0 || v >= variant_count<T>(p, role)) return nullptr;
+  switch (role) {
+  case ROLE_X_R2C: return &xr2c_variants<T>(n)[v].name;
+  case ROLE_X_C2R: return &xc2r_variants<T>(n)[v].name;
+  case ROLE_COL_Y: case ROLE_COL_Z: return &col_variants<T>(n)[v].name;
+  case ROLE_PFC_X: return &pfcx_variants<T>(n)[v].name;
+  case ROLE_PFC_Z: return &pfcz_variants<T>(n)[v].name;
+  default: return nullptr;
   }
 }
 
@@ -282,35 +214,52 @@ template <typename T>
 hipError_t col_pass(pfc_hip_plan_s *p, int axis, int dir, const void *src0,
                     void *dst0, const void *src1, void *dst1, int nfields,
                     int pitch, hipStream_t s) {
+  const int role = axis == 1 ? ROLE_COL_Y : ROLE_COL_Z;
   const int n = axis == 1 ? p->ny : p->nz;
+  const auto &vs = col_variants<T>(n);
+  const auto &v = vs[p->var[role]];
   ColArgs<T> a{};
   a.src[0] = (const cplx<T> *)src0;
   a.dst[0] = (cplx<T> *)dst0;
   a.src[1] = (const cplx<T> *)src1;
   a.dst[1] = (cplx<T> *)dst1;
-  a.tw = (const cplx<T> *)(axis == 1 ? p->tw_y : p->tw_z);
+  a.tw = twp<T>(p, n, v.plan_id);
   a.W = p->W;
   a.s_pos = axis == 1 ? (long)pitch : (long)pitch * p->ny;
   a.s_outer = axis == 1 ? (long)pitch * p->ny : (long)pitch;
   const long n_outer = axis == 1 ? p->nz : p->ny;
-  return with_len<T>(n, [&](auto l) {
-    using Lx = decltype(l);
-    a.ntx = (p->W + Lx::tile() - 1) / Lx::tile();
-    return Lx::col(dir, a, n_outer, nfields, s);
-  });
+  return v.fn(dir, a, n_outer, nfields, s);
+}
+
+template <typename T>
+hipError_t x_c2r_pass(pfc_hip_plan_s *p, const void *in, void *out, int pitch,
+                      double scale, hipStream_t s) {
+  const int m = p->nx / 2;
+  const auto &v = xc2r_variants<T>(m)[p->var[ROLE_X_C2R]];
+  XArgs<T> a{};
+  a.in = (const T *)in;
+  a.out = (T *)out;
+  a.tw = twp<T>(p, m, v.plan_id);
+  a.wn = (const cplx<T> *)p->wn;
+  a.rows = (long)p->ny * p->nz;
+  a.pitch = pitch;
+  a.scale = (T)scale;
+  return v.fn(a, s);
 }
 
 template <typename T>
 int r2c_impl(pfc_hip_plan_s *p, const void *in, void *out, hipStream_t s) {
+  const int m = p->nx / 2;
+  const auto &v = xr2c_variants<T>(m)[p->var[ROLE_X_R2C]];
   XArgs<T> a{};
   a.in = (const T *)in;
   a.out = (T *)out;
-  a.tw = (const cplx<T> *)p->tw_xm;
+  a.tw = twp<T>(p, m, v.plan_id);
   a.wn = (const cplx<T> *)p->wn;
   a.rows = (long)p->ny * p->nz;
   a.pitch = p->W;
   a.scale = (T)1;
-  hipError_t e = with_len<T>(p->nx / 2, [&](auto l) { return l.x_r2c(a, s); });
+  hipError_t e = v.fn(a, s);
   PFC_HIP_TRY(e, "r2c x pass");
   if (p->ny > 1) {
     e = col_pass<T>(p, 1, -1, out, out, nullptr, nullptr, 1, p->W, s);
@@ -341,15 +290,8 @@ int c2r_impl(pfc_hip_plan_s *p, const void *in, void *out, hipStream_t s) {
     PFC_HIP_TRY(e, "c2r y pass");
     cur = p->work;
   }
-  XArgs<T> a{};
-  a.in = (const T *)cur;
-  a.out = (T *)out;
-  a.tw = (const cplx<T> *)p->tw_xm;
-  a.wn = (const cplx<T> *)p->wn;
-  a.rows = (long)p->ny * p->nz;
-  a.pitch = p->W;
-  a.scale = (T)(1.0 / ((double)p->nx * (double)p->ny * (double)p->nz));
-  e = with_len<T>(p->nx / 2, [&](auto l) { return l.x_c2r(a, s); });
+  e = x_c2r_pass<T>(p, cur, out, p->W,
+                    1.0 / ((double)p->nx * (double)p->ny * (double)p->nz), s);
   PFC_HIP_TRY(e, "c2r x pass");
   return 0;
 }
@@ -364,14 +306,15 @@ int step_impl(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
   };
   mark(0);
   {
+    const auto &v = pfcx_variants<T>(p->nx)[p->var[ROLE_PFC_X]];
     PfcXArgs<T> a{};
     a.psi = (const T *)psi;
     a.outA = (cplx<T> *)p->A;
     a.outB = (cplx<T> *)p->B;
-    a.tw = (const cplx<T> *)p->tw_xn;
+    a.tw = twp<T>(p, p->nx, v.plan_id);
     a.rows = (long)p->ny * p->nz;
     a.pitch = p->P;
-    e = with_len<T>(p->nx, [&](auto l) { return l.pfc_x(a, s); });
+    e = v.fn(a, s);
     PFC_HIP_TRY(e, "pfc forward x pass");
   }
   mark(1);
@@ -381,20 +324,16 @@ int step_impl(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
   }
   mark(2);
   {
+    const auto &v = pfcz_variants<T>(p->nz)[p->var[ROLE_PFC_Z]];
     PfcZArgs<T> a{};
     a.A = (cplx<T> *)p->A;
     a.B = (const cplx<T> *)p->B;
-    a.opL = (const T *)p->opL;
-    a.opN = (const T *)p->opN;
-    a.tw = (const cplx<T> *)p->tw_z;
+    a.tw = twp<T>(p, p->nz, v.plan_id);
+    a.op = p->op;
     a.s_pos = (long)p->P * p->ny;
     a.pitch = p->P;
     a.W = p->W;
-    e = with_len<T>(p->nz, [&](auto l) {
-      using Lx = decltype(l);
-      a.ntx = (p->W + Lx::tile() - 1) / Lx::tile();
-      return Lx::pfc_z(a, p->ny, s);
-    });
+    e = v.fn(a, p->ny, s);
     PFC_HIP_TRY(e, "pfc z pass");
   }
   mark(3);
@@ -403,18 +342,9 @@ int step_impl(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
     PFC_HIP_TRY(e, "pfc inverse y pass");
   }
   mark(4);
-  {
-    XArgs<T> a{};
-    a.in = (const T *)p->A;
-    a.out = (T *)psi;
-    a.tw = (const cplx<T> *)p->tw_xm;
-    a.wn = (const cplx<T> *)p->wn;
-    a.rows = (long)p->ny * p->nz;
-    a.pitch = p->P;
-    a.scale = (T)1; // 1/(nx ny nz) is folded into opL/opN
-    e = with_len<T>(p->nx / 2, [&](auto l) { return l.x_c2r(a, s); });
-    PFC_HIP_TRY(e, "pfc inverse x pass");
-  }
+  // 1/(nx ny nz) is folded into opL/opN
+  e = x_c2r_pass<T>(p, p->A, psi, p->P, 1.0, s);
+  PFC_HIP_TRY(e, "pfc inverse x pass");
   mark(5);
   return 0;
 }
@@ -480,11 +410,10 @@ int pfc_hip_plan_create(int nx, int ny, int nz, int precision, int flags,
     return fail(PFC_HIP_EDEVICE, "pfc_hip_plan_create: no HIP device");
   }
   int rc = 0;
-  if (!rc) rc = upload_table(p, &p->tw_xm, line_twiddles(nx / 2), "x twiddles");
-  if (!rc && supported_len(nx))
-    rc = upload_table(p, &p->tw_xn, line_twiddles(nx), "x twiddles (full)");
-  if (!rc) rc = upload_table(p, &p->tw_y, line_twiddles(ny), "y twiddles");
-  if (!rc) rc = upload_table(p, &p->tw_z, line_twiddles(nz), "z twiddles");
+  if (!rc) rc = upload_line_tables(p, nx / 2);
+  if (!rc && supported_len(nx)) rc = upload_line_tables(p, nx);
+  if (!rc) rc = upload_line_tables(p, ny);
+  if (!rc) rc = upload_line_tables(p, nz);
   if (!rc) rc = upload_table(p, &p->wn, real_twiddles(nx), "real twiddles");
   if (rc) {
     free_all(p);
@@ -558,38 +487,43 @@ int pfc_hip_pfc_set_operators(pfc_hip_plan_t plan, int model_kind,
   int rc = 0;
   if (!rc) rc = dalloc(plan, &plan->A, nc * 2 * plan->esize, "step spectrum A");
   if (!rc) rc = dalloc(plan, &plan->B, nc * 2 * plan->esize, "step spectrum B");
-  if (!rc) rc = dalloc(plan, &plan->opL, nc * plan->esize, "opL");
-  if (!rc) rc = dalloc(plan, &plan->opN, nc * plan->esize, "opN");
   if (rc) return rc;
+  // kspace.hpp:97-103: f = 2*pi/(d*n); opL/opN are evaluated per mode in the
+  // z pass, scaled by the 1/(nx*ny*nz) of the backward transform.
   const double two_pi = 6.283185307179586;
-  PfcOpArgs a{};
-  a.fx = two_pi / (dx * plan->nx);
-  a.fy = two_pi / (dy * plan->ny);
-  a.fz = two_pi / (dz * plan->nz);
-  a.eps = eps;
-  a.dt = dt;
-  a.scale = 1.0 / ((double)plan->nx * (double)plan->ny * (double)plan->nz);
-  a.nx = plan->nx;
-  a.ny = plan->ny;
-  a.nz = plan->nz;
-  a.W = plan->W;
-  a.pitch = plan->P;
-  // zero the padding columns once so they stay finite
-  PFC_HIP_TRY(hipMemset(plan->opL, 0, nc * plan->esize), "memset opL");
-  PFC_HIP_TRY(hipMemset(plan->opN, 0, nc * plan->esize), "memset opN");
-  const long total = (long)plan->outbox();
-  const unsigned blocks = (unsigned)std::min<long>((total + 255) / 256, 1L << 20);
-  if (plan->prec == PFC_HIP_F64)
-    hipLaunchKernelGGL(k_pfc_operators<double>, dim3(blocks), dim3(256), 0, 0, a,
-                       (double *)plan->opL, (double *)plan->opN);
-  else
-    hipLaunchKernelGGL(k_pfc_operators<float>, dim3(blocks), dim3(256), 0, 0, a,
-                       (float *)plan->opL, (float *)plan->opN);
-  PFC_HIP_TRY(hipGetLastError(), "operator kernel launch");
-  PFC_HIP_TRY(hipDeviceSynchronize(), "operator kernel");
+  PfcOp o{};
+  o.fx = two_pi / (dx * plan->nx);
+  o.fy = two_pi / (dy * plan->ny);
+  o.fz = two_pi / (dz * plan->nz);
+  o.eps = eps;
+  o.dt = dt;
+  o.scale = 1.0 / ((double)plan->nx * (double)plan->ny * (double)plan->nz);
+  o.nx = plan->nx;
+  o.ny = plan->ny;
+  o.nz = plan->nz;
+  plan->op = o;
   plan->model_kind = model_kind;
   plan->dt = dt;
   plan->ops_ready = true;
+  return 0;
+}
+
+int pfc_hip_pfc_operators(pfc_hip_plan_t plan, void *opL, void *opN,
+                          void *stream) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!opL || !opN) return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_operators: null buffer");
+  if (!plan->ops_ready)
+    return fail(PFC_HIP_ESTATE, "pfc_hip_pfc_operators: operators not set");
+  if (int rc = set_device(plan)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned blocks = ic_blocks((long)plan->outbox());
+  if (plan->prec == PFC_HIP_F64)
+    hipLaunchKernelGGL(k_pfc_operators<double>, dim3(blocks), dim3(256), 0, s,
+                       plan->op, plan->W, (double *)opL, (double *)opN);
+  else
+    hipLaunchKernelGGL(k_pfc_operators<float>, dim3(blocks), dim3(256), 0, s,
+                       plan->op, plan->W, (float *)opL, (float *)opN);
+  PFC_HIP_TRY(hipGetLastError(), "operator kernel launch");
   return 0;
 }
 
@@ -620,9 +554,16 @@ int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,
     return fail(PFC_HIP_ESTATE, "pfc_hip_pfc_step_timed: operators not set");
   if (int rc = set_device(plan)) return rc;
   hipStream_t s = (hipStream_t)stream;
-  std::vector<hipEvent_t> ev(6 * (size_t)nsteps);
-  for (auto &e : ev) PFC_HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+  std::vector<hipEvent_t> ev(6 * (size_t)nsteps, nullptr);
   int rc = 0;
+  for (auto &e : ev) {
+    hipError_t he = hipEventCreate(&e);
+    if (he != hipSuccess) {
+      e = nullptr;
+      rc = fail(PFC_HIP_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(he));
+      break;
+    }
+  }
   for (int i = 0; i < nsteps && !rc; ++i)
     rc = plan->prec == PFC_HIP_F64 ? step_impl<double>(plan, psi, s, &ev[6 * i])
                                    : step_impl<float>(plan, psi, s, &ev[6 * i]);
@@ -639,7 +580,8 @@ int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,
         kernel_ms[k] += ms / nsteps;
       }
   }
-  for (auto &e : ev) (void)hipEventDestroy(e);
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
   return rc;
 }
 
@@ -708,6 +650,124 @@ int pfc_hip_ic_add_noise(pfc_hip_plan_t plan, void *psi, unsigned long long seed
                        s, (float *)psi, n, seed, lo, hi);
   PFC_HIP_TRY(hipGetLastError(), "ic_add_noise launch");
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Launch variants and autotuning.
+// ---------------------------------------------------------------------------
+int pfc_hip_variant_count(pfc_hip_plan_t plan, int role, int *count) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!count || role < 0 || role >= ROLE_COUNT)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_variant_count: bad role or null count");
+  *count = plan->prec == PFC_HIP_F64 ? variant_count<double>(plan, role)
+                                     : variant_count<float>(plan, role);
+  return 0;
+}
+
+const char *pfc_hip_variant_name(pfc_hip_plan_t plan, int role, int index) {
+  if (!plan || role < 0 || role >= ROLE_COUNT) return nullptr;
+  const std::string *n = plan->prec == PFC_HIP_F64
+                             ? variant_name<double>(plan, role, index)
+                             : variant_name<float>(plan, role, index);
+  return n ? n->c_str() : nullptr;
+}
+
+int pfc_hip_variant_get(pfc_hip_plan_t plan, int role, int *index) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!index || role < 0 || role >= ROLE_COUNT)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_variant_get: bad role or null index");
+  *index = plan->var[role];
+  return 0;
+}
+
+int pfc_hip_variant_set(pfc_hip_plan_t plan, int role, int index) {
+  if (int rc = check_plan(plan)) return rc;
+  if (role < 0 || role >= ROLE_COUNT)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_variant_set: bad role");
+  int count = 0;
+  pfc_hip_variant_count(plan, role, &count);
+  if (index < 0 || index >= count)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_variant_set: index " + std::to_string(index) +
+                                    " out of range (" + std::to_string(count) +
+                                    " variants)");
+  plan->var[role] = index;
+  return 0;
+}
+
+int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
+                         double *times_ms, int max_times) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!psi || reps < 1)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_autotune: null psi or reps < 1");
+  if (!plan->ops_ready)
+    return fail(PFC_HIP_ESTATE, "pfc_hip_pfc_autotune: operators not set");
+  if (int rc = set_device(plan)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  // psi is restored at the end from the c2r workspace (outbox complex >= inbox real)
+  if (int rc = dalloc(plan, &plan->work, plan->outbox() * 2 * plan->esize, "c2r workspace"))
+    return rc;
+  const size_t rbytes = plan->inbox() * plan->esize;
+  PFC_HIP_TRY(hipMemcpyAsync(plan->work, psi, rbytes, hipMemcpyDeviceToDevice, s),
+              "autotune backup");
+  std::vector<hipEvent_t> ev(6, nullptr);
+  int rc = 0;
+  for (auto &e : ev) {
+    hipError_t he = hipEventCreate(&e);
+    if (he != hipSuccess) {
+      e = nullptr;
+      rc = fail(PFC_HIP_EDEVICE, std::string("hipEventCreate: ") + hipGetErrorString(he));
+      break;
+    }
+  }
+  // roles of the step and the event intervals their kernels occupy
+  const int roles[4] = {ROLE_PFC_X, ROLE_COL_Y, ROLE_PFC_Z, ROLE_X_C2R};
+  int slot = 0;
+  for (int ri = 0; ri < 4 && !rc; ++ri) {
+    const int role = roles[ri];
+    int count = 0;
+    pfc_hip_variant_count(plan, role, &count);
+    double best = 1e300;
+    int best_v = plan->var[role];
+    for (int v = 0; v < count && !rc; ++v) {
+      plan->var[role] = v;
+      double total = 0;
+      for (int r = 0; r <= reps && !rc; ++r) { // r == 0 warms up
+        rc = plan->prec == PFC_HIP_F64 ? step_impl<double>(plan, psi, s, ev.data())
+                                       : step_impl<float>(plan, psi, s, ev.data());
+        if (rc) break;
+        hipError_t he = hipEventSynchronize(ev[5]);
+        if (he != hipSuccess) {
+          rc = fail(PFC_HIP_EDEVICE, hipGetErrorString(he));
+          break;
+        }
+        float a = 0, b = 0;
+        if (role == ROLE_PFC_X) hipEventElapsedTime(&a, ev[0], ev[1]);
+        if (role == ROLE_COL_Y) {
+          hipEventElapsedTime(&a, ev[1], ev[2]);
+          hipEventElapsedTime(&b, ev[3], ev[4]);
+        }
+        if (role == ROLE_PFC_Z) hipEventElapsedTime(&a, ev[2], ev[3]);
+        if (role == ROLE_X_C2R) hipEventElapsedTime(&a, ev[4], ev[5]);
+        if (r > 0) total += a + b;
+      }
+      const double ms = total / reps;
+      if (times_ms && slot < max_times) times_ms[slot] = ms;
+      ++slot;
+      if (ms < best) {
+        best = ms;
+        best_v = v;
+      }
+    }
+    plan->var[role] = best_v;
+  }
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (!rc) {
+    PFC_HIP_TRY(hipMemcpyAsync(psi, plan->work, rbytes, hipMemcpyDeviceToDevice, s),
+                "autotune restore");
+    PFC_HIP_TRY(hipStreamSynchronize(s), "autotune sync");
+  }
+  return rc;
 }
 
 } // extern "C"

@@ -1,0 +1,70 @@
+// variants.hpp -- launch variants of every pass, selectable per plan.
+//
+// A variant fixes the compile-time shape of one pass: the radix plan of the
+// line transform (radices and values per thread E), the tile width L of a
+// column pass or the rows per block of a row pass, and whether the LDS
+// exchange moves both planes at once or one plane at a time.  Index 0 of
+// each list is the default; the autotuner times the others.
+#pragma once
+
+#include "fft_kernels.hpp"
+
+#include <string>
+#include <vector>
+
+namespace pfc_amd {
+
+enum Role {
+  ROLE_X_R2C = 0, // x forward of the plain R2C (nx/2-point lines)
+  ROLE_X_C2R = 1, // x inverse (nx/2-point lines), plain C2R and the step
+  ROLE_COL_Y = 2, // y column passes (plain transforms and the step)
+  ROLE_COL_Z = 3, // z column passes of the plain transforms
+  ROLE_PFC_X = 4, // step forward x (nx-point lines, psi and psi^3)
+  ROLE_PFC_Z = 5, // step z pass (forward, combine, inverse)
+  ROLE_COUNT = 6
+};
+
+template <typename T> struct ColV {
+  std::string name;
+  int plan_id; // index of the line plan (selects the twiddle table)
+  hipError_t (*fn)(int dir, ColArgs<T> a, long n_outer, int nfields,
+                   hipStream_t s);
+};
+template <typename T> struct ZV {
+  std::string name;
+  int plan_id;
+  hipError_t (*fn)(PfcZArgs<T> a, long ny, hipStream_t s);
+};
+template <typename T> struct XV {
+  std::string name;
+  int plan_id;
+  hipError_t (*fn)(XArgs<T> a, hipStream_t s);
+};
+template <typename T> struct PXV {
+  std::string name;
+  int plan_id;
+  hipError_t (*fn)(PfcXArgs<T> a, hipStream_t s);
+};
+
+// Variant lists for a line length n (empty when n is unsupported).
+template <typename T> const std::vector<ColV<T>> &col_variants(int n);
+template <typename T> const std::vector<ZV<T>> &pfcz_variants(int n);
+template <typename T> const std::vector<XV<T>> &xr2c_variants(int m);
+template <typename T> const std::vector<XV<T>> &xc2r_variants(int m);
+template <typename T> const std::vector<PXV<T>> &pfcx_variants(int n);
+#define PFC_AMD_DECLARE_LISTS(T)                                               \
+  template <> const std::vector<ColV<T>> &col_variants<T>(int);                \
+  template <> const std::vector<ZV<T>> &pfcz_variants<T>(int);                 \
+  template <> const std::vector<XV<T>> &xr2c_variants<T>(int);                 \
+  template <> const std::vector<XV<T>> &xc2r_variants<T>(int);                 \
+  template <> const std::vector<PXV<T>> &pfcx_variants<T>(int);
+PFC_AMD_DECLARE_LISTS(double)
+PFC_AMD_DECLARE_LISTS(float)
+
+// Radices of line plan `plan_id` of length n (host twiddle tables are built
+// per (length, plan) since the table layout follows the pass sequence).
+std::vector<int> radices_for(int n, int plan_id);
+int plan_count(int n);
+bool supported_len(int n);
+
+} // namespace pfc_amd

@@ -1,0 +1,6 @@
+// Variant lists and kernels for double precision.
+#include "variants_impl.hpp"
+
+namespace pfc_amd {
+PFC_AMD_DEFINE_LISTS(double)
+} // namespace pfc_amd

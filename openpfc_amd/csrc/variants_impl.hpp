@@ -1,0 +1,281 @@
+// variants_impl.hpp -- builds the variant lists (included once per precision).
+#pragma once
+
+#include "variants.hpp"
+
+#include <utility>
+
+namespace pfc_amd {
+
+template <typename... Ps> struct PlanList {};
+
+// Line plans per length; index 0 is the default.
+template <int N> struct Plans;
+template <> struct Plans<1> { using L = PlanList<FftPlan<1, 1, RadixList<1>>>; };
+template <> struct Plans<2> { using L = PlanList<FftPlan<2, 2, RadixList<2>>>; };
+template <> struct Plans<4> { using L = PlanList<FftPlan<4, 4, RadixList<4>>>; };
+template <> struct Plans<8> { using L = PlanList<FftPlan<8, 8, RadixList<8>>>; };
+template <> struct Plans<16> {
+  using L = PlanList<FftPlan<16, 16, RadixList<16>>>;
+};
+template <> struct Plans<32> { using L = PlanList<FftPlan<32, 8, RadixList<8, 4>>>; };
+template <> struct Plans<64> { using L = PlanList<FftPlan<64, 8, RadixList<8, 8>>>; };
+template <> struct Plans<128> {
+  using L = PlanList<FftPlan<128, 16, RadixList<16, 8>>, FftPlan<128, 8, RadixList<8, 4, 4>>>;
+};
+template <> struct Plans<256> {
+  using L = PlanList<FftPlan<256, 16, RadixList<16, 16>>,
+                     FftPlan<256, 8, RadixList<8, 8, 4>>,
+                     FftPlan<256, 4, RadixList<4, 4, 4, 4>>>;
+};
+template <> struct Plans<512> {
+  using L = PlanList<FftPlan<512, 8, RadixList<8, 8, 8>>,
+                     FftPlan<512, 16, RadixList<16, 16, 2>>,
+                     FftPlan<512, 4, RadixList<4, 4, 4, 4, 2>>>;
+};
+template <> struct Plans<1024> {
+  using L = PlanList<FftPlan<1024, 16, RadixList<16, 16, 4>>,
+                     FftPlan<1024, 8, RadixList<8, 8, 4, 4>>,
+                     FftPlan<1024, 4, RadixList<4, 4, 4, 4, 4>>>;
+};
+
+template <typename List> struct RadixOf;
+template <int... Rs> struct RadixOf<RadixList<Rs...>> {
+  static std::vector<int> get() { return {Rs...}; }
+};
+
+template <typename Plan> std::string plan_name() {
+  std::string s = "r";
+  auto r = RadixOf<typename Plan::List>::get();
+  for (size_t i = 0; i < r.size(); ++i) s += (i ? "x" : "") + std::to_string(r[i]);
+  return s + "/E" + std::to_string(Plan::E);
+}
+
+constexpr int kMaxLds = 147456; // bytes of static LDS a block may use
+constexpr int kMaxThreads = 1024;
+
+// ---- launchers ---------------------------------------------------------------
+template <typename T, typename Plan, int L, bool S>
+hipError_t col_launch(int dir, ColArgs<T> a, long n_outer, int nfields,
+                      hipStream_t s) {
+  a.ntx = (a.W + L - 1) / L;
+  dim3 grid((unsigned)(a.ntx * n_outer), (unsigned)nfields), block(L * Plan::TL);
+  if (dir < 0)
+    hipLaunchKernelGGL((k_col<T, Plan, -1, L, S>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_col<T, Plan, +1, L, S>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T, typename Plan, int L, bool S>
+hipError_t pfcz_launch(PfcZArgs<T> a, long ny, hipStream_t s) {
+  a.ntx = (a.W + L - 1) / L;
+  dim3 grid((unsigned)(a.ntx * ny)), block(L * Plan::TL);
+  hipLaunchKernelGGL((k_pfc_z<T, Plan, L, S>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T, typename Plan, int LPB>
+hipError_t xr2c_launch(XArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)((a.rows + LPB - 1) / LPB)), block(LPB * Plan::TL);
+  hipLaunchKernelGGL((k_x_r2c<T, Plan, LPB>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T, typename Plan, int LPB>
+hipError_t xc2r_launch(XArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)((a.rows + LPB - 1) / LPB)), block(LPB * Plan::TL);
+  hipLaunchKernelGGL((k_x_c2r<T, Plan, LPB>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T, typename Plan, int LPB>
+hipError_t pfcx_launch(PfcXArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)((a.rows + LPB - 1) / LPB)), block(LPB * Plan::TL);
+  hipLaunchKernelGGL((k_pfc_x<T, Plan, LPB>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+// ---- list builders ------------------------------------------------------------
+// Column-shaped passes: tile widths 8, 4, 16 (128/64/256-byte rows for
+// double complex), both exchange modes, kept when the block fits.
+template <typename T, typename Plan, int L, bool S> constexpr bool col_fits() {
+  return L * Plan::TL <= kMaxThreads && L * Plan::TL >= 1 &&
+         (S ? 1 : 2) * L * row_padded(Plan::N) * (int)sizeof(T) <= kMaxLds;
+}
+
+template <typename T, typename Plan, int L, bool S>
+void add_col(std::vector<ColV<T>> &v, int pid) {
+  if constexpr (col_fits<T, Plan, L, S>())
+    v.push_back({plan_name<Plan>() + "/L" + std::to_string(L) + (S ? "/split" : "/full"),
+                 pid, &col_launch<T, Plan, L, S>});
+}
+template <typename T, typename Plan, int L, bool S>
+void add_z(std::vector<ZV<T>> &v, int pid) {
+  if constexpr (col_fits<T, Plan, L, S>())
+    v.push_back({plan_name<Plan>() + "/L" + std::to_string(L) + (S ? "/split" : "/full"),
+                 pid, &pfcz_launch<T, Plan, L, S>});
+}
+
+// Default first: 128-byte tile rows, exchange mode by LDS budget.
+template <typename T, typename Plan> constexpr int default_tile() {
+  constexpr int l = 128 / (2 * (int)sizeof(T));
+  return (l * Plan::TL > kMaxThreads) ? kMaxThreads / Plan::TL : l;
+}
+template <typename T, typename Plan> constexpr bool default_split() {
+  return 2 * default_tile<T, Plan>() * row_padded(Plan::N) * (int)sizeof(T) > 65536;
+}
+
+template <typename T, typename Plan, bool ALL>
+void col_for_plan(std::vector<ColV<T>> &v, int pid) {
+  constexpr int L0 = default_tile<T, Plan>();
+  constexpr bool S0 = default_split<T, Plan>();
+  if (pid == 0) add_col<T, Plan, L0, S0>(v, pid);
+  if constexpr (ALL) {
+    if (pid != 0 || L0 != 8 || S0 != true) add_col<T, Plan, 8, true>(v, pid);
+    if (pid != 0 || L0 != 8 || S0 != false) add_col<T, Plan, 8, false>(v, pid);
+    if (pid != 0 || L0 != 4 || S0 != true) add_col<T, Plan, 4, true>(v, pid);
+    if (pid != 0 || L0 != 4 || S0 != false) add_col<T, Plan, 4, false>(v, pid);
+    if (pid != 0 || L0 != 16 || S0 != true) add_col<T, Plan, 16, true>(v, pid);
+    if (pid != 0 || L0 != 16 || S0 != false) add_col<T, Plan, 16, false>(v, pid);
+  }
+}
+template <typename T, typename Plan, bool ALL>
+void z_for_plan(std::vector<ZV<T>> &v, int pid) {
+  constexpr int L0 = default_tile<T, Plan>();
+  constexpr bool S0 = default_split<T, Plan>();
+  if (pid == 0) add_z<T, Plan, L0, S0>(v, pid);
+  if constexpr (ALL) {
+    if (pid != 0 || L0 != 8 || S0 != true) add_z<T, Plan, 8, true>(v, pid);
+    if (pid != 0 || L0 != 8 || S0 != false) add_z<T, Plan, 8, false>(v, pid);
+    if (pid != 0 || L0 != 4 || S0 != true) add_z<T, Plan, 4, true>(v, pid);
+    if (pid != 0 || L0 != 4 || S0 != false) add_z<T, Plan, 4, false>(v, pid);
+    if (pid != 0 || L0 != 16 || S0 != true) add_z<T, Plan, 16, true>(v, pid);
+    if (pid != 0 || L0 != 16 || S0 != false) add_z<T, Plan, 16, false>(v, pid);
+  }
+}
+
+// Row-shaped passes: 1, 2 or 4 rows per block (or enough rows to fill a wave).
+template <typename Plan> constexpr int min_rows() {
+  return Plan::TL >= 64 ? 1 : 64 / Plan::TL;
+}
+template <typename T, typename Plan, int LPB> constexpr bool row_fits() {
+  return LPB * Plan::TL <= kMaxThreads &&
+         2 * LPB * row_padded(Plan::N) * (int)sizeof(T) <= kMaxLds;
+}
+template <typename T, typename Plan, bool ALL, typename V, typename Make>
+void rows_for_plan(std::vector<V> &v, int pid, Make make) {
+  constexpr int R0 = min_rows<Plan>();
+  make(std::integral_constant<int, R0>{});
+  if constexpr (ALL) {
+    if constexpr (row_fits<T, Plan, 2 * R0>()) make(std::integral_constant<int, 2 * R0>{});
+    if constexpr (row_fits<T, Plan, 4 * R0>()) make(std::integral_constant<int, 4 * R0>{});
+  }
+  (void)pid;
+}
+
+template <typename T, int N, typename F, typename... Ps>
+void for_plans(PlanList<Ps...>, F &&f) {
+  int pid = 0;
+  ((f(Ps{}, pid++)), ...);
+}
+
+// Lengths with several plans and tile shapes get the full sweep; the rest a
+// single default variant.
+template <int N> constexpr bool sweep_len() { return N >= 128; }
+
+template <typename T, int N> std::vector<ColV<T>> make_col() {
+  std::vector<ColV<T>> v;
+  for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+    col_for_plan<T, decltype(p), sweep_len<N>()>(v, pid);
+  });
+  return v;
+}
+template <typename T, int N> std::vector<ZV<T>> make_z() {
+  std::vector<ZV<T>> v;
+  for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+    z_for_plan<T, decltype(p), sweep_len<N>()>(v, pid);
+  });
+  return v;
+}
+template <typename T, int N> std::vector<XV<T>> make_xr2c() {
+  std::vector<XV<T>> v;
+  for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+    using P = decltype(p);
+    rows_for_plan<T, P, sweep_len<N>()>(v, pid, [&](auto r) {
+      constexpr int R = decltype(r)::value;
+      v.push_back({plan_name<P>() + "/rows" + std::to_string(R), pid,
+                   &xr2c_launch<T, P, R>});
+    });
+  });
+  return v;
+}
+template <typename T, int N> std::vector<XV<T>> make_xc2r() {
+  std::vector<XV<T>> v;
+  for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+    using P = decltype(p);
+    rows_for_plan<T, P, sweep_len<N>()>(v, pid, [&](auto r) {
+      constexpr int R = decltype(r)::value;
+      v.push_back({plan_name<P>() + "/rows" + std::to_string(R), pid,
+                   &xc2r_launch<T, P, R>});
+    });
+  });
+  return v;
+}
+template <typename T, int N> std::vector<PXV<T>> make_pfcx() {
+  std::vector<PXV<T>> v;
+  for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+    using P = decltype(p);
+    rows_for_plan<T, P, sweep_len<N>()>(v, pid, [&](auto r) {
+      constexpr int R = decltype(r)::value;
+      v.push_back({plan_name<P>() + "/rows" + std::to_string(R), pid,
+                   &pfcx_launch<T, P, R>});
+    });
+  });
+  return v;
+}
+
+#define PFC_AMD_SWITCH_LEN(n, ...)                                            \
+  switch (n) {                                                                 \
+  case 1: { constexpr int N_ = 1; __VA_ARGS__ }                                      \
+  case 2: { constexpr int N_ = 2; __VA_ARGS__ }                                      \
+  case 4: { constexpr int N_ = 4; __VA_ARGS__ }                                      \
+  case 8: { constexpr int N_ = 8; __VA_ARGS__ }                                      \
+  case 16: { constexpr int N_ = 16; __VA_ARGS__ }                                    \
+  case 32: { constexpr int N_ = 32; __VA_ARGS__ }                                    \
+  case 64: { constexpr int N_ = 64; __VA_ARGS__ }                                    \
+  case 128: { constexpr int N_ = 128; __VA_ARGS__ }                                  \
+  case 256: { constexpr int N_ = 256; __VA_ARGS__ }                                  \
+  case 512: { constexpr int N_ = 512; __VA_ARGS__ }                                  \
+  case 1024: { constexpr int N_ = 1024; __VA_ARGS__ }                                \
+  default: break;                                                              \
+  }
+
+#define PFC_AMD_DEFINE_LISTS(T)                                                \
+  template <> const std::vector<ColV<T>> &col_variants<T>(int n) {             \
+    static const std::vector<ColV<T>> empty;                                   \
+    PFC_AMD_SWITCH_LEN(n, static const auto v = make_col<T, N_>(); return v;)  \
+    return empty;                                                              \
+  }                                                                            \
+  template <> const std::vector<ZV<T>> &pfcz_variants<T>(int n) {              \
+    static const std::vector<ZV<T>> empty;                                     \
+    PFC_AMD_SWITCH_LEN(n, static const auto v = make_z<T, N_>(); return v;)    \
+    return empty;                                                              \
+  }                                                                            \
+  template <> const std::vector<XV<T>> &xr2c_variants<T>(int n) {              \
+    static const std::vector<XV<T>> empty;                                     \
+    PFC_AMD_SWITCH_LEN(n, static const auto v = make_xr2c<T, N_>(); return v;) \
+    return empty;                                                              \
+  }                                                                            \
+  template <> const std::vector<XV<T>> &xc2r_variants<T>(int n) {              \
+    static const std::vector<XV<T>> empty;                                     \
+    PFC_AMD_SWITCH_LEN(n, static const auto v = make_xc2r<T, N_>(); return v;) \
+    return empty;                                                              \
+  }                                                                            \
+  template <> const std::vector<PXV<T>> &pfcx_variants<T>(int n) {             \
+    static const std::vector<PXV<T>> empty;                                    \
+    PFC_AMD_SWITCH_LEN(n, static const auto v = make_pfcx<T, N_>(); return v;) \
+    return empty;                                                              \
+  }
+
+} // namespace pfc_amd
