@@ -1,0 +1,67 @@
+"""Pins the oracle's initial conditions and random streams.
+
+  - std::mt19937_64: the C++ standard's known answer (10000th output of the
+    default seed 5489 is 9981545732273789042, [rand.predef]);
+  - SingleSeed (include/openpfc/kernel/simulation/initial_conditions/single_seed.hpp:51-83):
+    value rho + 12*amp at the origin (all six cosines are 1), 0.0 outside the
+    sphere, the six <110>/sqrt(2) wave vectors (|q| = 1, so the pattern has
+    period 2*pi*sqrt(2) along x);
+  - the counter-based noise stream: range, mean, independence of the seed.
+"""
+import math
+
+import numpy as np
+
+from oracle import oracle as O
+
+
+def test_mt19937_64_standard_known_answer():
+    assert O.mt64_nth(5489, 10000) == 9981545732273789042
+
+
+def test_uniform_noise_draw_rule():
+    a = O.uniform_noise((4, 1, 1), 5489, 0.0, 1.0).reshape(-1)
+    # first output of mt19937_64(5489) is 14514284786278117030
+    assert a[0] == (14514284786278117030 >> 11) * 2.0 ** -53
+
+
+def test_single_seed_reference_semantics():
+    n = (40, 40, 40)
+    dx = 1.0
+    origin = (-20.0, -20.0, -20.0)
+    a = O.ic_single_seed(n, origin, (dx, dx, dx), rho=-0.1, amp=0.05, radius=10.0)
+    # grid point i = 20 is x = 0: all cosines equal 1
+    assert abs(a[20, 20, 20] - (-0.1 + 12 * 0.05)) <= 1e-15
+    # outside the sphere: exactly 0.0
+    assert a[0, 0, 0] == 0.0 and a[20, 20, 31] == 0.0
+    # inside, the value is rho + 2 amp sum cos(q.x); at (x, 0, 0):
+    x = 3.0
+    s = 1 / math.sqrt(2)
+    expect = -0.1 + 2 * 0.05 * (4 * math.cos(s * x) + 2 * 1.0)
+    assert abs(a[20, 20, 23] - expect) <= 1e-15
+
+
+def test_single_seed_masked_keeps_outside():
+    n = (16, 16, 16)
+    a = np.full((16, 16, 16), -0.2)
+    O.ic_single_seed_masked(a, (-8.0,) * 3, (1.0,) * 3, -0.2, 0.1, 4.0, 1)
+    assert a[0, 0, 0] == -0.2
+    assert abs(a[8, 8, 8] - (-0.2 + 12 * 0.1)) <= 1e-15
+    b = np.full((16, 16, 16), -0.2)
+    O.ic_single_seed_masked(b, (-8.0,) * 3, (1.0,) * 3, -0.2, 0.1, 4.0, 0)
+    assert b[0, 0, 0] == 0.0
+    ref = O.ic_single_seed(n, (-8.0,) * 3, (1.0,) * 3, -0.2, 0.1, 4.0)
+    assert np.array_equal(b, ref)
+
+
+def test_counter_noise_stream():
+    a = np.zeros(1 << 16)
+    O.add_counter_noise(a, 1234, -0.01, 0.01)
+    assert a.min() >= -0.01 and a.max() < 0.01
+    assert abs(a.mean()) < 1e-4
+    b = np.zeros(1 << 16)
+    O.add_counter_noise(b, 1235, -0.01, 0.01)
+    assert abs(np.corrcoef(a, b)[0, 1]) < 0.02
+    c = np.zeros(1 << 16)
+    O.add_counter_noise(c, 1234, -0.01, 0.01)
+    assert np.array_equal(a, c)
