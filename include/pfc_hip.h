@@ -165,9 +165,12 @@ int pfc_hip_pfc_operators(pfc_hip_plan_t plan, void *opL, void *opN, void *strea
 /*
  * Launch variants.  Every pass of the transforms and of the step exists in
  * several compile-time shapes (radix plan, values per thread, column tile
- * width or rows per block, LDS exchange mode).  Roles:
- *   0 x forward (plain R2C)  1 x inverse (C2R and step)  2 y columns
- *   3 z columns (plain transforms)  4 step forward x  5 step z pass
+ * width, rows or lines per block, LDS exchange mode), and the step has two
+ * spectral layouts.  Roles:
+ *   0 x forward (plain R2C)  1 x inverse (C2R and natural step)  2 y columns
+ *   3 z columns (plain transforms)  4 natural step forward x
+ *   5 natural step z pass  6 step layout (0 transposed T1/T2, 1 natural)
+ *   7..11 transposed step: forward x, forward y, z pass, inverse y, inverse x
  * Index 0 is the default of each role.  No reference counterpart: the
  * reference delegates shapes to rocFFT; its GPU autotuner only times 1-D
  * block sizes (include/openpfc/runtime/common/gpu_autotune.hpp:295-333).
@@ -179,20 +182,30 @@ enum {
   PFC_HIP_ROLE_COL_Z = 3,
   PFC_HIP_ROLE_PFC_X = 4,
   PFC_HIP_ROLE_PFC_Z = 5,
-  PFC_HIP_ROLE_COUNT = 6
+  PFC_HIP_ROLE_STEP_LAYOUT = 6,
+  PFC_HIP_ROLE_T_X_FWD = 7,
+  PFC_HIP_ROLE_T_Y_FWD = 8,
+  PFC_HIP_ROLE_T_Z = 9,
+  PFC_HIP_ROLE_T_Y_INV = 10,
+  PFC_HIP_ROLE_T_X_INV = 11,
+  PFC_HIP_ROLE_COUNT = 12
 };
 int pfc_hip_variant_count(pfc_hip_plan_t plan, int role, int *count);
 /* Name of a variant ("r16x16x4/E16/L8/split"), NULL if out of range. */
 const char *pfc_hip_variant_name(pfc_hip_plan_t plan, int role, int index);
 int pfc_hip_variant_get(pfc_hip_plan_t plan, int role, int *index);
+/* Selecting a step layout releases the other layout's step buffers. */
 int pfc_hip_variant_set(pfc_hip_plan_t plan, int role, int index);
 
 /*
- * Time every variant of the step's roles (forward x, y columns, z pass,
- * inverse x) on full steps of psi, `reps` timed steps each after one warm-up,
- * and keep the fastest per role.  psi is restored before return (the stream
- * is synchronised).  times_ms (may be NULL) receives the mean pass time of
- * every variant tried, roles in the order above, up to max_times entries.
+ * Time every variant of every step role on full steps of psi (`reps` timed
+ * steps after one warm-up each), keep the fastest per role, then keep the
+ * faster step layout.  psi is restored before return (the stream is
+ * synchronised).  times_ms (may be NULL) receives the mean pass time of each
+ * variant tried, roles in the order 4, 2, 5, 1, 7, 8, 9, 10, 11 (roles of an
+ * unsupported layout contribute no entries), followed by the step time of
+ * the transposed and of the natural layout (0 if not run); up to max_times
+ * entries are written.
  */
 int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
                          double *times_ms, int max_times);

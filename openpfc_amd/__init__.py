@@ -91,7 +91,9 @@ class HipFft:
         return self._fft_time
 
     # -- launch variants ------------------------------------------------------
-    ROLES = {"x_r2c": 0, "x_c2r": 1, "col_y": 2, "col_z": 3, "pfc_x": 4, "pfc_z": 5}
+    ROLES = {"x_r2c": 0, "x_c2r": 1, "col_y": 2, "col_z": 3, "pfc_x": 4, "pfc_z": 5,
+             "step_layout": 6, "t_x_fwd": 7, "t_y_fwd": 8, "t_z": 9, "t_y_inv": 10,
+             "t_x_inv": 11}
 
     def variants(self, role) -> list:
         r = self.ROLES.get(role, role)
@@ -223,20 +225,28 @@ class PfcStepper:
         return opL, opN
 
     def autotune(self, psi, reps: int = 2, stream=None) -> dict:
-        """Time every variant of the step's passes on psi (restored afterwards)
-        and keep the fastest; returns {role: [(name, ms), ...]}."""
+        """Time every variant of the step's passes on psi (restored afterwards),
+        keep the fastest per pass and the faster layout; returns
+        {role: [(name, ms), ...], "layout_ms": {...}, "chosen": {...}}."""
         self._check_psi(psi)
-        roles = ["pfc_x", "col_y", "pfc_z", "x_c2r"]
+        roles = ["pfc_x", "col_y", "pfc_z", "x_c2r", "t_x_fwd", "t_y_fwd", "t_z",
+                 "t_y_inv", "t_x_inv"]
+        layouts = self.fft.variants("step_layout")
+        if "transposed" not in layouts:
+            roles = roles[:4]
         names = {r: self.fft.variants(r) for r in roles}
-        total = sum(len(v) for v in names.values())
-        out = (ctypes.c_double * max(total, 1))()
+        total = sum(len(v) for v in names.values()) + 2
+        out = (ctypes.c_double * total)()
         check(self.fft._lib.pfc_hip_pfc_autotune(self.fft.handle, _ptr(psi), int(reps),
                                                  _stream_ptr(stream), out, total))
         res, i = {}, 0
         for r in roles:
             res[r] = [(n, out[i + k]) for k, n in enumerate(names[r])]
             i += len(names[r])
+        res["layout_ms"] = {"transposed": out[i], "natural": out[i + 1]}
         res["chosen"] = {r: names[r][self.fft.get_variant(r)] for r in roles}
+        res["chosen"]["step_layout"] = self.fft.variants("step_layout")[
+            self.fft.get_variant("step_layout")]
         return res
 
     def step_timed(self, psi, nsteps: int, stream=None):

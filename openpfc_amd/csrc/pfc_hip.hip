@@ -97,13 +97,15 @@ struct pfc_hip_plan_s {
   std::map<std::pair<int, int>, void *> tw;
   void *wn = nullptr;
   void *work = nullptr; // c2r workspace: W*ny*nz complex
-  // fused-step state
+  // fused-step state: natural layout (A, B pitched by P) or transposed
+  // layouts T1 (A1, B1) and T2 (A2, B2), W*ny*nz complex each
   void *A = nullptr, *B = nullptr;
+  void *A1 = nullptr, *B1 = nullptr, *A2 = nullptr, *B2 = nullptr;
   bool ops_ready = false;
   int model_kind = -1;
   double dt = 0;
   PfcOp op{};
-  int var[ROLE_COUNT] = {0, 0, 0, 0, 0, 0};
+  int var[ROLE_COUNT] = {};
   size_t bytes = 0;
 
   size_t inbox() const { return (size_t)nx * ny * nz; }
@@ -115,9 +117,15 @@ struct pfc_hip_plan_s {
     case ROLE_COL_Y: return ny;
     case ROLE_COL_Z: case ROLE_PFC_Z: return nz;
     case ROLE_PFC_X: return nx;
+    case ROLE_T_X_FWD: return nx;
+    case ROLE_T_Y_FWD: case ROLE_T_Y_INV: return ny;
+    case ROLE_T_Z: return nz;
+    case ROLE_T_X_INV: return nx / 2;
     default: return 0;
     }
   }
+  bool transposed() const { return var[ROLE_STEP_LAYOUT] == 0 && t_ok; }
+  bool t_ok = false; // transposed step supported for these extents
 };
 
 namespace {
@@ -175,7 +183,7 @@ void free_all(pfc_hip_plan_s *p) {
   for (auto &kv : p->tw)
     if (kv.second) (void)hipFree(kv.second);
   p->tw.clear();
-  void **bufs[] = {&p->wn, &p->work, &p->A, &p->B};
+  void **bufs[] = {&p->wn, &p->work, &p->A, &p->B, &p->A1, &p->B1, &p->A2, &p->B2};
   for (void **b : bufs) {
     if (*b) (void)hipFree(*b);
     *b = nullptr;
@@ -185,6 +193,9 @@ void free_all(pfc_hip_plan_s *p) {
 
 template <typename T> int variant_count(pfc_hip_plan_s *p, int role) {
   const int n = p->role_len(role);
+  if (role == ROLE_STEP_LAYOUT) return p->t_ok ? 2 : 1;
+  if (role >= ROLE_T_X_FWD && role <= ROLE_T_X_INV)
+    return (int)t_variants<T>(role - ROLE_T_X_FWD, n).size();
   switch (role) {
   case ROLE_X_R2C: return (int)xr2c_variants<T>(n).size();
   case ROLE_X_C2R: return (int)xc2r_variants<T>(n).size();
@@ -196,8 +207,12 @@ template <typename T> int variant_count(pfc_hip_plan_s *p, int role) {
 }
 
 template <typename T> const std::string *variant_name(pfc_hip_plan_s *p, int role, int v) {
+  static const std::string layouts[2] = {"transposed", "natural"};
   const int n = p->role_len(role);
   if (v < 0 || v >= variant_count<T>(p, role)) return nullptr;
+  if (role == ROLE_STEP_LAYOUT) return &layouts[p->t_ok ? v : 1];
+  if (role >= ROLE_T_X_FWD && role <= ROLE_T_X_INV)
+    return &t_variants<T>(role - ROLE_T_X_FWD, n)[v].name;
   switch (role) {
   case ROLE_X_R2C: return &xr2c_variants<T>(n)[v].name;
   case ROLE_X_C2R: return &xc2r_variants<T>(n)[v].name;
@@ -299,7 +314,7 @@ int c2r_impl(pfc_hip_plan_s *p, const void *in, void *out, hipStream_t s) {
 // One fused PFC step: 5 launches.  `ev` (optional) receives 6 events
 // bracketing the 5 passes.
 template <typename T>
-int step_impl(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
+int step_natural(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
   hipError_t e;
   auto mark = [&](int i) {
     if (ev) (void)hipEventRecord(ev[i], s);
@@ -347,6 +362,63 @@ int step_impl(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
   PFC_HIP_TRY(e, "pfc inverse x pass");
   mark(5);
   return 0;
+}
+
+// Step buffers of the active layout (allocated on first use).
+int step_buffers(pfc_hip_plan_s *p) {
+  int rc = 0;
+  if (p->transposed()) {
+    const size_t c = p->outbox() * 2 * p->esize;
+    if (!rc) rc = dalloc(p, &p->A1, c, "step T1 psi");
+    if (!rc) rc = dalloc(p, &p->B1, c, "step T1 N");
+    if (!rc) rc = dalloc(p, &p->A2, c, "step T2 psi");
+    if (!rc) rc = dalloc(p, &p->B2, c, "step T2 N");
+  } else {
+    const size_t c = p->padded() * 2 * p->esize;
+    if (!rc) rc = dalloc(p, &p->A, c, "step spectrum A");
+    if (!rc) rc = dalloc(p, &p->B, c, "step spectrum B");
+  }
+  return rc;
+}
+
+template <typename T>
+int step_transposed(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], s);
+  };
+  TStepArgs<T> a{};
+  a.psi_in = (const T *)psi;
+  a.psi_out = (T *)psi;
+  a.A1 = (cplx<T> *)p->A1;
+  a.B1 = (cplx<T> *)p->B1;
+  a.A2 = (cplx<T> *)p->A2;
+  a.B2 = (cplx<T> *)p->B2;
+  a.wn = (const cplx<T> *)p->wn;
+  a.op = p->op;
+  a.nx = p->nx;
+  a.ny = p->ny;
+  a.nz = p->nz;
+  a.W = p->W;
+  const int roles[5] = {ROLE_T_X_FWD, ROLE_T_Y_FWD, ROLE_T_Z, ROLE_T_Y_INV, ROLE_T_X_INV};
+  const char *what[5] = {"step x forward", "step y forward", "step z pass",
+                         "step y inverse", "step x inverse"};
+  mark(0);
+  for (int k = 0; k < 5; ++k) {
+    const int n = p->role_len(roles[k]);
+    const auto &v = t_variants<T>(k, n)[p->var[roles[k]]];
+    const cplx<T> *tw = twp<T>(p, n, v.plan_id);
+    a.tw_x = a.tw_xm = a.tw_y = a.tw_z = tw;
+    PFC_HIP_TRY(v.fn(a, s), what[k]);
+    mark(k + 1);
+  }
+  return 0;
+}
+
+template <typename T>
+int step_impl(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
+  if (int rc = step_buffers(p)) return rc;
+  return p->transposed() ? step_transposed<T>(p, psi, s, ev)
+                         : step_natural<T>(p, psi, s, ev);
 }
 
 unsigned ic_blocks(long n) {
@@ -415,6 +487,15 @@ int pfc_hip_plan_create(int nx, int ny, int nz, int precision, int flags,
   if (!rc) rc = upload_line_tables(p, ny);
   if (!rc) rc = upload_line_tables(p, nz);
   if (!rc) rc = upload_table(p, &p->wn, real_twiddles(nx), "real twiddles");
+  if (!rc) {
+    bool ok = true;
+    for (int r = ROLE_T_X_FWD; r <= ROLE_T_X_INV; ++r) {
+      int c = precision == PFC_HIP_F64 ? variant_count<double>(p, r)
+                                       : variant_count<float>(p, r);
+      ok = ok && c > 0;
+    }
+    p->t_ok = ok;
+  }
   if (rc) {
     free_all(p);
     delete p;
@@ -483,11 +564,7 @@ int pfc_hip_pfc_set_operators(pfc_hip_plan_t plan, int model_kind,
       !std::isfinite(dt))
     return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_set_operators: bad spacing/eps/dt");
   if (int rc = set_device(plan)) return rc;
-  const size_t nc = plan->padded();
-  int rc = 0;
-  if (!rc) rc = dalloc(plan, &plan->A, nc * 2 * plan->esize, "step spectrum A");
-  if (!rc) rc = dalloc(plan, &plan->B, nc * 2 * plan->esize, "step spectrum B");
-  if (rc) return rc;
+  if (int rc = step_buffers(plan)) return rc;
   // kspace.hpp:97-103: f = 2*pi/(d*n); opL/opN are evaluated per mode in the
   // z pass, scaled by the 1/(nx*ny*nz) of the backward transform.
   const double two_pi = 6.283185307179586;
@@ -690,7 +767,23 @@ int pfc_hip_variant_set(pfc_hip_plan_t plan, int role, int index) {
     return fail(PFC_HIP_EINVAL, "pfc_hip_variant_set: index " + std::to_string(index) +
                                     " out of range (" + std::to_string(count) +
                                     " variants)");
+  if (role == ROLE_STEP_LAYOUT && index != plan->var[role]) {
+    if (int rc = set_device(plan)) return rc;
+    (void)hipDeviceSynchronize();
+    void **drop[4] = {&plan->A1, &plan->B1, &plan->A2, &plan->B2};
+    void **drop_nat[2] = {&plan->A, &plan->B};
+    const size_t tb = plan->outbox() * 2 * plan->esize;
+    const size_t nb = plan->padded() * 2 * plan->esize;
+    if (index == 1) {
+      for (void **b : drop)
+        if (*b) { (void)hipFree(*b); *b = nullptr; plan->bytes -= tb; }
+    } else {
+      for (void **b : drop_nat)
+        if (*b) { (void)hipFree(*b); *b = nullptr; plan->bytes -= nb; }
+    }
+  }
   plan->var[role] = index;
+  if (role == ROLE_STEP_LAYOUT && plan->ops_ready) return step_buffers(plan);
   return 0;
 }
 
@@ -719,46 +812,71 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
       break;
     }
   }
-  // roles of the step and the event intervals their kernels occupy
-  const int roles[4] = {ROLE_PFC_X, ROLE_COL_Y, ROLE_PFC_Z, ROLE_X_C2R};
-  int slot = 0;
-  for (int ri = 0; ri < 4 && !rc; ++ri) {
-    const int role = roles[ri];
-    int count = 0;
-    pfc_hip_variant_count(plan, role, &count);
-    double best = 1e300;
-    int best_v = plan->var[role];
-    for (int v = 0; v < count && !rc; ++v) {
-      plan->var[role] = v;
-      double total = 0;
-      for (int r = 0; r <= reps && !rc; ++r) { // r == 0 warms up
-        rc = plan->prec == PFC_HIP_F64 ? step_impl<double>(plan, psi, s, ev.data())
-                                       : step_impl<float>(plan, psi, s, ev.data());
-        if (rc) break;
-        hipError_t he = hipEventSynchronize(ev[5]);
-        if (he != hipSuccess) {
-          rc = fail(PFC_HIP_EDEVICE, hipGetErrorString(he));
-          break;
-        }
-        float a = 0, b = 0;
-        if (role == ROLE_PFC_X) hipEventElapsedTime(&a, ev[0], ev[1]);
-        if (role == ROLE_COL_Y) {
-          hipEventElapsedTime(&a, ev[1], ev[2]);
-          hipEventElapsedTime(&b, ev[3], ev[4]);
-        }
-        if (role == ROLE_PFC_Z) hipEventElapsedTime(&a, ev[2], ev[3]);
-        if (role == ROLE_X_C2R) hipEventElapsedTime(&a, ev[4], ev[5]);
-        if (r > 0) total += a + b;
-      }
-      const double ms = total / reps;
-      if (times_ms && slot < max_times) times_ms[slot] = ms;
-      ++slot;
-      if (ms < best) {
-        best = ms;
-        best_v = v;
-      }
+  // Tuned roles in slot order, with the pass intervals (event pairs) their
+  // kernels occupy; the last two slots receive each layout's step time.
+  struct RoleSpan { int role, layout, p0, p1; };
+  const RoleSpan spans[9] = {
+      {ROLE_PFC_X, 1, 0, -1},   {ROLE_COL_Y, 1, 1, 3},   {ROLE_PFC_Z, 1, 2, -1},
+      {ROLE_X_C2R, 1, 4, -1},   {ROLE_T_X_FWD, 0, 0, -1}, {ROLE_T_Y_FWD, 0, 1, -1},
+      {ROLE_T_Z, 0, 2, -1},     {ROLE_T_Y_INV, 0, 3, -1}, {ROLE_T_X_INV, 0, 4, -1}};
+  auto run_step = [&](double *pass_ms) -> int {
+    int r = plan->prec == PFC_HIP_F64 ? step_impl<double>(plan, psi, s, ev.data())
+                                      : step_impl<float>(plan, psi, s, ev.data());
+    if (r) return r;
+    hipError_t he = hipEventSynchronize(ev[5]);
+    if (he != hipSuccess) return fail(PFC_HIP_EDEVICE, hipGetErrorString(he));
+    for (int k = 0; k < 5; ++k) {
+      float ms = 0;
+      hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      pass_ms[k] = ms;
     }
-    plan->var[role] = best_v;
+    return 0;
+  };
+  int slot = 0;
+  double layout_ms[2] = {0, 0};
+  const int layouts = plan->t_ok ? 2 : 1;
+  for (int li = 0; li < layouts && !rc; ++li) {
+    const int layout = plan->t_ok ? li : 1;
+    rc = pfc_hip_variant_set(plan, ROLE_STEP_LAYOUT, plan->t_ok ? layout : 0);
+    for (int si = 0; si < 9 && !rc; ++si) {
+      const RoleSpan &sp = spans[si];
+      if (sp.layout != layout) continue;
+      int count = 0;
+      pfc_hip_variant_count(plan, sp.role, &count);
+      double best = 1e300;
+      int best_v = plan->var[sp.role];
+      for (int v = 0; v < count && !rc; ++v) {
+        plan->var[sp.role] = v;
+        double total = 0, pm[5];
+        for (int r = 0; r <= reps && !rc; ++r) { // r == 0 warms up
+          rc = run_step(pm);
+          if (!rc && r > 0) total += pm[sp.p0] + (sp.p1 >= 0 ? pm[sp.p1] : 0.0);
+        }
+        const double ms = total / reps;
+        const int idx = slot + v;
+        if (times_ms && idx < max_times) times_ms[idx] = ms;
+        if (ms < best) {
+          best = ms;
+          best_v = v;
+        }
+      }
+      plan->var[sp.role] = best_v;
+      slot += count;
+    }
+    double total = 0, pm[5];
+    for (int r = 0; r <= reps && !rc; ++r) {
+      rc = run_step(pm);
+      if (!rc && r > 0) total += pm[0] + pm[1] + pm[2] + pm[3] + pm[4];
+    }
+    layout_ms[layout] = total / reps;
+  }
+  if (!rc && plan->t_ok) {
+    rc = pfc_hip_variant_set(plan, ROLE_STEP_LAYOUT,
+                             layout_ms[0] <= layout_ms[1] ? 0 : 1);
+  }
+  if (times_ms && slot + 1 < max_times) {
+    times_ms[slot] = layout_ms[0];
+    times_ms[slot + 1] = layout_ms[1];
   }
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);

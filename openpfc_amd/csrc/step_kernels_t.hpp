@@ -1,0 +1,316 @@
+// step_kernels_t.hpp -- the fused PFC step on transposed spectral layouts.
+//
+// Every pass reads or writes whole contiguous lines, one line per TL threads
+// (one wave for the 512- and 1024-point plans), and the change of layout
+// between passes goes through an LDS tile once per block:
+//
+//   psi  [z][y][x]   real, x contiguous           (caller's field)
+//   T1   [z][kx][y]  complex, y contiguous        (after forward x; inverse y)
+//   T2   [kx][ky][z] complex, z contiguous        (after forward y; z pass)
+//
+//   k_tx_fwd  psi -> T1 (Psi and N):  rows of psi, transposed store
+//   k_ty_fwd  T1 -> T2 (both fields): y lines of T1, transposed store
+//   k_tz      T2 -> T2 (in place):    z lines; forward both, ETD combine, inverse
+//   k_ty_inv  T2 -> T1:               transposed load, y lines of T1
+//   k_tx_inv  T1 -> psi:              transposed load, C2R rows of psi
+//
+// A block handles G lines that are adjacent in the transposed dimension, so
+// the strided side of each pass moves G*sizeof(complex) contiguous bytes
+// (128 bytes for G = 8 in double).  Algorithmic traffic is the same as the
+// natural-layout step: R + 2C, 4C, 3C, 2C, C + R.
+#pragma once
+
+#include "fft_kernels.hpp"
+
+namespace pfc_amd {
+
+// LDS plane size (elements) of G lines of an N-point plan, row layout.
+template <typename Plan, int G> constexpr int t_plane() {
+  return G * row_padded(Plan::N);
+}
+
+// Transpose tile: TL positions x G lines, one padding column per row so the
+// position-major writes of a line hit distinct banks.
+template <int G> __device__ __forceinline__ int tile_idx(int pos, int g) {
+  return pos * (G + 1) + g;
+}
+
+// Fetch, for every owned element, the value at line position pp(k) of the
+// same line through LDS; one plane at a time when SPLIT (half the LDS).
+template <typename T, int E, int TL, bool SPLIT, typename PartnerPos>
+__device__ __forceinline__ void fetch_partner(const cplx<T> (&v)[E], cplx<T> (&p)[E],
+                                              int t, T *re, T *im,
+                                              PartnerPos pp) {
+  if constexpr (SPLIT) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) re[pad_pos<true>(t + TL * e)] = v[e].x;
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) p[e].x = re[pad_pos<true>(pp(t + TL * e))];
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) re[pad_pos<true>(t + TL * e)] = v[e].y;
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) p[e].y = re[pad_pos<true>(pp(t + TL * e))];
+    __syncthreads();
+  } else {
+    row_fetch_partner<T, E, TL>(v, p, t, re, im, pp);
+  }
+}
+
+template <typename T> struct TStepArgs {
+  const T *psi_in;  // k_tx_fwd input
+  T *psi_out;       // k_tx_inv output
+  cplx<T> *A1, *B1; // T1 buffers (Psi, N)
+  cplx<T> *A2, *B2; // T2 buffers (Psi, N)
+  const cplx<T> *tw_x;  // nx-point line twiddles (forward x)
+  const cplx<T> *tw_xm; // nx/2-point line twiddles (inverse x)
+  const cplx<T> *wn;    // exp(-2 pi i k/nx), k = 0..nx/2
+  const cplx<T> *tw_y, *tw_z;
+  PfcOp op;
+  int nx, ny, nz, W;
+};
+
+// ----------------------------------------------------------------------------
+// psi -> T1: G consecutive rows y0..y0+G-1 of plane z per block.
+// ----------------------------------------------------------------------------
+template <typename T, typename Plan, int G, bool SPLIT>
+__global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd(TStepArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
+  constexpr int PL = t_plane<Plan, G>();
+  __shared__ T lds[(SPLIT ? 1 : 2) * PL];
+  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
+  const int ngy = (a.ny + G - 1) / G;
+  const int z = blockIdx.x / ngy, y0 = (blockIdx.x % ngy) * G;
+  const int y = y0 + g;
+  const bool active = y < a.ny;
+  const T *__restrict__ row = a.psi_in + ((long)z * a.ny + (active ? y : y0)) * N;
+
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const T u = row[t + TL * e];
+    v[e] = cplx<T>{u, u * u * u};
+  }
+  T *re = lds + g * RP;
+  T *im = lds + (SPLIT ? 0 : PL) + g * RP;
+  const LdsLine<T> ll{lds, g * RP, 1, PL};
+  LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_x);
+  cplx<T> pz[E];
+  fetch_partner<T, E, TL, SPLIT>(v, pz, t, re, im,
+                                 [](int k) { return k == 0 ? 0 : N - k; });
+  // split Z into Psi (v) and N (pz) for k = t + TL*e, e < E/2
+#pragma unroll
+  for (int e = 0; e <= E / 2 && e < E; ++e) {
+    const cplx<T> zk = v[e], zc = c_conj(pz[e]);
+    const cplx<T> s = c_add(zk, zc), d = c_sub(zk, zc);
+    v[e] = c_scale(s, (T)0.5);
+    pz[e] = cplx<T>{(T)0.5 * d.y, (T)-0.5 * d.x};
+  }
+  // k = N/2 (owned by t = 0 as element E/2): written directly
+  if (active && t == 0) {
+    const long o = ((long)z * a.W + N / 2) * a.ny + y;
+    a.A1[o] = v[E / 2];
+    a.B1[o] = pz[E / 2];
+  }
+  // transposed store of kx = TL*r + j, j < TL, r < E/2, through a TL x G tile
+  T *tr = lds, *ti = lds + TL * (G + 1);
+#pragma unroll
+  for (int f = 0; f < 2; ++f) {
+    cplx<T> *__restrict__ out = f ? a.B1 : a.A1;
+#pragma unroll
+    for (int r = 0; r < E / 2; ++r) {
+      const cplx<T> w = f ? pz[r] : v[r];
+      tr[tile_idx<G>(t, g)] = w.x;
+      ti[tile_idx<G>(t, g)] = w.y;
+      __syncthreads();
+      const int j = threadIdx.x / G, gg = threadIdx.x % G;
+      if (y0 + gg < a.ny) {
+        const long o = ((long)z * a.W + TL * r + j) * a.ny + y0 + gg;
+        out[o] = cplx<T>{tr[tile_idx<G>(j, gg)], ti[tile_idx<G>(j, gg)]};
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// T1 -> T2 forward y: block = (field, kx, z0..z0+G-1).
+// ----------------------------------------------------------------------------
+template <typename T, typename Plan, int G, bool SPLIT>
+__global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd(TStepArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
+  constexpr int PL = t_plane<Plan, G>();
+  __shared__ T lds[(SPLIT ? 1 : 2) * PL];
+  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
+  const int ngz = (a.nz + G - 1) / G;
+  const int kx = blockIdx.x / ngz, z0 = (blockIdx.x % ngz) * G;
+  const int z = z0 + g;
+  const bool active = z < a.nz;
+  const cplx<T> *__restrict__ in = blockIdx.y ? a.B1 : a.A1;
+  cplx<T> *__restrict__ out = blockIdx.y ? a.B2 : a.A2;
+  const cplx<T> *line = in + ((long)(active ? z : z0) * a.W + kx) * N;
+
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = line[t + TL * e];
+  const LdsLine<T> ll{lds, g * RP, 1, PL};
+  LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_y);
+  // transposed store: T2[(kx*ny + ky)*nz + z], ky = TL*r + j
+  T *tr = lds, *ti = lds + TL * (G + 1);
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    tr[tile_idx<G>(t, g)] = v[r].x;
+    ti[tile_idx<G>(t, g)] = v[r].y;
+    __syncthreads();
+    const int j = threadIdx.x / G, gg = threadIdx.x % G;
+    if (z0 + gg < a.nz) {
+      const long o = ((long)kx * N + TL * r + j) * a.nz + z0 + gg;
+      out[o] = cplx<T>{tr[tile_idx<G>(j, gg)], ti[tile_idx<G>(j, gg)]};
+    }
+    __syncthreads();
+  }
+}
+
+// ----------------------------------------------------------------------------
+// T2 z pass, in place: G consecutive z lines (kx*ny + ky) per block.
+// ----------------------------------------------------------------------------
+template <typename T, typename Plan, int G, bool SPLIT>
+__global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
+  constexpr int PL = t_plane<Plan, G>();
+  __shared__ T lds[(SPLIT ? 1 : 2) * PL];
+  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
+  const long nlines = (long)a.W * a.ny;
+  const long l0 = (long)blockIdx.x * G;
+  const long li = l0 + g;
+  const bool active = li < nlines;
+  const long l = active ? li : l0;
+  const int kx = (int)(l / a.ny), ky = (int)(l % a.ny);
+  const LdsLine<T> ll{lds, g * RP, 1, PL};
+
+  cplx<T> acc[E];
+  cplx<T> *__restrict__ la = a.A2 + l * N;
+#pragma unroll
+  for (int e = 0; e < E; ++e) acc[e] = la[t + TL * e];
+  LineFft<T, Plan, -1, SPLIT, true>::run(acc, t, ll, a.tw_z);
+  const double kxy = pfc_kxy(a.op, kx, ky);
+  T opn[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    double oL, oN;
+    pfc_op_mode(a.op, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), oL, oN);
+    acc[e] = c_scale(acc[e], (T)oL);
+    opn[e] = (T)oN;
+  }
+  {
+    const cplx<T> *__restrict__ lb = a.B2 + l * N;
+    cplx<T> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = lb[t + TL * e];
+    LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_z);
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc[e] = c_add(acc[e], c_scale(v[e], opn[e]));
+  }
+  LineFft<T, Plan, +1, SPLIT, true>::run(acc, t, ll, a.tw_z);
+  if (active) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) la[t + TL * e] = acc[e];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// T2 -> T1 inverse y: block = (kx, z0..z0+G-1); transposed load.
+// ----------------------------------------------------------------------------
+template <typename T, typename Plan, int G, bool SPLIT>
+__global__ void __launch_bounds__(G *Plan::TL) k_ty_inv(TStepArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
+  constexpr int PL = t_plane<Plan, G>();
+  __shared__ T lds[(SPLIT ? 1 : 2) * PL];
+  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
+  const int ngz = (a.nz + G - 1) / G;
+  const int kx = blockIdx.x / ngz, z0 = (blockIdx.x % ngz) * G;
+  const int z = z0 + g;
+
+  cplx<T> v[E];
+  T *tr = lds, *ti = lds + TL * (G + 1);
+#pragma unroll
+  for (int r = 0; r < E; ++r) {
+    const int j = threadIdx.x / G, gg = threadIdx.x % G;
+    cplx<T> w{0, 0};
+    if (z0 + gg < a.nz) w = a.A2[((long)kx * N + TL * r + j) * a.nz + z0 + gg];
+    tr[tile_idx<G>(j, gg)] = w.x;
+    ti[tile_idx<G>(j, gg)] = w.y;
+    __syncthreads();
+    v[r] = cplx<T>{tr[tile_idx<G>(t, g)], ti[tile_idx<G>(t, g)]};
+    __syncthreads();
+  }
+  const LdsLine<T> ll{lds, g * RP, 1, PL};
+  LineFft<T, Plan, +1, SPLIT, true>::run(v, t, ll, a.tw_y);
+  if (z < a.nz) {
+    cplx<T> *__restrict__ line = a.A1 + ((long)z * a.W + kx) * N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) line[t + TL * e] = v[e];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// T1 -> psi inverse x (C2R): block = (z, y0..y0+G-1); the G rows' half
+// spectra (W values each) are staged in LDS, then each line runs the
+// nx/2-point inverse.  Imaginary parts of X[0] and X[nx/2] are ignored.
+// Plan is the M = nx/2 point plan.
+// ----------------------------------------------------------------------------
+template <typename T, typename Plan, int G> constexpr int tx_inv_lds() {
+  constexpr int stage = 2 * (Plan::N + 1) * (G + 1);
+  constexpr int fft = 2 * t_plane<Plan, G>();
+  return stage > fft ? stage : fft;
+}
+
+template <typename T, typename Plan, int G>
+__global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N, RP = row_padded(M);
+  constexpr int PL = t_plane<Plan, G>();
+  constexpr int W = M + 1;
+  __shared__ T lds[tx_inv_lds<T, Plan, G>()];
+  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
+  const int ngy = (a.ny + G - 1) / G;
+  const int z = blockIdx.x / ngy, y0 = (blockIdx.x % ngy) * G;
+  const int y = y0 + g;
+
+  // stage: sr/si[kx*(G+1) + line]
+  T *sr = lds, *si = lds + W * (G + 1);
+  for (int i = threadIdx.x; i < W * G; i += G * TL) {
+    const int kx = i / G, gg = i % G;
+    cplx<T> w{0, 0};
+    if (y0 + gg < a.ny) w = a.A1[((long)z * a.W + kx) * a.ny + y0 + gg];
+    sr[tile_idx<G>(kx, gg)] = w.x;
+    si[tile_idx<G>(kx, gg)] = w.y;
+  }
+  __syncthreads();
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + TL * e;
+    cplx<T> xa{sr[tile_idx<G>(k, g)], si[tile_idx<G>(k, g)]};
+    cplx<T> xb{sr[tile_idx<G>(M - k, g)], -si[tile_idx<G>(M - k, g)]};
+    if (k == 0) {
+      xa.y = (T)0;
+      xb.y = (T)0;
+    }
+    const cplx<T> ev = c_add(xa, xb);
+    const cplx<T> od = c_mulc(c_sub(xa, xb), a.wn[k]);
+    v[e] = cplx<T>{ev.x - od.y, ev.y + od.x};
+  }
+  __syncthreads(); // the staging area becomes the FFT exchange area
+  const LdsLine<T> ll{lds, g * RP, 1, PL};
+  LineFft<T, Plan, +1, false, true>::run(v, t, ll, a.tw_xm);
+  if (y < a.ny) {
+    cplx<T> *__restrict__ out =
+        reinterpret_cast<cplx<T> *>(a.psi_out + ((long)z * a.ny + y) * (2L * M));
+#pragma unroll
+    for (int e = 0; e < E; ++e) out[t + TL * e] = v[e];
+  }
+}
+
+} // namespace pfc_amd

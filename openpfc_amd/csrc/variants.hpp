@@ -8,6 +8,7 @@
 #pragma once
 
 #include "fft_kernels.hpp"
+#include "step_kernels_t.hpp"
 
 #include <string>
 #include <vector>
@@ -21,8 +22,17 @@ enum Role {
   ROLE_COL_Z = 3, // z column passes of the plain transforms
   ROLE_PFC_X = 4, // step forward x (nx-point lines, psi and psi^3)
   ROLE_PFC_Z = 5, // step z pass (forward, combine, inverse)
-  ROLE_COUNT = 6
+  ROLE_STEP_LAYOUT = 6, // 0: transposed spectra (T1/T2), 1: natural layout
+  ROLE_T_X_FWD = 7,     // transposed step: psi -> T1
+  ROLE_T_Y_FWD = 8,     // T1 -> T2
+  ROLE_T_Z = 9,         // T2 z pass
+  ROLE_T_Y_INV = 10,    // T2 -> T1
+  ROLE_T_X_INV = 11,    // T1 -> psi
+  ROLE_COUNT = 12
 };
+
+// Kinds of transposed-step kernels (one variant list per kind and length).
+enum TKind { TK_X_FWD = 0, TK_Y_FWD = 1, TK_Z = 2, TK_Y_INV = 3, TK_X_INV = 4 };
 
 template <typename T> struct ColV {
   std::string name;
@@ -46,18 +56,26 @@ template <typename T> struct PXV {
   hipError_t (*fn)(PfcXArgs<T> a, hipStream_t s);
 };
 
+template <typename T> struct TV {
+  std::string name;
+  int plan_id;
+  hipError_t (*fn)(TStepArgs<T> a, hipStream_t s);
+};
+
 // Variant lists for a line length n (empty when n is unsupported).
 template <typename T> const std::vector<ColV<T>> &col_variants(int n);
 template <typename T> const std::vector<ZV<T>> &pfcz_variants(int n);
 template <typename T> const std::vector<XV<T>> &xr2c_variants(int m);
 template <typename T> const std::vector<XV<T>> &xc2r_variants(int m);
 template <typename T> const std::vector<PXV<T>> &pfcx_variants(int n);
+template <typename T> const std::vector<TV<T>> &t_variants(int kind, int n);
 #define PFC_AMD_DECLARE_LISTS(T)                                               \
   template <> const std::vector<ColV<T>> &col_variants<T>(int);                \
   template <> const std::vector<ZV<T>> &pfcz_variants<T>(int);                 \
   template <> const std::vector<XV<T>> &xr2c_variants<T>(int);                 \
   template <> const std::vector<XV<T>> &xc2r_variants<T>(int);                 \
-  template <> const std::vector<PXV<T>> &pfcx_variants<T>(int);
+  template <> const std::vector<PXV<T>> &pfcx_variants<T>(int);               \
+  template <> const std::vector<TV<T>> &t_variants<T>(int, int);
 PFC_AMD_DECLARE_LISTS(double)
 PFC_AMD_DECLARE_LISTS(float)
 

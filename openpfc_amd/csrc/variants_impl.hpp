@@ -235,6 +235,85 @@ template <typename T, int N> std::vector<PXV<T>> make_pfcx() {
   return v;
 }
 
+// ---- transposed-step kernels ----------------------------------------------------
+template <typename T, typename Plan, int G, bool S>
+hipError_t tx_fwd_launch(TStepArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
+  hipLaunchKernelGGL((k_tx_fwd<T, Plan, G, S>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+template <typename T, typename Plan, int G, bool S>
+hipError_t ty_fwd_launch(TStepArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)(a.W * ((a.nz + G - 1) / G)), 2), block(G * Plan::TL);
+  hipLaunchKernelGGL((k_ty_fwd<T, Plan, G, S>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+template <typename T, typename Plan, int G, bool S>
+hipError_t tz_launch(TStepArgs<T> a, hipStream_t s) {
+  const long lines = (long)a.W * a.ny;
+  dim3 grid((unsigned)((lines + G - 1) / G)), block(G * Plan::TL);
+  hipLaunchKernelGGL((k_tz<T, Plan, G, S>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+template <typename T, typename Plan, int G, bool S>
+hipError_t ty_inv_launch(TStepArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)(a.W * ((a.nz + G - 1) / G))), block(G * Plan::TL);
+  hipLaunchKernelGGL((k_ty_inv<T, Plan, G, S>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+template <typename T, typename Plan, int G, bool S>
+hipError_t tx_inv_launch(TStepArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
+  hipLaunchKernelGGL((k_tx_inv<T, Plan, G>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T, typename Plan, int G, bool S, int KIND> constexpr bool t_fits() {
+  if (G * Plan::TL > kMaxThreads || G * Plan::TL < 64) return false;
+  constexpr int fft = (S ? 1 : 2) * G * row_padded(Plan::N) * (int)sizeof(T);
+  if (KIND == TK_X_INV) {
+    constexpr int stage = 2 * (Plan::N + 1) * (G + 1) * (int)sizeof(T);
+    constexpr int full = 2 * G * row_padded(Plan::N) * (int)sizeof(T);
+    return !S && (stage > full ? stage : full) <= kMaxLds;
+  }
+  return fft <= kMaxLds;
+}
+
+template <typename T, typename Plan, int G, bool S, int KIND>
+void add_t(std::vector<TV<T>> &v, int pid) {
+  if constexpr (t_fits<T, Plan, G, S, KIND>()) {
+    const std::string nm = plan_name<Plan>() + "/G" + std::to_string(G) +
+                           (KIND == TK_X_INV ? "" : (S ? "/split" : "/full"));
+    if constexpr (KIND == TK_X_FWD) v.push_back({nm, pid, &tx_fwd_launch<T, Plan, G, S>});
+    if constexpr (KIND == TK_Y_FWD) v.push_back({nm, pid, &ty_fwd_launch<T, Plan, G, S>});
+    if constexpr (KIND == TK_Z) v.push_back({nm, pid, &tz_launch<T, Plan, G, S>});
+    if constexpr (KIND == TK_Y_INV) v.push_back({nm, pid, &ty_inv_launch<T, Plan, G, S>});
+    if constexpr (KIND == TK_X_INV) v.push_back({nm, pid, &tx_inv_launch<T, Plan, G, S>});
+  }
+}
+
+// Lines per block giving 128-byte chunks first (8 in double, 16 in float),
+// then half and double that.
+template <typename T, typename Plan, int KIND>
+void t_for_plan(std::vector<TV<T>> &v, int pid) {
+  constexpr int G0 = 128 / (2 * (int)sizeof(T));
+  add_t<T, Plan, G0, true, KIND>(v, pid);
+  add_t<T, Plan, G0, false, KIND>(v, pid);
+  add_t<T, Plan, G0 / 2, true, KIND>(v, pid);
+  add_t<T, Plan, G0 / 2, false, KIND>(v, pid);
+  add_t<T, Plan, 2 * G0, true, KIND>(v, pid);
+}
+
+template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
+  std::vector<TV<T>> v;
+  if constexpr (N >= 128) {
+    for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+      t_for_plan<T, decltype(p), KIND>(v, pid);
+    });
+  }
+  return v;
+}
+
 #define PFC_AMD_SWITCH_LEN(n, ...)                                            \
   switch (n) {                                                                 \
   case 1: { constexpr int N_ = 1; __VA_ARGS__ }                                      \
@@ -275,6 +354,16 @@ template <typename T, int N> std::vector<PXV<T>> make_pfcx() {
   template <> const std::vector<PXV<T>> &pfcx_variants<T>(int n) {             \
     static const std::vector<PXV<T>> empty;                                    \
     PFC_AMD_SWITCH_LEN(n, static const auto v = make_pfcx<T, N_>(); return v;) \
+    return empty;                                                              \
+  }                                                                            \
+  template <> const std::vector<TV<T>> &t_variants<T>(int kind, int n) {       \
+    static const std::vector<TV<T>> empty;                                     \
+    if (kind < 0 || kind > 4) return empty;                                    \
+    PFC_AMD_SWITCH_LEN(n, static const std::vector<TV<T>> v[5] = {             \
+        make_t_kind<T, N_, 0>(), make_t_kind<T, N_, 1>(),                      \
+        make_t_kind<T, N_, 2>(), make_t_kind<T, N_, 3>(),                      \
+        make_t_kind<T, N_, 4>()};                                              \
+                          return v[kind];)                                     \
     return empty;                                                              \
   }
 

@@ -45,7 +45,9 @@ def _rel_max(a, b):
 
 @pytest.mark.parametrize("n,steps", [((16, 16, 16), 20), ((32, 16, 8), 30),
                                      ((1024, 8, 4), 5), ((8, 1024, 4), 5),
-                                     ((8, 4, 1024), 5)])
+                                     ((8, 4, 1024), 5), ((256, 128, 128), 10),
+                                     ((1024, 256, 128), 3), ((256, 1024, 128), 3),
+                                     ((256, 128, 1024), 3)])
 def test_step_matches_oracle_small(torch_gpu, n, steps):
     print(f"step n={n} steps={steps}", flush=True)
     psi0 = _psi0(n)

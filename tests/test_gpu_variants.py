@@ -11,6 +11,8 @@ pytestmark = pytest.mark.gpu
 
 DX, DT, EPS = math.pi / 4, 0.5, -0.25
 GRIDS = [(256, 128, 512), (1024, 4, 8), (8, 1024, 4), (8, 4, 1024), (512, 256, 4)]
+# grids where the transposed step layout is available (nx/2, ny, nz >= 128)
+T_GRIDS = [(256, 128, 128), (1024, 128, 256), (256, 512, 128), (256, 128, 1024)]
 
 
 @pytest.mark.parametrize("n", GRIDS)
@@ -66,6 +68,35 @@ def test_all_step_variants(torch_gpu, n):
         f.set_variant(role, 0)
 
 
+@pytest.mark.parametrize("n", T_GRIDS)
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_all_transposed_step_variants(torch_gpu, n, prec):
+    """Both step layouts and every variant of the transposed passes."""
+    torch = torch_gpu
+    import openpfc_amd as P
+    psi0 = O.uniform_noise(n, 41, -0.01, 0.01, base=-0.2)
+    opL, opN = O.pfc_operators(n, (DX, DX, DX), EPS, DT)
+    ref = O.pfc_step(psi0, opL, opN, 2)
+    f = P.HipFft(*n, precision=P.F64 if prec == "f64" else P.F32)
+    assert f.variants("step_layout") == ["transposed", "natural"]
+    st = P.PfcStepper(f, (DX, DX, DX), EPS)
+    st.initialize(DT)
+    rdt = torch.float64 if prec == "f64" else torch.float32
+    tol = 1e-12 if prec == "f64" else 2e-5
+    cases = [("step_layout", 1)] + [(r, i) for r in ["t_x_fwd", "t_y_fwd", "t_z", "t_y_inv",
+                                                     "t_x_inv"]
+                                    for i in range(len(f.variants(r)))]
+    for role, i in cases:
+        name = f.variants(role)[i]
+        print(f"{n} {prec} {role}[{i}] {name}", flush=True)
+        f.set_variant(role, i)
+        psi = torch.from_numpy(psi0).to(rdt).cuda()
+        st.step(psi, 2)
+        err = np.abs(psi.double().cpu().numpy() - ref).max() / np.abs(ref).max()
+        assert err <= tol, f"{role} {name}: {err:.2e}"
+        f.set_variant(role, 0)
+
+
 def test_operators_match_oracle(torch_gpu):
     """On-the-fly opL/opN equal the reference formula (oracle, glibc exp) to
     a few ulp, scaled by 1/N^3."""
@@ -106,9 +137,12 @@ def test_autotune_restores_field_and_picks_valid(torch_gpu):
     psi = torch.from_numpy(psi0).cuda()
     res = st.autotune(psi, reps=1)
     assert np.array_equal(psi.cpu().numpy(), psi0)
-    for role in ["pfc_x", "col_y", "pfc_z", "x_c2r"]:
+    for role in ["pfc_x", "col_y", "pfc_z", "x_c2r", "t_x_fwd", "t_y_fwd", "t_z",
+                 "t_y_inv", "t_x_inv"]:
         assert res["chosen"][role] in f.variants(role)
         assert all(ms > 0 for _, ms in res[role])
+    assert res["layout_ms"]["transposed"] > 0 and res["layout_ms"]["natural"] > 0
+    assert res["chosen"]["step_layout"] in ("transposed", "natural")
     opL, opN = O.pfc_operators(n, (DX, DX, DX), EPS, DT)
     st.step(psi, 2)
     ref = O.pfc_step(psi0, opL, opN, 2)
