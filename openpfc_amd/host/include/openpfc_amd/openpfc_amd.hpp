@@ -1,0 +1,11 @@
+// openpfc_amd.hpp -- umbrella header of the C++ host layer.
+#pragma once
+
+#include "core.hpp"
+#include "fft.hpp"
+#include "field_modifier.hpp"
+#include "model.hpp"
+#include "pfc_model_hip.hpp"
+#include "simulator.hpp"
+#include "time.hpp"
+#include "world.hpp"

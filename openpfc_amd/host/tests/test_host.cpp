@@ -1,0 +1,281 @@
+// test_host.cpp -- checks of the C++ host layer (driven by tests/test_host_cpp.py).
+//
+//   test_host cpu                      Time/Simulator/World/modifier semantics
+//                                      against a host-only IFFT stand-in
+//   test_host gpu-fft                  HipFft overloads, known answers, errors
+//   test_host gpu-sim N STEPS OUT.bin  PFCModelHIP driven by Simulator from
+//                                      Constant(-0.2) + UniformNoise(1234, +-0.01);
+//                                      writes the final field (raw f64)
+#include <openpfc_amd/openpfc_amd.hpp>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace pfc;
+
+static int g_fail = 0;
+#define CHECK(cond)                                                                 \
+  do {                                                                              \
+    if (!(cond)) {                                                                  \
+      std::fprintf(stderr, "CHECK failed %s:%d: %s\n", __FILE__, __LINE__, #cond); \
+      ++g_fail;                                                                     \
+    }                                                                               \
+  } while (0)
+
+template <typename Ex, typename F> static bool throws(F &&f, const char *needle = nullptr) {
+  try {
+    f();
+  } catch (const Ex &e) {
+    return !needle || std::strstr(e.what(), needle) != nullptr;
+  } catch (...) {
+    return false;
+  }
+  return false;
+}
+
+// Host-only IFFT with the one-rank boxes (no transforms): lets the CPU checks
+// exercise Model, field::apply and the modifiers without a device.
+struct BoxOnlyFft final : fft::IFFT {
+  Int3 n;
+  explicit BoxOnlyFft(Int3 s) : n(s) {}
+  void forward(const fft::RealVector &, fft::ComplexVector &) override {
+    throw std::runtime_error("BoxOnlyFft: no transforms");
+  }
+  void backward(const fft::ComplexVector &, fft::RealVector &) override {
+    throw std::runtime_error("BoxOnlyFft: no transforms");
+  }
+  void reset_fft_time() override {}
+  double get_fft_time() const override { return 0; }
+  size_t size_inbox() const override { return (size_t)n[0] * n[1] * n[2]; }
+  size_t size_outbox() const override { return (size_t)(n[0] / 2 + 1) * n[1] * n[2]; }
+  size_t size_workspace() const override { return size_outbox(); }
+  size_t get_allocated_memory_bytes() const override { return 0; }
+  Box3i get_inbox_bounds() const override { return {{0, 0, 0}, {n[0] - 1, n[1] - 1, n[2] - 1}, n}; }
+  Box3i get_outbox_bounds() const override {
+    return {{0, 0, 0}, {n[0] / 2, n[1] - 1, n[2] - 1}, {n[0] / 2 + 1, n[1], n[2]}};
+  }
+};
+
+struct CountingModel final : Model {
+  RealField psi;
+  int steps = 0;
+  std::vector<double> times;
+  int prepares = 0, finalizes = 0;
+  CountingModel(fft::IFFT &f, const World &w) : Model(f, w) {}
+  void initialize(double) override {
+    psi.assign(get_fft().size_inbox(), 0.0);
+    add_real_field("default", psi);
+  }
+  void step(double t) override {
+    ++steps;
+    times.push_back(t);
+  }
+  void prepare_for_field_modifiers() override { ++prepares; }
+  void finalize_after_field_modifiers() override { ++finalizes; }
+};
+
+static int run_cpu() {
+  // Time: t in [0, 100], dt = 0.5 -> exactly 200 model.step calls
+  {
+    World w = domain::create(GridSize({8, 4, 2}), PhysicalOrigin({0, 0, 0}), GridSpacing({1, 1, 1}));
+    BoxOnlyFft f(w.size);
+    CountingModel m(f, w);
+    m.initialize(0.5);
+    Time t({0.0, 100.0, 0.5}, 5.0);
+    Simulator sim(m, t);
+    sim.add_initial_conditions(std::make_unique<Constant>(-0.2));
+    int saves = 0;
+    sim.add_results_writer([&](Model &, const Time &) { ++saves; });
+    while (!done(sim)) pfc::step(sim);
+    CHECK(m.steps == 200);
+    CHECK(t.get_increment() == 200);
+    CHECK(std::abs(m.times.front() - 0.5) < 1e-12 && std::abs(m.times.back() - 100.0) < 1e-12);
+    CHECK(saves == 21); // t = 0, 5, ..., 100
+    CHECK(m.psi[0] == -0.2 && m.psi.back() == -0.2);
+    CHECK(m.prepares >= 1 && m.finalizes == 1); // ICs once, no BCs
+  }
+  // Time validation (time.hpp:366-395)
+  CHECK(throws<std::invalid_argument>([] { Time t({-1.0, 1.0, 0.1}, 0.0); }));
+  CHECK(throws<std::invalid_argument>([] { Time t({1.0, 1.0, 0.1}, 0.0); }));
+  CHECK(throws<std::invalid_argument>([] { Time t({0.0, 1.0, 0.1}, 2.0); }));
+  // World coordinates and k-space helpers
+  {
+    World w = domain::create_centered(GridSize({64, 64, 64}), GridSpacing({M_PI / 4, M_PI / 4, M_PI / 4}));
+    CHECK(std::abs(w.origin[0] + 32 * M_PI / 4) < 1e-12);
+    Real3 x = domain::to_coords(w, {32, 0, 63});
+    CHECK(x[0] == 0.0 && x[1] == w.origin[1] && std::abs(x[2] - (w.origin[2] + 63 * M_PI / 4)) < 1e-12);
+    auto f = fft::kspace::k_frequency_scaling(w);
+    CHECK(std::abs(f[0] - 0.125) < 1e-15);
+    CHECK(fft::kspace::k_component(8, 64, f[0]) == 1.0);
+    CHECK(fft::kspace::k_component(63, 64, f[0]) == -0.125);
+    CHECK(fft::kspace::k_laplacian_value(1, 2, 3) == -14.0);
+  }
+  // SingleSeed: rho + 12 amp at the origin, 0 outside
+  {
+    World w = domain::create(GridSize({16, 16, 16}), PhysicalOrigin({-8, -8, -8}), GridSpacing({1, 1, 1}));
+    BoxOnlyFft f(w.size);
+    CountingModel m(f, w);
+    m.initialize(1.0);
+    SingleSeed s;
+    s.set_density(-0.1);
+    s.set_amplitude(0.05);
+    s.set_radius(4.0);
+    s.apply(m, 0.0);
+    const size_t c = (8 * 16 + 8) * 16 + 8;
+    CHECK(std::abs(m.psi[c] - (-0.1 + 12 * 0.05)) < 1e-15);
+    CHECK(m.psi[0] == 0.0);
+    // field size mismatch is rejected
+    RealField bad(10);
+    CHECK(throws<std::invalid_argument>(
+        [&] { field::apply(bad, w, f, [](const Real3 &) { return 0.0; }); }, "inbox"));
+    CHECK(throws<std::invalid_argument>([&] { m.get_real_field("nope"); }));
+  }
+  // backend names (backend_from_string.hpp)
+  CHECK(fft::backend_from_string("hip") == fft::Backend::HIP);
+  CHECK(fft::backend_from_string("rocm") == fft::Backend::HIP);
+  CHECK(throws<std::invalid_argument>([] { fft::backend_from_string("opencl"); }));
+  return g_fail;
+}
+
+static std::vector<std::complex<double>> dft_r2c(const std::vector<double> &a, Int3 n) {
+  const int w = n[0] / 2 + 1;
+  std::vector<std::complex<double>> out((size_t)w * n[1] * n[2]);
+  for (int kz = 0; kz < n[2]; ++kz)
+    for (int ky = 0; ky < n[1]; ++ky)
+      for (int kx = 0; kx < w; ++kx) {
+        std::complex<long double> s = 0;
+        for (int z = 0; z < n[2]; ++z)
+          for (int y = 0; y < n[1]; ++y)
+            for (int x = 0; x < n[0]; ++x) {
+              long double ang = -2.0L * M_PIl *
+                                ((long double)(kx * x % n[0]) / n[0] +
+                                 (long double)(ky * y % n[1]) / n[1] +
+                                 (long double)(kz * z % n[2]) / n[2]);
+              s += (long double)a[((size_t)z * n[1] + y) * n[0] + x] *
+                   std::complex<long double>(cosl(ang), sinl(ang));
+            }
+        out[((size_t)kz * n[1] + ky) * w + kx] = {(double)s.real(), (double)s.imag()};
+      }
+  return out;
+}
+
+static int run_gpu_fft() {
+  World w = domain::create(GridSize({8, 4, 2}), PhysicalOrigin({0, 0, 0}), GridSpacing({1, 1, 1}));
+  fft::HipFft f(w);
+  CHECK(f.size_inbox() == 64 && f.size_outbox() == 5 * 4 * 2 && f.size_workspace() == 40);
+  Box3i ob = f.get_outbox_bounds();
+  CHECK(ob.high[0] == 4 && ob.size[0] == 5);
+  std::vector<double> a(64);
+  for (size_t i = 0; i < a.size(); ++i) a[i] = std::sin(0.37 * i) + 0.1 * (i % 5);
+  auto ref = dft_r2c(a, w.size);
+  // host-vector overloads (staged)
+  std::vector<std::complex<double>> X(f.size_outbox());
+  f.forward(a, X);
+  double err = 0, mx = 0;
+  for (size_t i = 0; i < X.size(); ++i) {
+    err = std::max(err, std::abs(X[i] - ref[i]));
+    mx = std::max(mx, std::abs(ref[i]));
+  }
+  CHECK(err <= 1e-13 * mx);
+  std::vector<double> back(64);
+  f.backward(X, back);
+  for (size_t i = 0; i < a.size(); ++i) CHECK(std::abs(back[i] - a[i]) <= 1e-14);
+  CHECK(f.get_fft_time() > 0);
+  // DataBuffer overloads
+  core::DataBuffer<backend::HipTag, double> din(64), dout(64);
+  core::DataBuffer<backend::HipTag, std::complex<double>> dX(40);
+  din.copy_from_host(a);
+  f.forward(din, dX);
+  auto hX = dX.to_host();
+  for (size_t i = 0; i < hX.size(); ++i) CHECK(std::abs(hX[i] - X[i]) <= 1e-15 * mx);
+  f.backward(dX, dout);
+  auto hb = dout.to_host();
+  for (size_t i = 0; i < hb.size(); ++i) CHECK(std::abs(hb[i] - a[i]) <= 1e-14);
+  // size and precision errors (fft_heffte_backend.hpp:53-61)
+  std::vector<double> small(63);
+  CHECK(throws<std::invalid_argument>([&] { f.forward(small, X); },
+                                      "FFT_Impl::forward: real buffer size 63 != size_inbox() 64"));
+  std::vector<std::complex<double>> Xs(39);
+  CHECK(throws<std::invalid_argument>([&] { f.forward(a, Xs); }, "size_outbox"));
+  CHECK(throws<std::invalid_argument>([&] { f.backward(Xs, back); }, "size_outbox"));
+  core::DataBuffer<backend::HipTag, float> fin(64);
+  core::DataBuffer<backend::HipTag, std::complex<float>> fX(40);
+  CHECK(throws<std::runtime_error>([&] { f.forward(fin, fX); }, "precision"));
+  // reference KAT (test_fft.cpp:59-70): 2x1x1 backward of {1, 2} -> 1.5
+  {
+    World w2 = domain::create(GridSize({2, 1, 1}), PhysicalOrigin({0, 0, 0}), GridSpacing({1, 1, 1}));
+    fft::HipFft f2(w2);
+    std::vector<std::complex<double>> in = {{1, 0}, {2, 0}};
+    std::vector<double> out(2);
+    f2.backward(in, out);
+    CHECK(std::abs(out[0] - 1.5) <= 0.01);
+  }
+  // unsupported extents raise
+  CHECK(throws<std::runtime_error>([] {
+    fft::HipFft bad(domain::create(GridSize({6, 6, 6}), PhysicalOrigin({0, 0, 0}),
+                                   GridSpacing({1, 1, 1})));
+  }, "unsupported"));
+  // float plan through DataBuffer<float>
+  {
+    fft::HipFft ff(w, PFC_HIP_F32);
+    std::vector<float> af(a.begin(), a.end());
+    core::DataBuffer<backend::HipTag, float> fa(64);
+    fa.copy_from_host(af);
+    ff.forward(fa, fX);
+    auto h = fX.to_host();
+    for (size_t i = 0; i < h.size(); ++i)
+      CHECK(std::abs(std::complex<double>(h[i]) - ref[i]) <= 2e-6 * mx);
+  }
+  return g_fail;
+}
+
+static int run_gpu_sim(int n, int steps, const char *out) {
+  const double dx = M_PI / 4, dt = 0.5;
+  World w = domain::create_centered(GridSize({n, n, n}), GridSpacing({dx, dx, dx}));
+  fft::HipFft f(w);
+  PFCModelHIP model(f, w, -0.25);
+  initialize(model, dt);
+  Time time({0.0, steps * dt, dt}, 0.0);
+  Simulator sim(model, time);
+  auto c = std::make_unique<Constant>(-0.2);
+  c->set_field_name("density");
+  sim.add_initial_conditions(std::move(c));
+  auto noise = std::make_unique<UniformNoise>(1234, -0.01, 0.01);
+  noise->set_field_name("density");
+  sim.add_initial_conditions(std::move(noise));
+  while (!done(sim)) pfc::step(sim);
+  model.prepare_for_field_modifiers();
+  const RealField &psi = get_real_field(model, "density");
+  std::ofstream o(out, std::ios::binary);
+  o.write(reinterpret_cast<const char *>(psi.data()), (std::streamsize)(psi.size() * sizeof(double)));
+  std::printf("steps %d increment %d\n", steps, time.get_increment());
+  CHECK(time.get_increment() == steps);
+  return g_fail;
+}
+
+int main(int argc, char **argv) {
+  const std::string mode = argc > 1 ? argv[1] : "cpu";
+  int rc = 0;
+  try {
+    if (mode == "cpu") rc = run_cpu();
+    else if (mode == "gpu-fft") rc = run_gpu_fft();
+    else if (mode == "gpu-sim" && argc == 5) rc = run_gpu_sim(std::atoi(argv[2]), std::atoi(argv[3]), argv[4]);
+    else {
+      std::fprintf(stderr, "usage: test_host cpu | gpu-fft | gpu-sim N STEPS OUT\n");
+      return 2;
+    }
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "exception: %s\n", e.what());
+    return 3;
+  }
+  std::printf("%s: %s (%d failed checks)\n", mode.c_str(), rc ? "FAIL" : "ok", rc);
+  return rc ? 1 : 0;
+}

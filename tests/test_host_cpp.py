@@ -1,0 +1,54 @@
+"""Drives the C++ host-layer checks (openpfc_amd/host/tests/test_host.cpp).
+
+CPU: Time/Simulator step counts and save points (time.hpp:732-754,972-978;
+simulator_integrator.hpp:43-61), World coordinates, k-space helpers,
+SingleSeed/Constant values, error behaviour.  GPU: HipFft overloads against a
+direct DFT, the reference's size-mismatch messages, and a PFCModelHIP run
+through Simulator compared with the CPU oracle from the same initial field.
+"""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "openpfc_amd", "host", "test_host")
+
+
+def _run(*args, timeout=120):
+    if not os.path.exists(BIN):
+        pytest.fail(f"{BIN} missing: run __graft_entry__.build()")
+    r = subprocess.run([BIN, *args], capture_output=True, text=True, timeout=timeout)
+    print(r.stdout, r.stderr)
+    return r
+
+
+def test_host_layer_cpu():
+    r = _run("cpu")
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+def test_host_layer_hipfft(torch_gpu):
+    r = _run("gpu-fft")
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,steps", [(32, 30), (64, 50)])
+def test_host_pfc_model_simulator_matches_oracle(torch_gpu, tmp_path, n, steps):
+    out = tmp_path / "psi.bin"
+    r = _run("gpu-sim", str(n), str(steps), str(out))
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(out, dtype=np.float64).reshape(n, n, n)
+    psi0 = np.full((n, n, n), -0.2)
+    O.add_counter_noise(psi0, 1234, -0.01, 0.01)
+    dx = math.pi / 4
+    opL, opN = O.pfc_operators((n, n, n), (dx, dx, dx), -0.25, 0.5)
+    ref = O.pfc_step(psi0, opL, opN, steps)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err <= 1e-10, err
