@@ -37,9 +37,35 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 # steps as well); 0.1 is stable and near the equilibrium amplitude.
 SEED_AMP = 0.1
 PASS_NAMES = ["x_fwd_pfc", "y_fwd", "z_fused", "y_inv", "x_inv_c2r"]
-KERNEL_NAMES = {  # device kernel symbol stems, for matching rocprof output
-    "x_fwd_pfc": "k_pfc_x", "y_fwd": "k_col", "z_fused": "k_pfc_z",
-    "y_inv": "k_col", "x_inv_c2r": "k_x_c2r"}
+# (kernel symbol, variant role) of each pass in the two step layouts
+PASS_KERNELS = {
+    "transposed": [("k_tx_fwd", "t_x_fwd"), ("k_ty_fwd", "t_y_fwd"), ("k_tz", "t_z"),
+                   ("k_ty_inv", "t_y_inv"), ("k_tx_inv", "t_x_inv")],
+    "natural": [("k_pfc_x", "pfc_x"), ("k_col", "col_y"), ("k_pfc_z", "pfc_z"),
+                ("k_col", "col_y"), ("k_x_c2r", "x_c2r")]}
+STEP_ROLES = ["step_layout", "pfc_x", "col_y", "pfc_z", "x_c2r", "t_x_fwd", "t_y_fwd",
+              "t_z", "t_y_inv", "t_x_inv"]
+
+
+def variant_key(kernel, prec, n, variant, direction=None):
+    """rocprof short name (openpfc_amd.profile_summary.short_name) of a variant."""
+    import re
+    parts = variant.split("/")
+    radices = parts[0][1:]
+    e = parts[1][1:]
+    nn = n
+    if kernel in ("k_tx_inv", "k_x_c2r"):
+        nn = n // 2
+    rest = []
+    for p in parts[2:]:
+        if p.startswith(("G", "L", "rows")):
+            rest.append(re.sub(r"^(G|L|rows)", "", p))
+        elif p in ("split", "full"):
+            rest.append("true" if p == "split" else "false")
+    if kernel == "k_col":
+        rest = [str(direction)] + rest
+    t = "double" if prec == "f64" else "float"
+    return f"{kernel}<{t},N{nn}/E{e},r{radices}" + ("," + ",".join(rest) if rest else "") + ">"
 
 
 def pass_bytes(n: int, esize: int):
@@ -68,6 +94,24 @@ def setup(P, torch, n, prec, dx, dt, r):
     P.ic_single_seed(fft, psi, origin, (dx, dx, dx), -0.2, SEED_AMP, 64 * dx, keep_outside=True)
     torch.cuda.synchronize()
     return fft, st, psi
+
+
+def choose_variants(P, fft, st, psi, args, log_prefix):
+    """Autotune (untimed) or apply forced variants; returns {role: name}."""
+    import json as _json
+    if args.variants:
+        with open(args.variants) as fh:
+            want = _json.load(fh)
+        for role in STEP_ROLES:
+            if role in want:
+                names = fft.variants(role)
+                if want[role] in names:
+                    fft.set_variant(role, names.index(want[role]))
+    elif not args.no_autotune:
+        t0 = time.perf_counter()
+        st.autotune(psi, reps=2)
+        log(f"{log_prefix} autotune {time.perf_counter() - t0:.1f} s")
+    return {r: fft.variants(r)[fft.get_variant(r)] for r in STEP_ROLES if fft.variants(r)}
 
 
 def time_steps(torch, st, psi, steps, warmup):
@@ -129,6 +173,14 @@ def main(argv=None):
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the (untimed) launch-variant autotuner")
+    ap.add_argument("--variants", help="JSON file {role: variant name} to force instead of "
+                                       "autotuning (e.g. a previous run's choice)")
+    ap.add_argument("--save-variants", help="write the chosen variants to this JSON file")
+    ap.add_argument("--traffic", default=None,
+                    help="traffic table from openpfc_amd.profile_summary (default: "
+                         "profiles/traffic_pfc_step_<n>_f64.json when present)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also time a device-to-device copy of 2 x 4 GiB (the plain "
                          "copy kernel whose PMC byte counts scale the traffic figure)")
@@ -149,6 +201,10 @@ def main(argv=None):
     # ---- FP64 (value) -----------------------------------------------------
     log(f"fp64 {n}^3: setup")
     fft, st, psi = setup(P, torch, n, P.F64, dx, dt, r)
+    chosen64 = choose_variants(P, fft, st, psi, args, "fp64")
+    if args.save_variants:
+        with open(args.save_variants, "w") as fh:
+            json.dump(chosen64, fh, indent=1)
     ms64, wall64 = time_steps(torch, st, psi, args.steps, args.warmup)
     kms = st.step_timed(psi, 3)
     finite = bool(torch.isfinite(psi).all().item())
@@ -161,6 +217,16 @@ def main(argv=None):
     dname = PASS_NAMES[dom]
     achieved = pb[dname] / (kms[dom] * 1e-3)
     step_bytes = sum(pb.values())
+    layout = chosen64.get("step_layout", "natural")
+    kern, role = PASS_KERNELS[layout][dom]
+    kkey = variant_key(kern, "f64", n, chosen64[role], direction=(-1 if dom == 1 else 1))
+    traffic = None
+    tpath = args.traffic or os.path.join(ROOT, "profiles", f"traffic_pfc_step_{n}_f64.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tk = json.load(fh).get("kernels", {})
+        if kkey in tk:
+            traffic = tk[kkey]["traffic_bytes"]
 
     out = {
         "metric": METRIC,
@@ -183,10 +249,11 @@ def main(argv=None):
         "step_alg_bytes": step_bytes,
         "wall_s_timed": wall64,
         "kernels_ms": dict(zip(PASS_NAMES, kms)),
-        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dname], "pass": dname,
+        "roofline": {"bound": "hbm", "kernel": kkey, "pass": dname,
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "alg_bytes_per_launch": pb[dname]},
+        "variants": chosen64,
         "checks": {"finite": finite, "mean_after": mean64},
     }
 
@@ -195,6 +262,7 @@ def main(argv=None):
     if not args.no_fp32:
         log("fp32: setup")
         fft, st, psi = setup(P, torch, n, P.F32, dx, dt, r)
+        chosen32 = choose_variants(P, fft, st, psi, args, "fp32")
         ms32, _ = time_steps(torch, st, psi, args.steps, args.warmup)
         k32 = st.step_timed(psi, 3)
         del psi, st, fft
@@ -204,7 +272,7 @@ def main(argv=None):
                        "ns_per_dof_step": ms32 * 1e6 / n ** 3,
                        "speedup_vs_fp64": ms64 / ms32,
                        "step_hbm_frac": sum(pass_bytes(n, 4).values()) / (ms32 * 1e-3) / HBM_PEAK,
-                       "kernels_ms": dict(zip(PASS_NAMES, k32))}
+                       "kernels_ms": dict(zip(PASS_NAMES, k32)), "variants": chosen32}
 
     # ---- CPU baseline (oracle, rank 0, N=1) --------------------------------
     if not args.no_cpu_baseline:

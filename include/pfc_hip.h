@@ -117,6 +117,18 @@ int pfc_hip_pfc_set_operators(pfc_hip_plan_t plan, int model_kind,
 int pfc_hip_pfc_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *stream);
 
 /*
+ * Mean-split step (the FP32-mixed mode): with enable != 0 the field passed
+ * to pfc_hip_pfc_step holds the deviation d = psi - mean, and `mean` (kept in
+ * double by the plan) is the conserved average of psi.  The step then forms
+ * the nonlinearity as d*(3 mean^2 + (3 mean + d)*d) = psi^3 - mean^3 and keeps
+ * the k = 0 mode of d at zero, so <psi> = mean is conserved exactly and the
+ * precision of the field is spent on the deviation.  Mathematically the same
+ * step as PFCModel::step (opN(0) = 0 there).  enable = 0 restores the plain
+ * step.  No reference counterpart.
+ */
+int pfc_hip_pfc_set_mean(pfc_hip_plan_t plan, int enable, double mean);
+
+/*
  * Measurement variant of pfc_hip_pfc_step: same work, with HIP events
  * around every kernel of the step.  On return (the stream is synchronised)
  * kernel_ms[i] holds the mean duration in milliseconds of pass i over the

@@ -200,6 +200,13 @@ class PfcStepper:
                                                       4, float(dt)))
         self.dt = float(dt)
 
+    def set_mean(self, mean, enable: bool = True) -> None:
+        """Mean-split (FP32-mixed) mode: the field passed to step() then holds
+        the deviation psi - mean; `mean` is kept in double and conserved."""
+        check(self.fft._lib.pfc_hip_pfc_set_mean(self.fft.handle, int(bool(enable)),
+                                                 float(mean)))
+        self.mean = float(mean) if enable else None
+
     def _check_psi(self, psi):
         import torch
         rdt = torch.float64 if self.fft.precision == F64 else torch.float32

@@ -52,6 +52,7 @@ SIGNATURES = {
     "pfc_hip_ic_add_noise": (ctypes.c_int, [_vp, _vp, ctypes.c_ulonglong, ctypes.c_double,
                                             ctypes.c_double, _vp]),
     "pfc_hip_pfc_operators": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "pfc_hip_pfc_set_mean": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
     "pfc_hip_variant_count": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "pfc_hip_variant_name": (ctypes.c_char_p, [_vp, ctypes.c_int, ctypes.c_int]),
     "pfc_hip_variant_get": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),

@@ -193,7 +193,21 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_x_c2r(XArgs<T> a) {
 // for k = 0..N/2.  The nonlinearity is psi*psi*psi, evaluated in that order
 // (10_complete_pfc_model.cpp:131-133).
 // ----------------------------------------------------------------------------
+// Nonlinearity of the step.  Plain: psi*psi*psi in the reference's order
+// (10_complete_pfc_model.cpp:131-133).  Mean-split (the field holds the
+// deviation d = psi - m from an exactly conserved mean m): d*(3m^2 + (3m + d)*d),
+// which equals psi^3 - m^3; the constant m^3 only reaches k = 0, where opN
+// vanishes, so every other mode is unchanged.
+template <typename T> struct Nonlin {
+  T c1, c2; // 3m, 3m^2 (used when split != 0)
+  int split;
+  __device__ __forceinline__ T operator()(T u) const {
+    return split ? u * (c2 + (c1 + u) * u) : u * u * u;
+  }
+};
+
 template <typename T> struct PfcXArgs {
+  Nonlin<T> nl;
   const T *psi;
   cplx<T> *outA; // Psi half spectrum
   cplx<T> *outB; // N half spectrum
@@ -217,7 +231,7 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_pfc_x(PfcXArgs<T> a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const T u = in[t + TL * e];
-    v[e] = cplx<T>{u, u * u * u};
+    v[e] = cplx<T>{u, a.nl(u)};
   }
   T *re = lds + c * RP;
   T *im = lds + LPB * RP + c * RP;
@@ -319,6 +333,7 @@ __global__ void k_pfc_operators(PfcOp o, int W, T *opL, T *opN) {
 // with opL/opN evaluated per mode on the fly (no operator tables in HBM).
 // ----------------------------------------------------------------------------
 template <typename T> struct PfcZArgs {
+  int zero_k0; // mean-split step: the k = 0 mode of the deviation stays 0
   cplx<T> *A;
   const cplx<T> *B;
   const cplx<T> *tw;
@@ -366,6 +381,7 @@ __global__ void __launch_bounds__(L *Plan::TL) k_pfc_z(PfcZArgs<T> a) {
 #pragma unroll
     for (int e = 0; e < E; ++e) acc[e] = c_add(acc[e], c_scale(v[e], opn[e]));
   }
+  if (a.zero_k0 && x == 0 && y == 0 && t == 0) acc[0] = cplx<T>{0, 0};
   LineFft<T, Plan, +1, SPLIT, true>::run(acc, t, ll, a.tw);
   if (active) {
 #pragma unroll

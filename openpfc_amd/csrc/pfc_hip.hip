@@ -105,6 +105,8 @@ struct pfc_hip_plan_s {
   int model_kind = -1;
   double dt = 0;
   PfcOp op{};
+  bool mean_split = false; // field holds psi - mean; mean kept in double here
+  double mean = 0.0;
   int var[ROLE_COUNT] = {};
   size_t bytes = 0;
 
@@ -177,6 +179,14 @@ int upload_line_tables(pfc_hip_plan_s *p, int n) {
 template <typename T> const cplx<T> *twp(pfc_hip_plan_s *p, int n, int pid) {
   auto it = p->tw.find(std::make_pair(n, pid));
   return it == p->tw.end() ? nullptr : (const cplx<T> *)it->second;
+}
+
+template <typename T> Nonlin<T> nonlin(const pfc_hip_plan_s *p) {
+  Nonlin<T> n{};
+  n.split = p->mean_split ? 1 : 0;
+  n.c1 = (T)(3.0 * p->mean);
+  n.c2 = (T)(3.0 * p->mean * p->mean);
+  return n;
 }
 
 void free_all(pfc_hip_plan_s *p) {
@@ -323,6 +333,7 @@ int step_natural(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
   {
     const auto &v = pfcx_variants<T>(p->nx)[p->var[ROLE_PFC_X]];
     PfcXArgs<T> a{};
+    a.nl = nonlin<T>(p);
     a.psi = (const T *)psi;
     a.outA = (cplx<T> *)p->A;
     a.outB = (cplx<T> *)p->B;
@@ -341,6 +352,7 @@ int step_natural(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
   {
     const auto &v = pfcz_variants<T>(p->nz)[p->var[ROLE_PFC_Z]];
     PfcZArgs<T> a{};
+    a.zero_k0 = p->mean_split ? 1 : 0;
     a.A = (cplx<T> *)p->A;
     a.B = (const cplx<T> *)p->B;
     a.tw = twp<T>(p, p->nz, v.plan_id);
@@ -395,6 +407,8 @@ int step_transposed(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev)
   a.B2 = (cplx<T> *)p->B2;
   a.wn = (const cplx<T> *)p->wn;
   a.op = p->op;
+  a.nl = nonlin<T>(p);
+  a.zero_k0 = p->mean_split ? 1 : 0;
   a.nx = p->nx;
   a.ny = p->ny;
   a.nz = p->nz;
@@ -827,7 +841,7 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
     if (he != hipSuccess) return fail(PFC_HIP_EDEVICE, hipGetErrorString(he));
     for (int k = 0; k < 5; ++k) {
       float ms = 0;
-      hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
       pass_ms[k] = ms;
     }
     return 0;
@@ -886,6 +900,14 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
     PFC_HIP_TRY(hipStreamSynchronize(s), "autotune sync");
   }
   return rc;
+}
+
+int pfc_hip_pfc_set_mean(pfc_hip_plan_t plan, int enable, double mean) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!std::isfinite(mean)) return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_set_mean: mean not finite");
+  plan->mean_split = enable != 0;
+  plan->mean = enable ? mean : 0.0;
+  return 0;
 }
 
 } // extern "C"

@@ -60,6 +60,8 @@ __device__ __forceinline__ void fetch_partner(const cplx<T> (&v)[E], cplx<T> (&p
 }
 
 template <typename T> struct TStepArgs {
+  Nonlin<T> nl;     // nonlinearity (plain or mean-split)
+  int zero_k0;      // mean-split step: keep the k = 0 mode of the deviation at 0
   const T *psi_in;  // k_tx_fwd input
   T *psi_out;       // k_tx_inv output
   cplx<T> *A1, *B1; // T1 buffers (Psi, N)
@@ -91,7 +93,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd(TStepArgs<T> a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const T u = row[t + TL * e];
-    v[e] = cplx<T>{u, u * u * u};
+    v[e] = cplx<T>{u, a.nl(u)};
   }
   T *re = lds + g * RP;
   T *im = lds + (SPLIT ? 0 : PL) + g * RP;
@@ -213,6 +215,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
 #pragma unroll
     for (int e = 0; e < E; ++e) acc[e] = c_add(acc[e], c_scale(v[e], opn[e]));
   }
+  if (a.zero_k0 && l == 0 && t == 0) acc[0] = cplx<T>{0, 0};
   LineFft<T, Plan, +1, SPLIT, true>::run(acc, t, ll, a.tw_z);
   if (active) {
 #pragma unroll

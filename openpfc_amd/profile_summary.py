@@ -1,0 +1,132 @@
+"""Summarise rocprofv3 CSV output of bench.py into profiles/ text.
+
+    python -m openpfc_amd.profile_summary --trace DIR [--fetch DIR --write DIR]
+        --cmd "<command that produced them>" --out profiles/NAME.md
+        [--traffic-json profiles/NAME.json]
+
+--trace   a `rocprofv3 --kernel-trace --stats -f csv` output directory
+--fetch   a `rocprofv3 --pmc FETCH_SIZE -f csv` output directory
+--write   a `rocprofv3 --pmc WRITE_SIZE -f csv` output directory
+
+FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what the same
+counter reads for the plain device-to-device copy of `bench.py
+--calibrate-copy` (the largest __amd_rocclr_copyBuffer dispatch: it reads and
+writes 4 GiB).  The scaled bytes per dispatch of every kernel go to the
+markdown table and, with --traffic-json, to a JSON file bench.py uses for
+roofline.traffic.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import json
+import os
+import re
+from collections import defaultdict
+
+COPY_BYTES = 4 << 30  # bench.py --calibrate-copy: 4 GiB read + 4 GiB written
+
+
+def _csv(d: str, suffix: str):
+    f = glob.glob(os.path.join(d, f"*{suffix}"))
+    if not f:
+        raise FileNotFoundError(f"no *{suffix} in {d}")
+    with open(f[0]) as fh:
+        return list(csv.DictReader(fh))
+
+
+def short_name(name: str) -> str:
+    """pfc_amd kernels -> 'k_tz<double, r16x16x4/E16, G8, split>' style names."""
+    m = re.match(r"void pfc_amd::(k_\w+)<(.*)>\(", name)
+    if not m:
+        return name.split("(")[0][:80]
+    args = m.group(2)
+    t = "double" if args.startswith("double") else ("float" if args.startswith("float") else "")
+    radices = re.search(r"RadixList<([\d, ]+)>", args)
+    npl = re.search(r"FftPlan<(\d+), (\d+)", args)
+    rest = re.sub(r".*RadixList<[\d, ]+> >,?\s*", "", args)
+    desc = []
+    if npl:
+        desc.append(f"N{npl.group(1)}/E{npl.group(2)}")
+    if radices:
+        desc.append("r" + radices.group(1).replace(", ", "x"))
+    if rest:
+        desc.append(rest.replace(" ", ""))
+    return f"{m.group(1)}<{t}{',' if desc else ''}{','.join(desc)}>"
+
+
+def kernel_stats(trace_dir: str):
+    rows = _csv(trace_dir, "kernel_stats.csv")
+    out = []
+    for r in rows:
+        out.append({"name": short_name(r["Name"]), "calls": int(r["Calls"]),
+                    "avg_ms": float(r["AverageNs"]) / 1e6,
+                    "min_ms": float(r["MinNs"]) / 1e6, "max_ms": float(r["MaxNs"]) / 1e6,
+                    "pct": float(r["Percentage"])})
+    return out
+
+
+def counter_means(pmc_dir: str):
+    rows = _csv(pmc_dir, "counter_collection.csv")
+    per = defaultdict(list)
+    copy = []
+    for r in rows:
+        v = float(r["Counter_Value"])
+        if r["Kernel_Name"].startswith("__amd_rocclr_copyBuffer"):
+            copy.append(v)
+        per[short_name(r["Kernel_Name"])].append(v)
+    return {k: sum(v) / len(v) for k, v in per.items()}, (max(copy) if copy else None)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--trace", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--cmd", required=True)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--traffic-json")
+    ap.add_argument("--title", default="rocprofv3 kernel summary")
+    a = ap.parse_args(argv)
+
+    stats = kernel_stats(a.trace)
+    lines = [f"# {a.title}", "", "Command (run from the repository root on one MI355X):", "",
+             "```", a.cmd, "```", "",
+             "| kernel | calls | avg ms | min ms | max ms | % time |",
+             "|---|---:|---:|---:|---:|---:|"]
+    for s in sorted(stats, key=lambda s: -s["pct"]):
+        if s["pct"] < 0.05:
+            continue
+        lines.append(f"| `{s['name']}` | {s['calls']} | {s['avg_ms']:.4f} | {s['min_ms']:.4f} "
+                     f"| {s['max_ms']:.4f} | {s['pct']:.2f} |")
+    traffic = {}
+    if a.fetch and a.write:
+        fm, fcopy = counter_means(a.fetch)
+        wm, wcopy = counter_means(a.write)
+        if not fcopy or not wcopy:
+            raise RuntimeError("no copy-kernel dispatch in the PMC runs (use --calibrate-copy)")
+        sf = COPY_BYTES / (fcopy * 1024.0)
+        sw = COPY_BYTES / (wcopy * 1024.0)
+        lines += ["", "## HBM traffic per dispatch (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)", "",
+                  f"Calibration: the 4 GiB device copy reads {fcopy:.0f} KiB (FETCH_SIZE) and "
+                  f"writes {wcopy:.0f} KiB (WRITE_SIZE), so FETCH_SIZE is scaled by {sf:.4f} "
+                  f"and WRITE_SIZE by {sw:.4f}.", "",
+                  "| kernel | read GB | write GB | total GB |", "|---|---:|---:|---:|"]
+        for k in sorted(set(fm) & set(wm)):
+            if not k.startswith("k_"):
+                continue
+            rb, wb = fm[k] * 1024 * sf, wm[k] * 1024 * sw
+            traffic[k] = {"read_bytes": rb, "write_bytes": wb, "traffic_bytes": rb + wb}
+            lines.append(f"| `{k}` | {rb / 1e9:.3f} | {wb / 1e9:.3f} | {(rb + wb) / 1e9:.3f} |")
+        if a.traffic_json:
+            with open(a.traffic_json, "w") as fh:
+                json.dump({"command": a.cmd, "scale": {"fetch": sf, "write": sw},
+                           "kernels": traffic}, fh, indent=1)
+    with open(a.out, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+if __name__ == "__main__":
+    main()

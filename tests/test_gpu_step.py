@@ -105,3 +105,48 @@ def test_step_before_operators_is_state_error(torch_gpu):
     with pytest.raises(P.PfcHipError) as ei:
         st.step(psi, 1)
     assert ei.value.code == 5
+
+
+@pytest.mark.parametrize("n,steps", [((64, 64, 64), 1000), ((256, 128, 128), 200)])
+def test_fp32_mixed_meets_north_star_bar(torch_gpu, n, steps):
+    """FP32 field and transforms, FP64 conserved mean (mean-split step):
+    relative L2 error <= 1e-4 against the FP64 oracle after the run, and the
+    mean exactly conserved."""
+    torch = torch_gpu
+    import openpfc_amd as P
+    psi0 = _psi0(n)
+    opL, opN = O.pfc_operators(n, (DX, DX, DX), EPS, DT)
+    ref = O.pfc_step(psi0, opL, opN, steps)
+    mean = float(psi0.mean())
+    f = P.HipFft(*n, precision=P.F32)
+    st = P.PfcStepper(f, (DX, DX, DX), EPS)
+    st.initialize(DT)
+    st.set_mean(mean)
+    d = torch.from_numpy(psi0 - mean).float().cuda()
+    st.step(d, steps)
+    got = mean + d.double().cpu().numpy()
+    l2 = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    print(f"fp32-mixed {n} x {steps}: rel L2 {l2:.3e}, rel max {_rel_max(got, ref):.3e}, "
+          f"mean drift {abs(got.mean() - mean):.2e}")
+    assert l2 <= 1e-4
+    assert abs(got.mean() - mean) <= 1e-9
+
+
+def test_fp64_mean_split_matches_plain(torch_gpu):
+    """In FP64 the mean-split step is the same step: the mean is restored
+    exactly and the field agrees with the oracle to the FP64 bar."""
+    torch = torch_gpu
+    import openpfc_amd as P
+    n = (256, 128, 128)
+    psi0 = _psi0(n)
+    opL, opN = O.pfc_operators(n, (DX, DX, DX), EPS, DT)
+    ref = O.pfc_step(psi0, opL, opN, 20)
+    mean = float(psi0.mean())
+    f = P.HipFft(*n)
+    st = P.PfcStepper(f, (DX, DX, DX), EPS)
+    st.initialize(DT)
+    st.set_mean(mean)
+    d = torch.from_numpy(psi0 - mean).cuda()
+    st.step(d, 20)
+    got = mean + d.cpu().numpy()
+    assert _rel_max(got, ref) <= 1e-10
