@@ -12,7 +12,9 @@ psi_hat = opL*F(psi) + opN*F(psi^3), psi = F^-1(psi_hat)
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
 Prints one JSON line.  `value` is FP64 steps/s with the field resident in
-HBM; the same workload in FP32 is reported under "fp32".  `roofline` is the
+HBM; the same workload in FP32-mixed mode (FP32 deviation field and
+transforms, FP64 conserved mean) is reported under "fp32_mixed" with its
+error against the FP64 run after the same steps.  `roofline` is the
 dominant kernel's algorithmic bytes per launch over its mean launch duration
 (HIP events on the stream the kernels run on).  `cpu_baseline` times the CPU
 oracle (the repository's C restatement of the reference path, OpenMP) on one
@@ -164,6 +166,34 @@ def copy_calibration(torch, nbytes=4 << 30, reps=5):
     return {"bytes_per_launch": 2 * nbytes, "ms": ms, "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
 
 
+def accuracy_run(P, torch, n, dx, dt, r, steps, every=100):
+    """Configuration 3: FP64 and FP32-mixed advanced in lockstep from the
+    same field; relative L2 / max error of FP32-mixed against FP64 every
+    `every` steps (the FP64 GPU step is held to 1e-10 of the CPU oracle)."""
+    fft64, st64, psi = setup(P, torch, n, P.F64, dx, dt, r)
+    fft32 = P.HipFft(n, n, n, precision=P.F32)
+    st32 = P.PfcStepper(fft32, (dx, dx, dx), r)
+    st32.initialize(dt)
+    mean0 = float(psi.mean().item())
+    d = (psi - mean0).float()
+    st32.set_mean(mean0)
+    rows = []
+    done = 0
+    while done < steps:
+        k = min(every, steps - done)
+        st64.step(psi, k)
+        st32.step(d, k)
+        done += k
+        diff = d.double() + mean0 - psi
+        l2 = (torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(psi)).item()
+        mx = (diff.abs().max() / psi.abs().max()).item()
+        drift = abs((d.double().mean() + mean0).item() - mean0)
+        rows.append({"step": done, "rel_l2": l2, "rel_max": mx, "mean_drift": drift})
+        log(f"accuracy step {done}: rel L2 {l2:.3e} rel max {mx:.3e}")
+    return {"metric": "fp32-mixed error vs fp64", "grid": [n, n, n], "steps": steps,
+            "rows": rows}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,6 +211,9 @@ def main(argv=None):
     ap.add_argument("--traffic", default=None,
                     help="traffic table from openpfc_amd.profile_summary (default: "
                          "profiles/traffic_pfc_step_<n>_f64.json when present)")
+    ap.add_argument("--accuracy", type=int, default=0,
+                    help="run FP64 and FP32-mixed in lockstep for this many steps and "
+                         "report the FP32-mixed error every 100 steps (no timing)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also time a device-to-device copy of 2 x 4 GiB (the plain "
                          "copy kernel whose PMC byte counts scale the traffic figure)")
@@ -197,10 +230,14 @@ def main(argv=None):
 
     torch.cuda.set_device(0)
     n, dx, dt, r = args.n, math.pi / 4, 0.5, -0.25
+    if args.accuracy:
+        print(json.dumps(accuracy_run(P, torch, n, dx, dt, r, args.accuracy)), flush=True)
+        return 0
 
     # ---- FP64 (value) -----------------------------------------------------
     log(f"fp64 {n}^3: setup")
     fft, st, psi = setup(P, torch, n, P.F64, dx, dt, r)
+    psi0 = None if args.no_fp32 else psi.clone()
     chosen64 = choose_variants(P, fft, st, psi, args, "fp64")
     if args.save_variants:
         with open(args.save_variants, "w") as fh:
@@ -209,7 +246,8 @@ def main(argv=None):
     kms = st.step_timed(psi, 3)
     finite = bool(torch.isfinite(psi).all().item())
     mean64 = psi.mean().item()
-    del psi, st, fft
+    psi64 = psi
+    del st, fft
     torch.cuda.empty_cache()
 
     pb = pass_bytes(n, 8)
@@ -257,22 +295,38 @@ def main(argv=None):
         "checks": {"finite": finite, "mean_after": mean64},
     }
 
-    # ---- FP32 (same workload) ---------------------------------------------
+    # ---- FP32-mixed (same workload, same step sequence) ---------------------
     log(f"fp64 done: {ms64:.3f} ms/step")
     if not args.no_fp32:
-        log("fp32: setup")
-        fft, st, psi = setup(P, torch, n, P.F32, dx, dt, r)
-        chosen32 = choose_variants(P, fft, st, psi, args, "fp32")
-        ms32, _ = time_steps(torch, st, psi, args.steps, args.warmup)
-        k32 = st.step_timed(psi, 3)
-        del psi, st, fft
+        log("fp32-mixed: setup")
+        fft = P.HipFft(n, n, n, precision=P.F32)
+        st = P.PfcStepper(fft, (dx, dx, dx), r)
+        st.initialize(dt)
+        mean0 = float(psi0.double().mean().item())
+        d = (psi0 - mean0).float()
+        del psi0
+        st.set_mean(mean0)
+        chosen32 = choose_variants(P, fft, st, d, args, "fp32")
+        ms32, _ = time_steps(torch, st, d, args.steps, args.warmup)
+        k32 = st.step_timed(d, 3)
+        got = d.double() + mean0
+        diff = got - psi64
+        l2 = (torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(psi64)).item()
+        mx = (diff.abs().max() / psi64.abs().max()).item()
+        drift = abs(got.mean().item() - mean0)
+        del d, got, diff, st, fft
         torch.cuda.empty_cache()
-        out["fp32"] = {"mode": "fp32 storage and arithmetic (plain)",
-                       "steps_per_s": 1000.0 / ms32, "ms_per_step": ms32,
-                       "ns_per_dof_step": ms32 * 1e6 / n ** 3,
-                       "speedup_vs_fp64": ms64 / ms32,
-                       "step_hbm_frac": sum(pass_bytes(n, 4).values()) / (ms32 * 1e-3) / HBM_PEAK,
-                       "kernels_ms": dict(zip(PASS_NAMES, k32)), "variants": chosen32}
+        out["fp32_mixed"] = {
+            "mode": "fp32 field and transforms; fp64 conserved mean (mean-split step, k=0 pinned)",
+            "steps_per_s": 1000.0 / ms32, "ms_per_step": ms32,
+            "ns_per_dof_step": ms32 * 1e6 / n ** 3,
+            "speedup_vs_fp64": ms64 / ms32,
+            "step_hbm_frac": sum(pass_bytes(n, 4).values()) / (ms32 * 1e-3) / HBM_PEAK,
+            "kernels_ms": dict(zip(PASS_NAMES, k32)), "variants": chosen32,
+            "error_vs_fp64_after": {"steps": args.warmup + args.steps + 3,
+                                    "rel_l2": l2, "rel_max": mx, "mean_drift": drift}}
+    del psi64
+    torch.cuda.empty_cache()
 
     # ---- CPU baseline (oracle, rank 0, N=1) --------------------------------
     if not args.no_cpu_baseline:
