@@ -72,11 +72,10 @@ def variant_key(kernel, prec, n, variant, direction=None):
 
 def pass_bytes(n: int, esize: int):
     """Algorithmic HBM bytes per launch of each pass: one read and one write of
-    every array the pass streams (opL/opN tables included, as they are read)."""
+    every array the pass streams (opL/opN are computed per mode, not read)."""
     R = esize * n ** 3                     # one real field
     C = 2 * esize * (n // 2 + 1) * n * n   # one half spectrum
-    T = esize * (n // 2 + 1) * n * n       # one operator table
-    return {"x_fwd_pfc": R + 2 * C, "y_fwd": 4 * C, "z_fused": 3 * C + 2 * T,
+    return {"x_fwd_pfc": R + 2 * C, "y_fwd": 4 * C, "z_fused": 3 * C,
             "y_inv": 2 * C, "x_inv_c2r": C + R}
 
 

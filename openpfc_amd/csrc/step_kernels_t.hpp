@@ -77,7 +77,7 @@ template <typename T> struct TStepArgs {
 // ----------------------------------------------------------------------------
 // psi -> T1: G consecutive rows y0..y0+G-1 of plane z per block.
 // ----------------------------------------------------------------------------
-template <typename T, typename Plan, int G, bool SPLIT>
+template <typename T, typename Plan, int G, bool SPLIT, bool TILE>
 __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd(TStepArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
   constexpr int PL = t_plane<Plan, G>();
@@ -116,6 +116,18 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd(TStepArgs<T> a) {
     a.A1[o] = v[E / 2];
     a.B1[o] = pz[E / 2];
   }
+  if constexpr (!TILE) {
+    // direct strided stores: the G lines of the block fill each chunk
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < E / 2; ++r) {
+        const long o = ((long)z * a.W + TL * r + t) * a.ny + y;
+        a.A1[o] = v[r];
+        a.B1[o] = pz[r];
+      }
+    }
+    return;
+  }
   // transposed store of kx = TL*r + j, j < TL, r < E/2, through a TL x G tile
   T *tr = lds, *ti = lds + TL * (G + 1);
 #pragma unroll
@@ -140,7 +152,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd(TStepArgs<T> a) {
 // ----------------------------------------------------------------------------
 // T1 -> T2 forward y: block = (field, kx, z0..z0+G-1).
 // ----------------------------------------------------------------------------
-template <typename T, typename Plan, int G, bool SPLIT>
+template <typename T, typename Plan, int G, bool SPLIT, bool TILE>
 __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd(TStepArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
   constexpr int PL = t_plane<Plan, G>();
@@ -159,6 +171,13 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd(TStepArgs<T> a) {
   for (int e = 0; e < E; ++e) v[e] = line[t + TL * e];
   const LdsLine<T> ll{lds, g * RP, 1, PL};
   LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_y);
+  if constexpr (!TILE) {
+    if (active) {
+#pragma unroll
+      for (int r = 0; r < E; ++r) out[((long)kx * N + TL * r + t) * a.nz + z] = v[r];
+    }
+    return;
+  }
   // transposed store: T2[(kx*ny + ky)*nz + z], ky = TL*r + j
   T *tr = lds, *ti = lds + TL * (G + 1);
 #pragma unroll
@@ -192,29 +211,31 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
   const int kx = (int)(l / a.ny), ky = (int)(l % a.ny);
   const LdsLine<T> ll{lds, g * RP, 1, PL};
 
-  cplx<T> acc[E];
-  cplx<T> *__restrict__ la = a.A2 + l * N;
-#pragma unroll
-  for (int e = 0; e < E; ++e) acc[e] = la[t + TL * e];
-  LineFft<T, Plan, -1, SPLIT, true>::run(acc, t, ll, a.tw_z);
+  // operators first, while nothing else is live (exp and division need
+  // many temporaries); then both lines are loaded together
   const double kxy = pfc_kxy(a.op, kx, ky);
-  T opn[E];
+  T opl[E], opn[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     double oL, oN;
     pfc_op_mode(a.op, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), oL, oN);
-    acc[e] = c_scale(acc[e], (T)oL);
+    opl[e] = (T)oL;
     opn[e] = (T)oN;
   }
-  {
-    const cplx<T> *__restrict__ lb = a.B2 + l * N;
-    cplx<T> v[E];
+  cplx<T> *__restrict__ la = a.A2 + l * N;
+  const cplx<T> *__restrict__ lb = a.B2 + l * N;
+  cplx<T> acc[E], v[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = lb[t + TL * e];
-    LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_z);
-#pragma unroll
-    for (int e = 0; e < E; ++e) acc[e] = c_add(acc[e], c_scale(v[e], opn[e]));
+  for (int e = 0; e < E; ++e) {
+    acc[e] = la[t + TL * e];
+    v[e] = lb[t + TL * e];
   }
+  LineFft<T, Plan, -1, SPLIT, true>::run(acc, t, ll, a.tw_z);
+#pragma unroll
+  for (int e = 0; e < E; ++e) acc[e] = c_scale(acc[e], opl[e]);
+  LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_z);
+#pragma unroll
+  for (int e = 0; e < E; ++e) acc[e] = c_add(acc[e], c_scale(v[e], opn[e]));
   if (a.zero_k0 && l == 0 && t == 0) acc[0] = cplx<T>{0, 0};
   LineFft<T, Plan, +1, SPLIT, true>::run(acc, t, ll, a.tw_z);
   if (active) {
@@ -226,7 +247,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
 // ----------------------------------------------------------------------------
 // T2 -> T1 inverse y: block = (kx, z0..z0+G-1); transposed load.
 // ----------------------------------------------------------------------------
-template <typename T, typename Plan, int G, bool SPLIT>
+template <typename T, typename Plan, int G, bool SPLIT, bool TILE>
 __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv(TStepArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
   constexpr int PL = t_plane<Plan, G>();
@@ -237,9 +258,14 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv(TStepArgs<T> a) {
   const int z = z0 + g;
 
   cplx<T> v[E];
+  if constexpr (!TILE) {
+    const int zz = z < a.nz ? z : z0;
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r] = a.A2[((long)kx * N + TL * r + t) * a.nz + zz];
+  }
   T *tr = lds, *ti = lds + TL * (G + 1);
 #pragma unroll
-  for (int r = 0; r < E; ++r) {
+  for (int r = 0; r < (TILE ? E : 0); ++r) {
     const int j = threadIdx.x / G, gg = threadIdx.x % G;
     cplx<T> w{0, 0};
     if (z0 + gg < a.nz) w = a.A2[((long)kx * N + TL * r + j) * a.nz + z0 + gg];
@@ -264,23 +290,42 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv(TStepArgs<T> a) {
 // nx/2-point inverse.  Imaginary parts of X[0] and X[nx/2] are ignored.
 // Plan is the M = nx/2 point plan.
 // ----------------------------------------------------------------------------
-template <typename T, typename Plan, int G> constexpr int tx_inv_lds() {
-  constexpr int stage = 2 * (Plan::N + 1) * (G + 1);
+template <typename T, typename Plan, int G, bool TILE> constexpr int tx_inv_lds() {
+  constexpr int stage = TILE ? 2 * (Plan::N + 1) * (G + 1) : 0;
   constexpr int fft = 2 * t_plane<Plan, G>();
   return stage > fft ? stage : fft;
 }
 
-template <typename T, typename Plan, int G>
+template <typename T, typename Plan, int G, bool TILE>
 __global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N, RP = row_padded(M);
   constexpr int PL = t_plane<Plan, G>();
   constexpr int W = M + 1;
-  __shared__ T lds[tx_inv_lds<T, Plan, G>()];
+  __shared__ T lds[tx_inv_lds<T, Plan, G, TILE>()];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngy = (a.ny + G - 1) / G;
   const int z = blockIdx.x / ngy, y0 = (blockIdx.x % ngy) * G;
   const int y = y0 + g;
 
+  cplx<T> v[E];
+  if constexpr (!TILE) {
+    // direct strided loads of X[k] and X[M-k] of this line
+    const int yy = y < a.ny ? y : y0;
+    const cplx<T> *__restrict__ col = a.A1 + (long)z * a.W * a.ny + yy;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = t + TL * e;
+      cplx<T> xa = col[(long)k * a.ny];
+      cplx<T> xb = c_conj(col[(long)(M - k) * a.ny]);
+      if (k == 0) {
+        xa.y = (T)0;
+        xb.y = (T)0;
+      }
+      const cplx<T> ev = c_add(xa, xb);
+      const cplx<T> od = c_mulc(c_sub(xa, xb), a.wn[k]);
+      v[e] = cplx<T>{ev.x - od.y, ev.y + od.x};
+    }
+  } else {
   // stage: sr/si[kx*(G+1) + line]
   T *sr = lds, *si = lds + W * (G + 1);
   for (int i = threadIdx.x; i < W * G; i += G * TL) {
@@ -291,7 +336,6 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
     si[tile_idx<G>(kx, gg)] = w.y;
   }
   __syncthreads();
-  cplx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = t + TL * e;
@@ -306,6 +350,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
     v[e] = cplx<T>{ev.x - od.y, ev.y + od.x};
   }
   __syncthreads(); // the staging area becomes the FFT exchange area
+  }
   const LdsLine<T> ll{lds, g * RP, 1, PL};
   LineFft<T, Plan, +1, false, true>::run(v, t, ll, a.tw_xm);
   if (y < a.ny) {

@@ -236,72 +236,82 @@ template <typename T, int N> std::vector<PXV<T>> make_pfcx() {
 }
 
 // ---- transposed-step kernels ----------------------------------------------------
-template <typename T, typename Plan, int G, bool S>
+template <typename T, typename Plan, int G, bool S, bool TILE>
 hipError_t tx_fwd_launch(TStepArgs<T> a, hipStream_t s) {
   dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
-  hipLaunchKernelGGL((k_tx_fwd<T, Plan, G, S>), grid, block, 0, s, a);
+  hipLaunchKernelGGL((k_tx_fwd<T, Plan, G, S, TILE>), grid, block, 0, s, a);
   return hipGetLastError();
 }
-template <typename T, typename Plan, int G, bool S>
+template <typename T, typename Plan, int G, bool S, bool TILE>
 hipError_t ty_fwd_launch(TStepArgs<T> a, hipStream_t s) {
   dim3 grid((unsigned)(a.W * ((a.nz + G - 1) / G)), 2), block(G * Plan::TL);
-  hipLaunchKernelGGL((k_ty_fwd<T, Plan, G, S>), grid, block, 0, s, a);
+  hipLaunchKernelGGL((k_ty_fwd<T, Plan, G, S, TILE>), grid, block, 0, s, a);
   return hipGetLastError();
 }
-template <typename T, typename Plan, int G, bool S>
+template <typename T, typename Plan, int G, bool S, bool TILE>
 hipError_t tz_launch(TStepArgs<T> a, hipStream_t s) {
   const long lines = (long)a.W * a.ny;
   dim3 grid((unsigned)((lines + G - 1) / G)), block(G * Plan::TL);
   hipLaunchKernelGGL((k_tz<T, Plan, G, S>), grid, block, 0, s, a);
   return hipGetLastError();
 }
-template <typename T, typename Plan, int G, bool S>
+template <typename T, typename Plan, int G, bool S, bool TILE>
 hipError_t ty_inv_launch(TStepArgs<T> a, hipStream_t s) {
   dim3 grid((unsigned)(a.W * ((a.nz + G - 1) / G))), block(G * Plan::TL);
-  hipLaunchKernelGGL((k_ty_inv<T, Plan, G, S>), grid, block, 0, s, a);
+  hipLaunchKernelGGL((k_ty_inv<T, Plan, G, S, TILE>), grid, block, 0, s, a);
   return hipGetLastError();
 }
-template <typename T, typename Plan, int G, bool S>
+template <typename T, typename Plan, int G, bool S, bool TILE>
 hipError_t tx_inv_launch(TStepArgs<T> a, hipStream_t s) {
   dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
-  hipLaunchKernelGGL((k_tx_inv<T, Plan, G>), grid, block, 0, s, a);
+  hipLaunchKernelGGL((k_tx_inv<T, Plan, G, TILE>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 
-template <typename T, typename Plan, int G, bool S, int KIND> constexpr bool t_fits() {
+template <typename T, typename Plan, int G, bool S, bool TILE, int KIND>
+constexpr bool t_fits() {
   if (G * Plan::TL > kMaxThreads || G * Plan::TL < 64) return false;
+  if (KIND == TK_Z && !TILE) return false; // the z pass has no transposition
   constexpr int fft = (S ? 1 : 2) * G * row_padded(Plan::N) * (int)sizeof(T);
   if (KIND == TK_X_INV) {
-    constexpr int stage = 2 * (Plan::N + 1) * (G + 1) * (int)sizeof(T);
+    constexpr int stage = TILE ? 2 * (Plan::N + 1) * (G + 1) * (int)sizeof(T) : 0;
     constexpr int full = 2 * G * row_padded(Plan::N) * (int)sizeof(T);
     return !S && (stage > full ? stage : full) <= kMaxLds;
   }
   return fft <= kMaxLds;
 }
 
-template <typename T, typename Plan, int G, bool S, int KIND>
+template <typename T, typename Plan, int G, bool S, bool TILE, int KIND>
 void add_t(std::vector<TV<T>> &v, int pid) {
-  if constexpr (t_fits<T, Plan, G, S, KIND>()) {
+  if constexpr (t_fits<T, Plan, G, S, TILE, KIND>()) {
     const std::string nm = plan_name<Plan>() + "/G" + std::to_string(G) +
-                           (KIND == TK_X_INV ? "" : (S ? "/split" : "/full"));
-    if constexpr (KIND == TK_X_FWD) v.push_back({nm, pid, &tx_fwd_launch<T, Plan, G, S>});
-    if constexpr (KIND == TK_Y_FWD) v.push_back({nm, pid, &ty_fwd_launch<T, Plan, G, S>});
-    if constexpr (KIND == TK_Z) v.push_back({nm, pid, &tz_launch<T, Plan, G, S>});
-    if constexpr (KIND == TK_Y_INV) v.push_back({nm, pid, &ty_inv_launch<T, Plan, G, S>});
-    if constexpr (KIND == TK_X_INV) v.push_back({nm, pid, &tx_inv_launch<T, Plan, G, S>});
+                           (KIND == TK_X_INV ? "" : (S ? "/split" : "/full")) +
+                           (TILE ? "" : "/direct");
+    if constexpr (KIND == TK_X_FWD)
+      v.push_back({nm, pid, &tx_fwd_launch<T, Plan, G, S, TILE>});
+    if constexpr (KIND == TK_Y_FWD)
+      v.push_back({nm, pid, &ty_fwd_launch<T, Plan, G, S, TILE>});
+    if constexpr (KIND == TK_Z) v.push_back({nm, pid, &tz_launch<T, Plan, G, S, TILE>});
+    if constexpr (KIND == TK_Y_INV)
+      v.push_back({nm, pid, &ty_inv_launch<T, Plan, G, S, TILE>});
+    if constexpr (KIND == TK_X_INV)
+      v.push_back({nm, pid, &tx_inv_launch<T, Plan, G, S, TILE>});
   }
 }
 
 // Lines per block giving 128-byte chunks first (8 in double, 16 in float),
-// then half and double that.
+// then half and double that; LDS-tiled and direct transpositions.
 template <typename T, typename Plan, int KIND>
 void t_for_plan(std::vector<TV<T>> &v, int pid) {
   constexpr int G0 = 128 / (2 * (int)sizeof(T));
-  add_t<T, Plan, G0, true, KIND>(v, pid);
-  add_t<T, Plan, G0, false, KIND>(v, pid);
-  add_t<T, Plan, G0 / 2, true, KIND>(v, pid);
-  add_t<T, Plan, G0 / 2, false, KIND>(v, pid);
-  add_t<T, Plan, 2 * G0, true, KIND>(v, pid);
+  add_t<T, Plan, G0, true, true, KIND>(v, pid);
+  add_t<T, Plan, G0, false, true, KIND>(v, pid);
+  add_t<T, Plan, G0 / 2, true, true, KIND>(v, pid);
+  add_t<T, Plan, G0 / 2, false, true, KIND>(v, pid);
+  add_t<T, Plan, 2 * G0, true, true, KIND>(v, pid);
+  add_t<T, Plan, G0, true, false, KIND>(v, pid);
+  add_t<T, Plan, G0, false, false, KIND>(v, pid);
+  add_t<T, Plan, G0 / 2, false, false, KIND>(v, pid);
 }
 
 template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
