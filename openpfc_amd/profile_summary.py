@@ -10,8 +10,10 @@
 
 FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what the same
 counter reads for the plain device-to-device copy of `bench.py
---calibrate-copy` (the largest __amd_rocclr_copyBuffer dispatch: it reads and
-writes 4 GiB).  The scaled bytes per dispatch of every kernel go to the
+--calibrate-copy`, which reads and writes 4 GiB per dispatch and runs last:
+the calibration value is the median over the trailing __amd_rocclr_copyBuffer
+dispatches with the grid size of the last one (other copies, such as the
+autotuner's backup of the field, come earlier and differ in size).  The scaled bytes per dispatch of every kernel go to the
 markdown table and, with --traffic-json, to a JSON file bench.py uses for
 roofline.traffic.
 """
@@ -74,9 +76,20 @@ def counter_means(pmc_dir: str):
     for r in rows:
         v = float(r["Counter_Value"])
         if r["Kernel_Name"].startswith("__amd_rocclr_copyBuffer"):
-            copy.append(v)
+            copy.append((int(r["Dispatch_Id"]), r["Grid_Size"], v))
         per[short_name(r["Kernel_Name"])].append(v)
-    return {k: sum(v) / len(v) for k, v in per.items()}, (max(copy) if copy else None)
+    cal = None
+    if copy:
+        copy.sort()
+        grid = copy[-1][1]
+        tail = []
+        for _, g, v in reversed(copy):
+            if g != grid:
+                break
+            tail.append(v)
+        tail.sort()
+        cal = tail[len(tail) // 2]
+    return {k: sum(v) / len(v) for k, v in per.items()}, cal
 
 
 def main(argv=None):
@@ -88,6 +101,8 @@ def main(argv=None):
     ap.add_argument("--out", required=True)
     ap.add_argument("--traffic-json")
     ap.add_argument("--title", default="rocprofv3 kernel summary")
+    ap.add_argument("--min-pct", type=float, default=0.5,
+                    help="list kernels with at least this share of GPU time")
     a = ap.parse_args(argv)
 
     stats = kernel_stats(a.trace)
@@ -95,9 +110,11 @@ def main(argv=None):
              "```", a.cmd, "```", "",
              "| kernel | calls | avg ms | min ms | max ms | % time |",
              "|---|---:|---:|---:|---:|---:|"]
+    shown = set()
     for s in sorted(stats, key=lambda s: -s["pct"]):
-        if s["pct"] < 0.05:
+        if s["pct"] < a.min_pct:
             continue
+        shown.add(s["name"])
         lines.append(f"| `{s['name']}` | {s['calls']} | {s['avg_ms']:.4f} | {s['min_ms']:.4f} "
                      f"| {s['max_ms']:.4f} | {s['pct']:.2f} |")
     traffic = {}
@@ -118,7 +135,9 @@ def main(argv=None):
                 continue
             rb, wb = fm[k] * 1024 * sf, wm[k] * 1024 * sw
             traffic[k] = {"read_bytes": rb, "write_bytes": wb, "traffic_bytes": rb + wb}
-            lines.append(f"| `{k}` | {rb / 1e9:.3f} | {wb / 1e9:.3f} | {(rb + wb) / 1e9:.3f} |")
+            if k in shown:
+                lines.append(f"| `{k}` | {rb / 1e9:.3f} | {wb / 1e9:.3f} | "
+                             f"{(rb + wb) / 1e9:.3f} |")
         if a.traffic_json:
             with open(a.traffic_json, "w") as fh:
                 json.dump({"command": a.cmd, "scale": {"fetch": sf, "write": sw},
