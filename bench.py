@@ -10,6 +10,7 @@ psi_hat = opL*F(psi) + opN*F(psi^3), psi = F^-1(psi_hat)
 (docs/api/examples/10_complete_pfc_model.cpp:121-144).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --model tungsten     # Tungsten mean-field step (see below)
 
 Prints one JSON line.  `value` is FP64 steps/s with the field resident in
 HBM; the same workload in FP32-mixed mode (FP32 deviation field and
@@ -19,6 +20,12 @@ dominant kernel's algorithmic bytes per launch over its mean launch duration
 (HIP events on the stream the kernels run on).  `cpu_baseline` times the CPU
 oracle (the repository's C restatement of the reference path, OpenMP) on one
 step of the same 1024^3 workload.
+
+--model tungsten times the Tungsten mean-field step (TungstenHIP drop-in) on
+the reference's scalability workload (apps/tungsten/src/tungsten_scalability.cpp:
+95-135): spacing 1, n0 = -0.4, T = 0.5 (other parameters at their defaults),
+dt = 0.01, psi = -0.4 + 0.1 sin(2 pi i/N^3), 3 warm-up steps, FP64 with the
+FP32 mode (TungstenHIP<float>) beside it.
 """
 from __future__ import annotations
 
@@ -47,16 +54,29 @@ PASS_KERNELS = {
                 ("k_col", "col_y"), ("k_x_c2r", "x_c2r")]}
 STEP_ROLES = ["step_layout", "pfc_x", "col_y", "pfc_z", "x_c2r", "t_x_fwd", "t_y_fwd",
               "t_z", "t_y_inv", "t_x_inv"]
+TUNG_METRIC = "Tungsten mean-field step, steps/s at 1024^3 FP64 (% of HBM peak)"
+TUNG_PASS_NAMES = ["x_fwd_r2c", "y_fwd", "z_meanfield", "y_inv_mf", "x_fused_nl",
+                   "y_fwd_n", "z_combine", "y_inv", "x_inv_c2r"]
+TUNG_PASS_KERNELS = {
+    "transposed": [("k_tx_r2c", "t_x_r2c"), ("k_ty_fwd", "t_y_fwd"), ("k_tz", "t_z"),
+                   ("k_ty_inv", "t_y_inv"), ("k_tx_tung", "t_x_tung"), ("k_ty_fwd", "t_y_fwd"),
+                   ("k_tz", "t_z"), ("k_ty_inv", "t_y_inv"), ("k_tx_inv", "t_x_inv")],
+    "natural": [("k_x_r2c", "x_r2c"), ("k_col", "col_y"), ("k_pfc_z", "pfc_z"),
+                ("k_col", "col_y"), ("k_x_tung", "x_tung"), ("k_col", "col_y"),
+                ("k_pfc_z", "pfc_z"), ("k_col", "col_y"), ("k_x_c2r", "x_c2r")]}
+TUNG_STEP_ROLES = ["step_layout", "x_r2c", "col_y", "pfc_z", "x_tung", "x_c2r", "t_x_r2c",
+                   "t_y_fwd", "t_z", "t_y_inv", "t_x_tung", "t_x_inv"]
 
 
-def variant_key(kernel, prec, n, variant, direction=None):
-    """rocprof short name (openpfc_amd.profile_summary.short_name) of a variant."""
+def variant_key(kernel, prec, n, variant, direction=None, mode=0):
+    """rocprof short name (openpfc_amd.profile_summary.short_name) of a variant;
+    `mode` is the z-pass kind (0 PFC, 1 Tungsten mean field, 2 Tungsten combine)."""
     import re
     parts = variant.split("/")
     radices = parts[0][1:]
     e = parts[1][1:]
     nn = n
-    if kernel in ("k_tx_inv", "k_x_c2r"):
+    if kernel in ("k_tx_inv", "k_x_c2r", "k_x_r2c", "k_tx_r2c", "k_x_tung", "k_tx_tung"):
         nn = n // 2
     rest = []
     for p in parts[2:]:
@@ -66,6 +86,10 @@ def variant_key(kernel, prec, n, variant, direction=None):
             rest.append("true" if p == "split" else "false")
     if kernel == "k_col":
         rest = [str(direction)] + rest
+    if kernel in ("k_tz", "k_pfc_z"):
+        rest.append(str(mode))
+    elif kernel.startswith("k_t"):
+        rest.append("false" if "direct" in parts else "true")
     t = "double" if prec == "f64" else "float"
     return f"{kernel}<{t},N{nn}/E{e},r{radices}" + ("," + ",".join(rest) if rest else "") + ">"
 
@@ -77,6 +101,18 @@ def pass_bytes(n: int, esize: int):
     C = 2 * esize * (n // 2 + 1) * n * n   # one half spectrum
     return {"x_fwd_pfc": R + 2 * C, "y_fwd": 4 * C, "z_fused": 3 * C,
             "y_inv": 2 * C, "x_inv_c2r": C + R}
+
+
+def tung_pass_bytes(n: int, esize: int):
+    """Algorithmic bytes per launch of the nine Tungsten passes (3R + 18C in
+    all): psi is read by the forward x pass and again by the fused x pass, the
+    first z pass writes the full spectrum of psi and the mean field, the
+    second reads both spectra; the multipliers are computed per mode."""
+    R = esize * n ** 3
+    C = 2 * esize * (n // 2 + 1) * n * n
+    return {"x_fwd_r2c": R + C, "y_fwd": 2 * C, "z_meanfield": 3 * C, "y_inv_mf": 2 * C,
+            "x_fused_nl": R + 2 * C, "y_fwd_n": 2 * C, "z_combine": 3 * C, "y_inv": 2 * C,
+            "x_inv_c2r": C + R}
 
 
 def log(msg):
@@ -97,13 +133,13 @@ def setup(P, torch, n, prec, dx, dt, r):
     return fft, st, psi
 
 
-def choose_variants(P, fft, st, psi, args, log_prefix):
+def choose_variants(P, fft, st, psi, args, log_prefix, roles=STEP_ROLES):
     """Autotune (untimed) or apply forced variants; returns {role: name}."""
     import json as _json
     if args.variants:
         with open(args.variants) as fh:
             want = _json.load(fh)
-        for role in STEP_ROLES:
+        for role in roles:
             if role in want:
                 names = fft.variants(role)
                 if want[role] in names:
@@ -112,7 +148,7 @@ def choose_variants(P, fft, st, psi, args, log_prefix):
         t0 = time.perf_counter()
         st.autotune(psi, reps=2)
         log(f"{log_prefix} autotune {time.perf_counter() - t0:.1f} s")
-    return {r: fft.variants(r)[fft.get_variant(r)] for r in STEP_ROLES if fft.variants(r)}
+    return {r: fft.variants(r)[fft.get_variant(r)] for r in roles if fft.variants(r)}
 
 
 def time_steps(torch, st, psi, steps, warmup):
@@ -193,9 +229,125 @@ def accuracy_run(P, torch, n, dx, dt, r, steps, every=100):
             "rows": rows}
 
 
+def tung_setup(P, torch, n, prec):
+    """tungsten_scalability.cpp:95-135: spacing 1, n0 = -0.4, T = 0.5, dt = 0.01,
+    psi = -0.4 + 0.1 sin(2 pi i / N^3)."""
+    fft = P.HipFft(n, n, n, precision=prec)
+    st = P.TungstenStepper(fft, (1.0, 1.0, 1.0), P.TungstenParams(n0=-0.4, T=0.5))
+    st.initialize(0.01)
+    i = torch.arange(n ** 3, dtype=torch.float64, device="cuda")
+    psi = -0.4 + 0.1 * torch.sin(2.0 * math.pi * i / float(n ** 3))
+    del i
+    if prec == P.F32:
+        psi = psi.float()
+    torch.cuda.synchronize()
+    return fft, st, psi
+
+
+def tung_cpu_baseline(n, nsteps):
+    import numpy as np
+    from oracle import oracle as O
+
+    import openpfc_amd.tungsten as T
+    prm = T.TungstenParams(n0=-0.4, T=0.5)
+    N = n ** 3
+    a = (-0.4 + 0.1 * np.sin(2.0 * math.pi * np.arange(N, dtype=np.float64) / N))
+    a = a.reshape(n, n, n)
+    fMF, expL, nW = O.tungsten_operators((n, n, n), (1.0, 1.0, 1.0), prm.operator_params(),
+                                         0.01)
+    t0 = time.perf_counter()
+    O.tungsten_step(a, fMF, expL, nW, prm.nonlinear_params(), nsteps)
+    return nsteps / (time.perf_counter() - t0), O.num_threads()
+
+
+def tungsten_main(args, P, torch):
+    n = args.n
+    log(f"tungsten fp64 {n}^3: setup")
+    fft, st, psi = tung_setup(P, torch, n, P.F64)
+    psi0 = psi.clone() if not args.no_fp32 else None
+    chosen = choose_variants(P, fft, st, psi, args, "tungsten fp64", TUNG_STEP_ROLES)
+    if args.save_variants:
+        with open(args.save_variants, "w") as fh:
+            json.dump(chosen, fh, indent=1)
+    ms, wall = time_steps(torch, st, psi, args.steps, args.warmup)
+    kms = st.step_timed(psi, 3)
+    finite = bool(torch.isfinite(psi).all().item())
+    del st, fft
+    torch.cuda.empty_cache()
+    pb = tung_pass_bytes(n, 8)
+    dom = max(range(9), key=lambda i: kms[i])
+    dname = TUNG_PASS_NAMES[dom]
+    achieved = pb[dname] / (kms[dom] * 1e-3)
+    step_bytes = sum(pb.values())
+    layout = chosen.get("step_layout", "natural")
+    kern, role = TUNG_PASS_KERNELS[layout][dom]
+    kkey = variant_key(kern, "f64", n, chosen[role],
+                       direction=(-1 if dom in (1, 5) else 1),
+                       mode={2: 1, 6: 2}.get(dom, 0))
+    traffic = None
+    tpath = args.traffic or os.path.join(ROOT, "profiles", f"traffic_tungsten_step_{n}_f64.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tk = json.load(fh).get("kernels", {})
+        if kkey in tk:
+            traffic = tk[kkey]["traffic_bytes"]
+    out = {
+        "metric": TUNG_METRIC, "value": 1000.0 / ms, "unit": "steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: -0.4 + 0.1 sin(2 pi i/N^3) (tungsten_scalability workload)",
+        "config": {"workload": f"Tungsten mean-field ETD1 step {n}^3 FP64", "grid": [n, n, n],
+                   "spacing": 1.0, "dt": 0.01, "n0": -0.4, "T": 0.5},
+        "ns_per_dof_step": ms * 1e6 / n ** 3,
+        "step_hbm_frac": step_bytes / (ms * 1e-3) / HBM_PEAK,
+        "step_alg_bytes": step_bytes, "wall_s_timed": wall,
+        "kernels_ms": dict(zip(TUNG_PASS_NAMES, kms)),
+        "roofline": {"bound": "hbm", "kernel": kkey, "pass": dname,
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "alg_bytes_per_launch": pb[dname]},
+        "variants": chosen, "checks": {"finite": finite}}
+    log(f"tungsten fp64 done: {ms:.3f} ms/step")
+    if not args.no_fp32:
+        fft = P.HipFft(n, n, n, precision=P.F32)
+        st = P.TungstenStepper(fft, (1.0, 1.0, 1.0), P.TungstenParams(n0=-0.4, T=0.5))
+        st.initialize(0.01)
+        d = psi0.float()
+        del psi0
+        chosen32 = choose_variants(P, fft, st, d, args, "tungsten fp32", TUNG_STEP_ROLES)
+        ms32, _ = time_steps(torch, st, d, args.steps, args.warmup)
+        k32 = st.step_timed(d, 3)
+        diff = d.double() - psi
+        l2 = (torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(psi)).item()
+        del d, diff, st, fft
+        torch.cuda.empty_cache()
+        out["fp32"] = {"mode": "TungstenHIP<float>: fp32 field, transforms and multipliers",
+                       "steps_per_s": 1000.0 / ms32, "ms_per_step": ms32,
+                       "speedup_vs_fp64": ms / ms32,
+                       "step_hbm_frac": sum(tung_pass_bytes(n, 4).values()) / (ms32 * 1e-3)
+                       / HBM_PEAK,
+                       "kernels_ms": dict(zip(TUNG_PASS_NAMES, k32)), "variants": chosen32,
+                       "rel_l2_vs_fp64_after": {"steps": args.warmup + args.steps + 3,
+                                                "rel_l2": l2}}
+    del psi
+    torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
+        log("tungsten cpu baseline (oracle) start")
+        sps, threads = tung_cpu_baseline(n, args.cpu_steps)
+        out["cpu_baseline"] = {
+            "value": sps, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{args.cpu_steps} step(s) of the same {n}^3 FP64 workload, oracle C "
+                      f"restatement (own Stockham FFT, OpenMP {threads} threads)"}
+    if args.calibrate_copy:
+        out["copy_calibration"] = copy_calibration(torch)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--model", choices=["pfc", "tungsten"], default="pfc")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1024)
@@ -228,6 +380,8 @@ def main(argv=None):
     import openpfc_amd as P
 
     torch.cuda.set_device(0)
+    if args.model == "tungsten":
+        return tungsten_main(args, P, torch)
     n, dx, dt, r = args.n, math.pi / 4, 0.5, -0.25
     if args.accuracy:
         print(json.dumps(accuracy_run(P, torch, n, dx, dt, r, args.accuracy)), flush=True)

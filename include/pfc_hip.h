@@ -44,7 +44,10 @@ enum {
 
 enum { PFC_HIP_F64 = 0, PFC_HIP_F32 = 1 };
 
-enum { PFC_HIP_MODEL_PFC = 0 };
+enum { PFC_HIP_MODEL_PFC = 0, PFC_HIP_MODEL_TUNGSTEN = 1 };
+
+/* Number of parameters of PFC_HIP_MODEL_TUNGSTEN (see pfc_hip_pfc_set_operators). */
+#define PFC_HIP_TUNGSTEN_NPARAMS 17
 
 typedef struct pfc_hip_plan_s *pfc_hip_plan_t;
 
@@ -104,6 +107,17 @@ int pfc_hip_c2r(pfc_hip_plan_t plan, const void *in, void *out, void *stream);
  *     opL = exp(L dt), opN = (opL-1)/L * (-k^2) (dt*(-k^2) when |L| <= 1e-14),
  *     L = -k^2 (eps + (1-k^2)^2)
  *     (docs/api/examples/10_complete_pfc_model.cpp:166-213).
+ *   model_kind PFC_HIP_MODEL_TUNGSTEN, params (nparams 17) =
+ *     {dx, dy, dz, alpha2, lambda2, alpha_farTol, alpha_highOrd, Bx, T, T0,
+ *      stabP, p2_bar, q2_bar, p3_bar, p4_bar, q3_bar, q4_bar}
+ *     with alpha2 = 2*alpha^2 and lambda2 = 2*lambda^2 (the OperatorParams
+ *     of apps/tungsten/include/tungsten/common/tungsten_spectral.hpp:55-73
+ *     followed by the nonlinear coefficients): the mean-field filter
+ *     fMF = exp(kLap/lambda2), exp(L dt) and n_weight = kLap*phi1(L) with
+ *     L = kLap*opCk (tungsten_spectral.hpp:83-104,
+ *     tungsten_etd_workspace.hpp:99-130).  Replaces
+ *     Tungsten::prepare_operators (cpu/tungsten_model.hpp:223-279).
+ * The step evaluates these per mode on the fly; nothing is tabulated.
  */
 int pfc_hip_pfc_set_operators(pfc_hip_plan_t plan, int model_kind,
                               const double *params, int nparams, double dt);
@@ -115,6 +129,22 @@ int pfc_hip_pfc_set_operators(pfc_hip_plan_t plan, int model_kind,
  * nsteps times.  Requires pfc_hip_pfc_set_operators.
  */
 int pfc_hip_pfc_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *stream);
+
+/*
+ * Advance psi (device, inbox real values, in place) by nsteps Tungsten
+ * mean-field ETD1 steps (requires the PFC_HIP_MODEL_TUNGSTEN operators):
+ *   psi_MF = F^-1(fMF * F(psi)),
+ *   N = p3*psi^2 + p4*psi^3 + q3*psi_MF^2 + q4*psi_MF^3 - stabP*psi,
+ *   psi <- F^-1(exp(L dt)*F(psi) + n_weight*F(N)).
+ * psiMF (device, inbox real values, may be NULL) receives psi_MF of the last
+ * step's input field, as TungstenHIP::get_psiMF holds after step(); it must
+ * not alias psi.  pfc_hip_pfc_step on a Tungsten plan is this call with
+ * psiMF = NULL.  Replaces Tungsten::step / TungstenHIP::step
+ * (apps/tungsten/include/tungsten/cpu/tungsten_model.hpp:306-366;
+ * hip/tungsten_model.hpp:263-303).
+ */
+int pfc_hip_tungsten_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *psiMF,
+                          void *stream);
 
 /*
  * Mean-split step (the FP32-mixed mode): with enable != 0 the field passed
@@ -132,9 +162,12 @@ int pfc_hip_pfc_set_mean(pfc_hip_plan_t plan, int enable, double mean);
  * Measurement variant of pfc_hip_pfc_step: same work, with HIP events
  * around every kernel of the step.  On return (the stream is synchronised)
  * kernel_ms[i] holds the mean duration in milliseconds of pass i over the
- * nsteps steps, pass order: 0 forward-x (psi, psi^3), 1 forward-y (both
- * spectra), 2 z (forward, combine, inverse), 3 inverse-y, 4 inverse-x.
- * nkernels must be >= 5.  No reference counterpart (profiling aid).
+ * nsteps steps.  PFC (nkernels >= 5): 0 forward-x (psi, psi^3), 1 forward-y
+ * (both spectra), 2 z (forward, combine, inverse), 3 inverse-y, 4 inverse-x.
+ * Tungsten (nkernels >= 9): 0 forward-x (psi), 1 forward-y, 2 z (forward,
+ * mean-field filter, inverse), 3 inverse-y, 4 x (inverse, N, forward),
+ * 5 forward-y, 6 z (forward, combine, inverse), 7 inverse-y, 8 inverse-x.
+ * No reference counterpart (profiling aid).
  */
 int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,
                            void *stream, double *kernel_ms, int nkernels);
@@ -175,6 +208,15 @@ int pfc_hip_ic_add_noise(pfc_hip_plan_t plan, void *psi, unsigned long long seed
 int pfc_hip_pfc_operators(pfc_hip_plan_t plan, void *opL, void *opN, void *stream);
 
 /*
+ * Tungsten counterpart: filterMF, exp(L dt) and n_weight in the outbox
+ * layout, each scaled by 1/(nx*ny*nz) as the step applies them.
+ * Counterpart of filterMF and the TungstenEtdWorkspace weights
+ * (apps/tungsten/include/tungsten/cpu/tungsten_model.hpp:223-279).
+ */
+int pfc_hip_tungsten_operators(pfc_hip_plan_t plan, void *filterMF, void *expL,
+                               void *nW, void *stream);
+
+/*
  * Launch variants.  Every pass of the transforms and of the step exists in
  * several compile-time shapes (radix plan, values per thread, column tile
  * width, rows or lines per block, LDS exchange mode), and the step has two
@@ -183,7 +225,11 @@ int pfc_hip_pfc_operators(pfc_hip_plan_t plan, void *opL, void *opN, void *strea
  *   3 z columns (plain transforms)  4 natural step forward x
  *   5 natural step z pass  6 step layout (0 transposed T1/T2, 1 natural)
  *   7..11 transposed step: forward x, forward y, z pass, inverse y, inverse x
- * Index 0 is the default of each role.  No reference counterpart: the
+ *   12 natural Tungsten step: fused x pass (inverse, N, forward)
+ *   13 transposed Tungsten step: forward x of psi
+ *   14 transposed Tungsten step: fused x pass
+ * The Tungsten step also uses roles 0, 1, 2, 5 (natural) and 8..11
+ * (transposed).  Index 0 is the default of each role.  No reference counterpart: the
  * reference delegates shapes to rocFFT; its GPU autotuner only times 1-D
  * block sizes (include/openpfc/runtime/common/gpu_autotune.hpp:295-333).
  */
@@ -200,7 +246,10 @@ enum {
   PFC_HIP_ROLE_T_Z = 9,
   PFC_HIP_ROLE_T_Y_INV = 10,
   PFC_HIP_ROLE_T_X_INV = 11,
-  PFC_HIP_ROLE_COUNT = 12
+  PFC_HIP_ROLE_X_TUNG = 12,
+  PFC_HIP_ROLE_T_X_R2C = 13,
+  PFC_HIP_ROLE_T_X_TUNG = 14,
+  PFC_HIP_ROLE_COUNT = 15
 };
 int pfc_hip_variant_count(pfc_hip_plan_t plan, int role, int *count);
 /* Name of a variant ("r16x16x4/E16/L8/split"), NULL if out of range. */
@@ -213,11 +262,12 @@ int pfc_hip_variant_set(pfc_hip_plan_t plan, int role, int index);
  * Time every variant of every step role on full steps of psi (`reps` timed
  * steps after one warm-up each), keep the fastest per role, then keep the
  * faster step layout.  psi is restored before return (the stream is
- * synchronised).  times_ms (may be NULL) receives the mean pass time of each
- * variant tried, roles in the order 4, 2, 5, 1, 7, 8, 9, 10, 11 (roles of an
- * unsupported layout contribute no entries), followed by the step time of
- * the transposed and of the natural layout (0 if not run); up to max_times
- * entries are written.
+ * synchronised).  times_ms (may be NULL) receives the mean time per step of
+ * the passes each variant runs, for every variant tried, roles in the order
+ * 4, 2, 5, 1, 7, 8, 9, 10, 11 (PFC) or 0, 2, 5, 12, 1, 13, 8, 9, 10, 14, 11
+ * (Tungsten); roles of an unsupported layout contribute no entries.  Then
+ * follow the step times of the transposed and of the natural layout (0 if
+ * not run); up to max_times entries are written.
  */
 int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
                          double *times_ms, int max_times);

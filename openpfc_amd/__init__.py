@@ -15,9 +15,12 @@ import ctypes
 import time
 
 from . import _lib
-from ._lib import F32, F64, MODEL_PFC, PfcHipError, check, load
+from ._lib import (F32, F64, MODEL_PFC, MODEL_TUNGSTEN, TUNGSTEN_NPARAMS, PfcHipError,
+                   check, load)
+from .tungsten import TungstenParams
 
-__all__ = ["HipFft", "PfcStepper", "PfcHipError", "F32", "F64", "load"]
+__all__ = ["HipFft", "PfcStepper", "TungstenStepper", "TungstenParams", "PfcHipError",
+           "F32", "F64", "load"]
 
 
 def _stream_ptr(stream) -> ctypes.c_void_p:
@@ -93,7 +96,7 @@ class HipFft:
     # -- launch variants ------------------------------------------------------
     ROLES = {"x_r2c": 0, "x_c2r": 1, "col_y": 2, "col_z": 3, "pfc_x": 4, "pfc_z": 5,
              "step_layout": 6, "t_x_fwd": 7, "t_y_fwd": 8, "t_z": 9, "t_y_inv": 10,
-             "t_x_inv": 11}
+             "t_x_inv": 11, "x_tung": 12, "t_x_r2c": 13, "t_x_tung": 14}
 
     def variants(self, role) -> list:
         r = self.ROLES.get(role, role)
@@ -180,13 +183,71 @@ def ic_add_noise(fft: HipFft, psi, seed: int, lo: float, hi: float, stream=None)
                                         float(lo), float(hi), _stream_ptr(stream)))
 
 
-class PfcStepper:
+class _Stepper:
+    """Shared parts of the device-resident steppers (field checks, pass
+    timing, autotuning).  Subclasses set NPASSES, TUNE_ROLES (tuned roles in
+    the library's slot order: natural-layout roles first, NATURAL_ROLES of
+    them) and the operator setup."""
+
+    NPASSES = 5
+    TUNE_ROLES: list = []
+    NATURAL_ROLES = 0
+
+    def _check_psi(self, psi, what="step"):
+        import torch
+        rdt = torch.float64 if self.fft.precision == F64 else torch.float32
+        if psi.numel() != self.fft.size_inbox() or psi.dtype != rdt or not psi.is_cuda \
+                or not psi.is_contiguous():
+            raise ValueError(f"{type(self).__name__}.{what}: psi must be a contiguous "
+                             f"device tensor of size_inbox() {self.fft.size_inbox()} "
+                             f"elements of {rdt}")
+
+    def autotune(self, psi, reps: int = 2, stream=None) -> dict:
+        """Time every variant of the step's passes on psi (restored afterwards),
+        keep the fastest per pass and the faster layout; returns
+        {role: [(name, ms), ...], "layout_ms": {...}, "chosen": {...}}."""
+        self._check_psi(psi, "autotune")
+        roles = list(self.TUNE_ROLES)
+        layouts = self.fft.variants("step_layout")
+        if "transposed" not in layouts:
+            roles = roles[:self.NATURAL_ROLES]
+        names = {r: self.fft.variants(r) for r in roles}
+        total = sum(len(v) for v in names.values()) + 2
+        out = (ctypes.c_double * total)()
+        check(self.fft._lib.pfc_hip_pfc_autotune(self.fft.handle, _ptr(psi), int(reps),
+                                                 _stream_ptr(stream), out, total))
+        res, i = {}, 0
+        for r in roles:
+            res[r] = [(n, out[i + k]) for k, n in enumerate(names[r])]
+            i += len(names[r])
+        res["layout_ms"] = {"transposed": out[i], "natural": out[i + 1]}
+        res["chosen"] = {r: names[r][self.fft.get_variant(r)] for r in roles}
+        res["chosen"]["step_layout"] = self.fft.variants("step_layout")[
+            self.fft.get_variant("step_layout")]
+        return res
+
+    def step_timed(self, psi, nsteps: int, stream=None):
+        """Run nsteps with HIP events around each pass; returns the mean
+        duration of each of the NPASSES passes in ms (stream synchronised)."""
+        self._check_psi(psi, "step_timed")
+        out = (ctypes.c_double * self.NPASSES)()
+        check(self.fft._lib.pfc_hip_pfc_step_timed(self.fft.handle, _ptr(psi), int(nsteps),
+                                                   _stream_ptr(stream), out, self.NPASSES))
+        return list(out)
+
+
+class PfcStepper(_Stepper):
     """Device-resident plain conserved PFC ETD1 stepper over a HipFft plan.
 
     Mirrors PFCModel::initialize/step (docs/api/examples/10_complete_pfc_model.cpp:
     94-144,166-213): ``initialize(dt)`` builds opL/opN on the device,
     ``step(psi, n)`` advances ``psi`` in place by n steps.
     """
+
+    NPASSES = 5
+    TUNE_ROLES = ["pfc_x", "col_y", "pfc_z", "x_c2r", "t_x_fwd", "t_y_fwd", "t_z",
+                  "t_y_inv", "t_x_inv"]
+    NATURAL_ROLES = 4
 
     def __init__(self, fft: HipFft, spacing, epsilon: float):
         self.fft = fft
@@ -207,14 +268,6 @@ class PfcStepper:
                                                  float(mean)))
         self.mean = float(mean) if enable else None
 
-    def _check_psi(self, psi):
-        import torch
-        rdt = torch.float64 if self.fft.precision == F64 else torch.float32
-        if psi.numel() != self.fft.size_inbox() or psi.dtype != rdt or not psi.is_cuda \
-                or not psi.is_contiguous():
-            raise ValueError("PfcStepper.step: psi must be a contiguous device tensor of "
-                             f"size_inbox() {self.fft.size_inbox()} elements of {rdt}")
-
     def step(self, psi, nsteps: int = 1, stream=None) -> None:
         self._check_psi(psi)
         check(self.fft._lib.pfc_hip_pfc_step(self.fft.handle, _ptr(psi), int(nsteps),
@@ -231,36 +284,54 @@ class PfcStepper:
                                                   _stream_ptr(stream)))
         return opL, opN
 
-    def autotune(self, psi, reps: int = 2, stream=None) -> dict:
-        """Time every variant of the step's passes on psi (restored afterwards),
-        keep the fastest per pass and the faster layout; returns
-        {role: [(name, ms), ...], "layout_ms": {...}, "chosen": {...}}."""
-        self._check_psi(psi)
-        roles = ["pfc_x", "col_y", "pfc_z", "x_c2r", "t_x_fwd", "t_y_fwd", "t_z",
-                 "t_y_inv", "t_x_inv"]
-        layouts = self.fft.variants("step_layout")
-        if "transposed" not in layouts:
-            roles = roles[:4]
-        names = {r: self.fft.variants(r) for r in roles}
-        total = sum(len(v) for v in names.values()) + 2
-        out = (ctypes.c_double * total)()
-        check(self.fft._lib.pfc_hip_pfc_autotune(self.fft.handle, _ptr(psi), int(reps),
-                                                 _stream_ptr(stream), out, total))
-        res, i = {}, 0
-        for r in roles:
-            res[r] = [(n, out[i + k]) for k, n in enumerate(names[r])]
-            i += len(names[r])
-        res["layout_ms"] = {"transposed": out[i], "natural": out[i + 1]}
-        res["chosen"] = {r: names[r][self.fft.get_variant(r)] for r in roles}
-        res["chosen"]["step_layout"] = self.fft.variants("step_layout")[
-            self.fft.get_variant("step_layout")]
-        return res
 
-    def step_timed(self, psi, nsteps: int, stream=None):
-        """Run nsteps with HIP events around each of the 5 passes; returns
-        the mean duration of each pass in ms (stream synchronised)."""
+class TungstenStepper(_Stepper):
+    """Device-resident Tungsten mean-field PFC stepper over a HipFft plan: the
+    TungstenHIP drop-in (apps/tungsten/include/tungsten/hip/tungsten_model.hpp:
+    57-330; the step of cpu/tungsten_model.hpp:306-366).
+
+    ``initialize(dt)`` sets the operators from ``params`` (a TungstenParams:
+    the reference's base parameters, derived ones computed as the reference
+    does); ``step(psi, n, psi_mf)`` advances psi in place and, when psi_mf is
+    given, leaves in it the mean field of the last step's input (what
+    TungstenHIP::get_psiMF holds after step()).
+    """
+
+    NPASSES = 9
+    TUNE_ROLES = ["x_r2c", "col_y", "pfc_z", "x_tung", "x_c2r", "t_x_r2c", "t_y_fwd",
+                  "t_z", "t_y_inv", "t_x_tung", "t_x_inv"]
+    NATURAL_ROLES = 5
+
+    def __init__(self, fft: HipFft, spacing, params: TungstenParams | None = None):
+        self.fft = fft
+        self.spacing = tuple(float(s) for s in spacing)
+        self.params = params if params is not None else TungstenParams()
+        self.dt = None
+
+    def initialize(self, dt: float) -> None:
+        vals = self.params.abi_params(self.spacing)
+        arr = (ctypes.c_double * TUNGSTEN_NPARAMS)(*vals)
+        check(self.fft._lib.pfc_hip_pfc_set_operators(self.fft.handle, MODEL_TUNGSTEN, arr,
+                                                      TUNGSTEN_NPARAMS, float(dt)))
+        self.dt = float(dt)
+
+    def step(self, psi, nsteps: int = 1, psi_mf=None, stream=None) -> None:
         self._check_psi(psi)
-        out = (ctypes.c_double * 5)()
-        check(self.fft._lib.pfc_hip_pfc_step_timed(self.fft.handle, _ptr(psi), int(nsteps),
-                                                   _stream_ptr(stream), out, 5))
-        return list(out)
+        if psi_mf is not None:
+            self._check_psi(psi_mf, "step (psi_mf)")
+        check(self.fft._lib.pfc_hip_tungsten_step(
+            self.fft.handle, _ptr(psi), int(nsteps),
+            _ptr(psi_mf) if psi_mf is not None else None, _stream_ptr(stream)))
+
+    def operators(self, stream=None):
+        """(filterMF, expL, nW) as device tensors in the outbox layout, each
+        scaled by 1/(nx ny nz) as the step applies them."""
+        import torch
+        dt = torch.float64 if self.fft.precision == F64 else torch.float32
+        nx, ny, nz = self.fft.n
+        f = torch.empty((nz, ny, nx // 2 + 1), dtype=dt, device="cuda")
+        e = torch.empty_like(f)
+        n = torch.empty_like(f)
+        check(self.fft._lib.pfc_hip_tungsten_operators(self.fft.handle, _ptr(f), _ptr(e),
+                                                       _ptr(n), _stream_ptr(stream)))
+        return f, e, n

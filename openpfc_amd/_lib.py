@@ -22,6 +22,8 @@ PFC_HIP_ESTATE = 5
 F64 = 0
 F32 = 1
 MODEL_PFC = 0
+MODEL_TUNGSTEN = 1
+TUNGSTEN_NPARAMS = 17
 
 # Every symbol include/pfc_hip.h declares, with its ctypes signature.
 _vp = ctypes.c_void_p
@@ -52,6 +54,8 @@ SIGNATURES = {
     "pfc_hip_ic_add_noise": (ctypes.c_int, [_vp, _vp, ctypes.c_ulonglong, ctypes.c_double,
                                             ctypes.c_double, _vp]),
     "pfc_hip_pfc_operators": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "pfc_hip_tungsten_operators": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "pfc_hip_tungsten_step": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
     "pfc_hip_pfc_set_mean": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
     "pfc_hip_variant_count": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "pfc_hip_variant_name": (ctypes.c_char_p, [_vp, ctypes.c_int, ctypes.c_int]),

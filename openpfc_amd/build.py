@@ -43,7 +43,8 @@ def _run(cmd, quiet=True):
         print(out)
 
 
-SOURCES = ["pfc_hip.hip", "variants_common.hip", "variants_f64.hip", "variants_f32.hip"]
+SOURCES = ["pfc_hip.hip", "variants_common.hip", "variants_f64.hip", "variants_f32.hip",
+           "variants_t_f64.hip", "variants_t_f32.hip"]
 OBJDIR = os.path.join(HERE, "_build")
 
 

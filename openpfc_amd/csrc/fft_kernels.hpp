@@ -95,6 +95,39 @@ __device__ __forceinline__ void row_fetch_partner(const cplx<T> (&v)[E],
 }
 
 // ----------------------------------------------------------------------------
+// Half-length real transforms.  A real row x of length nx = 2M is read as the
+// M complex values z[m] = x[2m] + i*x[2m+1] and transformed with an M-point
+// complex FFT Z.  r2c_split forms the half spectrum
+//   X[k] = (Z[k] + conj Z[M-k])/2 + w^k * (-i/2)(Z[k] - conj Z[M-k]),
+// w = exp(-2 pi i/nx), from zk = Z[k] and zmk = Z[(M-k) mod M];
+// X[M] = Re Z[0] - Im Z[0].  c2r_merge is its inverse: from X[k] and X[M-k]
+// it forms Z'[k] = (X[k] + conj X[M-k]) + i*conj(w^k)*(X[k] - conj X[M-k]),
+// whose M-point inverse is the unscaled real row (imaginary parts of X[0]
+// and X[M] ignored, as a half-complex inverse does).
+// ----------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ cplx<T> r2c_split(cplx<T> zk, cplx<T> zmk, cplx<T> w) {
+  const cplx<T> zc = c_conj(zmk);
+  const cplx<T> ev = c_scale(c_add(zk, zc), (T)0.5);
+  const cplx<T> d = c_sub(zk, zc);
+  const cplx<T> od = {(T)0.5 * d.y, (T)-0.5 * d.x}; // -i/2 * d
+  return c_add(ev, c_mul(w, od));
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> c2r_merge(cplx<T> xa, cplx<T> xmk, cplx<T> w,
+                                             bool k0) {
+  cplx<T> xb = c_conj(xmk);
+  if (k0) {
+    xa.y = (T)0;
+    xb.y = (T)0;
+  }
+  const cplx<T> ev = c_add(xa, xb);
+  const cplx<T> od = c_mulc(c_sub(xa, xb), w); // * exp(+2 pi i k/nx)
+  return cplx<T>{ev.x - od.y, ev.y + od.x};    // ev + i*od
+}
+
+// ----------------------------------------------------------------------------
 // x-axis R2C: real rows of length nx = 2M -> half spectrum rows (M+1 values).
 // Plan is the M-point complex transform; LPB rows per block.
 // ----------------------------------------------------------------------------
@@ -136,11 +169,7 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_x_r2c(XArgs<T> a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = t + TL * e;
-    const cplx<T> zk = v[e], zc = c_conj(pz[e]);
-    const cplx<T> ev = c_scale(c_add(zk, zc), (T)0.5);
-    const cplx<T> d = c_sub(zk, zc);
-    const cplx<T> od = {(T)0.5 * d.y, (T)-0.5 * d.x}; // -i/2 * d
-    out[k] = c_add(ev, c_mul(a.wn[k], od));
+    out[k] = r2c_split(v[e], pz[e], a.wn[k]);
   }
   if (t == 0) {
     // X[M] = Re Z0 - Im Z0 (v[0] holds Z0 on thread 0)
@@ -169,15 +198,7 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_x_c2r(XArgs<T> a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = t + TL * e;
-    cplx<T> xa = in[k];
-    cplx<T> xb = c_conj(in[M - k]);
-    if (k == 0) {
-      xa.y = (T)0;
-      xb.y = (T)0;
-    }
-    const cplx<T> ev = c_add(xa, xb);
-    const cplx<T> od = c_mulc(c_sub(xa, xb), a.wn[k]); // * exp(+2 pi i k/nx)
-    v[e] = cplx<T>{ev.x - od.y, ev.y + od.x};          // ev + i*od
+    v[e] = c2r_merge(in[k], in[M - k], a.wn[k], k == 0);
   }
   const LdsLine<T> ll{lds, c * RP, 1, LPB * RP};
   LineFft<T, Plan, +1, false, true>::run(v, t, ll, a.tw);
@@ -328,23 +349,144 @@ __global__ void k_pfc_operators(PfcOp o, int W, T *opL, T *opN) {
 }
 
 // ----------------------------------------------------------------------------
-// Fused PFC step, z pass: for L adjacent kx columns at one ky,
-//   A = Fz(Psi) ; B = Fz(Nl) ; A = opL*A + opN*B ; Psi = Fz^-1(A)
-// with opL/opN evaluated per mode on the fly (no operator tables in HBM).
+// Tungsten mean-field PFC operators for one mode, in the reference's formula
+// and evaluation order (apps/tungsten/include/tungsten/common/
+// tungsten_spectral.hpp:83-104; tungsten_etd_workspace.hpp:99-130;
+// include/openpfc/kernel/integrator/spectral_exp_coefficients.hpp:68-82):
+//   kLap = -(kx*kx + ky*ky + kz*kz)
+//   fMF = exp(kLap/lambda2)                        (mean-field filter chi(k))
+//   k_val = sqrt(-kLap) - 1, k2 = k_val^2
+//   gf = k_val < 0 ? exp(-(k2 + rTol*k_val^highOrd)/alpha2)   (highOrd != 0)
+//                  : 1 - (1/alpha2)*k2
+//   opCk = ((stabP + p2_bar) - Bx*exp(-T/T0)*gf) + q2_bar*fMF
+//   L = kLap*opCk, expL = exp(L*dt),
+//   nW = kLap * (|L| < 1e-12 ? dt + 0.5*L*dt*dt : expm1(L*dt)/L)
+// The mode-independent scalars rTol = -alpha2*log(farTol) - 1,
+// c_peak = Bx*exp(-T/T0) and sp2 = stabP + p2_bar are formed on the host in
+// the same order.  Step multipliers are scaled by PfcOp::scale.
+// ----------------------------------------------------------------------------
+struct TungOp {
+  double alpha2, lambda2, rTol, c_peak, sp2, q2;
+  int highOrd;
+};
+
+__device__ __forceinline__ double tung_filter(const TungOp &p, double kxy, double kz) {
+#pragma clang fp contract(off)
+  const double kLap = -(kxy + (kz * kz));
+  return exp(kLap / p.lambda2);
+}
+
+__device__ __forceinline__ void tung_op_mode(const PfcOp &o, const TungOp &p,
+                                             double kxy, double kz, double &fMF,
+                                             double &expL, double &nW) {
+#pragma clang fp contract(off)
+  const double kLap = -(kxy + (kz * kz));
+  fMF = exp(kLap / p.lambda2);
+  const double k_val = sqrt(-kLap) - 1.0;
+  const double k2 = k_val * k_val;
+  double gf;
+  if (k_val < 0.0) {
+    if (p.highOrd == 0)
+      gf = exp((-k2) / p.alpha2);
+    else
+      gf = exp((-(k2 + (p.rTol * pow(k_val, (double)p.highOrd)))) / p.alpha2);
+  } else {
+    gf = 1.0 - ((1.0 / p.alpha2) * k2);
+  }
+  const double opCk = (p.sp2 - (p.c_peak * gf)) + (p.q2 * fMF);
+  const double L = kLap * opCk;
+  const double arg = L * o.dt;
+  expL = exp(arg);
+  double phi1;
+  if (fabs(L) < 1e-12)
+    phi1 = o.dt + (((0.5 * L) * o.dt) * o.dt);
+  else
+    phi1 = expm1(arg) / L;
+  nW = kLap * phi1;
+}
+
+// Operator tables of the Tungsten step in the outbox layout (pitch W),
+// scaled by o.scale, for inspection and tests.
+template <typename T>
+__global__ void k_tung_operators(PfcOp o, TungOp p, int W, T *fMF, T *expL, T *nW) {
+#pragma clang fp contract(off)
+  const long total = (long)W * o.ny * o.nz;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % W);
+    const long jk = idx / W;
+    const int j = (int)(jk % o.ny);
+    const int k = (int)(jk / o.ny);
+    double f, e, n;
+    tung_op_mode(o, p, pfc_kxy(o, i, j), pfc_kc(k, o.nz, o.fz), f, e, n);
+    fMF[idx] = (T)(f * o.scale);
+    expL[idx] = (T)(e * o.scale);
+    nW[idx] = (T)(n * o.scale);
+  }
+}
+
+// Tungsten nonlinearity (apps/tungsten/include/tungsten/cpu/tungsten_model.hpp:
+// 331-348) with u = psi and v = psi_MF:
+//   N = p3*u^2 + p4*u^3 + q3*v^2 + q4*v^3, then N - stabP*u when stabP != 0,
+// every power and sum rounded in the reference's order.
+template <typename T> struct TungNl {
+  T p3, p4, q3, q4, stabP;
+  __device__ __forceinline__ T operator()(T u, T v) const {
+#pragma clang fp contract(off)
+    const T u2 = u * u, u3 = (u * u) * u, v2 = v * v, v3 = (v * v) * v;
+    T n = (((p3 * u2) + (p4 * u3)) + (q3 * v2)) + (q4 * v3);
+    if (stabP != (T)0) n = n - (stabP * u);
+    return n;
+  }
+};
+
+// z-pass kinds.  ZM_PFC: forward both spectra, ETD combine, inverse.
+// ZM_TMF (Tungsten, first z pass): forward Psi, keep the full spectrum, then
+// the inverse of fMF*Psi (the mean field).  ZM_TCOMB (Tungsten, second z
+// pass): forward N, combine with the kept spectrum, inverse.
+enum ZMode { ZM_PFC = 0, ZM_TMF = 1, ZM_TCOMB = 2 };
+
+// Multipliers of one mode for a z pass of kind MODE, scaled by o.scale:
+// (opL, opN) for ZM_PFC, (fMF, -) for ZM_TMF, (expL, nW) for ZM_TCOMB.
+template <int MODE>
+__device__ __forceinline__ void z_mode_ops(const PfcOp &o, const TungOp &p,
+                                           double kxy, double kz, double &ma,
+                                           double &mb) {
+  if constexpr (MODE == ZM_PFC) {
+    pfc_op_mode(o, kxy, kz, ma, mb);
+  } else if constexpr (MODE == ZM_TMF) {
+    ma = tung_filter(p, kxy, kz) * o.scale;
+    mb = 0.0;
+  } else {
+    double f, e, n;
+    tung_op_mode(o, p, kxy, kz, f, e, n);
+    ma = e * o.scale;
+    mb = n * o.scale;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Fused step, z pass (natural layout): for L adjacent kx columns at one ky,
+//   ZM_PFC:   A = Fz^-1(opL*Fz(A) + opN*Fz(B))
+//   ZM_TMF:   A = Fz(A); B = Fz^-1(fMF*A)
+//   ZM_TCOMB: A = Fz^-1(expL*A + nW*Fz(B))
+// with the multipliers evaluated per mode on the fly (no tables in HBM).
 // ----------------------------------------------------------------------------
 template <typename T> struct PfcZArgs {
   int zero_k0; // mean-split step: the k = 0 mode of the deviation stays 0
+  int mode;    // ZMode
   cplx<T> *A;
-  const cplx<T> *B;
+  cplx<T> *B;
   const cplx<T> *tw;
   PfcOp op;
+  TungOp top;
   long s_pos; // P*ny
   int pitch;  // P
   int W;      // nx/2+1
   int ntx;
 };
 
-template <typename T, typename Plan, int L, bool SPLIT>
+template <typename T, typename Plan, int L, bool SPLIT, int MODE>
 __global__ void __launch_bounds__(L *Plan::TL) k_pfc_z(PfcZArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N;
   constexpr int NP = row_padded(N);
@@ -356,37 +498,136 @@ __global__ void __launch_bounds__(L *Plan::TL) k_pfc_z(PfcZArgs<T> a) {
   const bool active = x < a.W;
   const long base = (long)y * a.pitch + x;
   const LdsLine<T> ll{lds, c, L, L * NP};
+  const double kxy = pfc_kxy(a.op, x, y);
 
   cplx<T> acc[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e)
-    acc[e] = active ? a.A[base + (long)(t + TL * e) * a.s_pos] : cplx<T>{0, 0};
-  LineFft<T, Plan, -1, SPLIT, true>::run(acc, t, ll, a.tw);
-
-  const double kxy = pfc_kxy(a.op, x, y);
-  T opn[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    double l, n;
-    pfc_op_mode(a.op, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), l, n);
-    acc[e] = c_scale(acc[e], (T)l);
-    opn[e] = (T)n;
-  }
-  {
-    cplx<T> v[E];
+  if constexpr (MODE == ZM_PFC) {
 #pragma unroll
     for (int e = 0; e < E; ++e)
-      v[e] = active ? a.B[base + (long)(t + TL * e) * a.s_pos] : cplx<T>{0, 0};
-    LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw);
+      acc[e] = active ? a.A[base + (long)(t + TL * e) * a.s_pos] : cplx<T>{0, 0};
+    LineFft<T, Plan, -1, SPLIT, true>::run(acc, t, ll, a.tw);
+    T opn[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) acc[e] = c_add(acc[e], c_scale(v[e], opn[e]));
+    for (int e = 0; e < E; ++e) {
+      double l, n;
+      pfc_op_mode(a.op, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), l, n);
+      acc[e] = c_scale(acc[e], (T)l);
+      opn[e] = (T)n;
+    }
+    {
+      cplx<T> v[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        v[e] = active ? a.B[base + (long)(t + TL * e) * a.s_pos] : cplx<T>{0, 0};
+      LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw);
+#pragma unroll
+      for (int e = 0; e < E; ++e) acc[e] = c_add(acc[e], c_scale(v[e], opn[e]));
+    }
+    if (a.zero_k0 && x == 0 && y == 0 && t == 0) acc[0] = cplx<T>{0, 0};
+  } else if constexpr (MODE == ZM_TMF) {
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      acc[e] = active ? a.A[base + (long)(t + TL * e) * a.s_pos] : cplx<T>{0, 0};
+    LineFft<T, Plan, -1, SPLIT, true>::run(acc, t, ll, a.tw);
+    if (active) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) a.A[base + (long)(t + TL * e) * a.s_pos] = acc[e];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      double f, unused;
+      z_mode_ops<ZM_TMF>(a.op, a.top, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), f, unused);
+      acc[e] = c_scale(acc[e], (T)f);
+    }
+  } else {
+    // multipliers first, while nothing else is live
+    T opl[E], opn[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      double l, n;
+      z_mode_ops<ZM_TCOMB>(a.op, a.top, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), l, n);
+      opl[e] = (T)l;
+      opn[e] = (T)n;
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      acc[e] = active ? a.B[base + (long)(t + TL * e) * a.s_pos] : cplx<T>{0, 0};
+    LineFft<T, Plan, -1, SPLIT, true>::run(acc, t, ll, a.tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const cplx<T> h = active ? a.A[base + (long)(t + TL * e) * a.s_pos] : cplx<T>{0, 0};
+      acc[e] = c_add(c_scale(h, opl[e]), c_scale(acc[e], opn[e]));
+    }
   }
-  if (a.zero_k0 && x == 0 && y == 0 && t == 0) acc[0] = cplx<T>{0, 0};
   LineFft<T, Plan, +1, SPLIT, true>::run(acc, t, ll, a.tw);
   if (active) {
+    cplx<T> *__restrict__ out = MODE == ZM_TMF ? a.B : a.A;
 #pragma unroll
-    for (int e = 0; e < E; ++e) a.A[base + (long)(t + TL * e) * a.s_pos] = acc[e];
+    for (int e = 0; e < E; ++e) out[base + (long)(t + TL * e) * a.s_pos] = acc[e];
   }
+}
+
+// ----------------------------------------------------------------------------
+// Tungsten step, fused x pass (natural layout).  Per row: the mean-field half
+// spectrum (inverse y and z already applied, 1/(nx ny nz) folded into fMF) is
+// brought to real space with the nx/2-point inverse, combined with the row of
+// psi into the nonlinearity N(psi, psi_MF), and N is transformed forward into
+// the same spectrum row.  psi_MF never goes to HBM unless mf_out is set.
+// ----------------------------------------------------------------------------
+template <typename T> struct TungXArgs {
+  TungNl<T> nl;
+  const T *psi;      // real rows (nx = 2M values)
+  cplx<T> *spec;     // in: mean-field half spectra; out: N half spectra
+  T *mf_out;         // optional copy of psi_MF (real rows), or null
+  const cplx<T> *tw; // M-point line twiddles
+  const cplx<T> *wn; // exp(-2 pi i k/nx), k = 0..M
+  long rows;
+  int pitch;
+};
+
+template <typename T, typename Plan, int LPB>
+__global__ void __launch_bounds__(LPB *Plan::TL) k_x_tung(TungXArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N;
+  constexpr int RP = row_padded(M);
+  __shared__ T lds[2 * LPB * RP];
+  const int t = threadIdx.x % TL, c = threadIdx.x / TL;
+  const long row = (long)blockIdx.x * LPB + c;
+  const bool active = row < a.rows;
+  const long r = active ? row : 0;
+  cplx<T> *__restrict__ sp = a.spec + r * a.pitch;
+  const LdsLine<T> ll{lds, c * RP, 1, LPB * RP};
+
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + TL * e;
+    v[e] = c2r_merge(sp[k], sp[M - k], a.wn[k], k == 0);
+  }
+  LineFft<T, Plan, +1, false, true>::run(v, t, ll, a.tw);
+  // v[e] = (psi_MF[2k], psi_MF[2k+1]), k = t + TL*e
+  const cplx<T> *__restrict__ pr =
+      reinterpret_cast<const cplx<T> *>(a.psi + r * (2L * M));
+  if (a.mf_out && active) {
+    cplx<T> *__restrict__ mo = reinterpret_cast<cplx<T> *>(a.mf_out + r * (2L * M));
+#pragma unroll
+    for (int e = 0; e < E; ++e) mo[t + TL * e] = v[e];
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const cplx<T> u = pr[t + TL * e];
+    v[e] = cplx<T>{a.nl(u.x, v[e].x), a.nl(u.y, v[e].y)};
+  }
+  LineFft<T, Plan, -1, false, true>::run(v, t, ll, a.tw);
+  cplx<T> pz[E];
+  row_fetch_partner<T, E, TL>(v, pz, t, lds + c * RP, lds + LPB * RP + c * RP,
+                              [](int k) { return k == 0 ? 0 : M - k; });
+  if (!active) return;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int k = t + TL * e;
+    sp[k] = r2c_split(v[e], pz[e], a.wn[k]);
+  }
+  if (t == 0) sp[M] = cplx<T>{v[0].x - v[0].y, (T)0};
 }
 
 // ----------------------------------------------------------------------------

@@ -104,7 +104,9 @@ struct pfc_hip_plan_s {
   bool ops_ready = false;
   int model_kind = -1;
   double dt = 0;
-  PfcOp op{};
+  PfcOp op{};    // k-grid, dt, 1/(nx ny nz); eps of the PFC model
+  TungOp top{};  // Tungsten operator scalars
+  double tq[5] = {}; // Tungsten nonlinearity: p3_bar, p4_bar, q3_bar, q4_bar, stabP
   bool mean_split = false; // field holds psi - mean; mean kept in double here
   double mean = 0.0;
   int var[ROLE_COUNT] = {};
@@ -123,10 +125,13 @@ struct pfc_hip_plan_s {
     case ROLE_T_Y_FWD: case ROLE_T_Y_INV: return ny;
     case ROLE_T_Z: return nz;
     case ROLE_T_X_INV: return nx / 2;
+    case ROLE_X_TUNG: case ROLE_T_X_R2C: case ROLE_T_X_TUNG: return nx / 2;
     default: return 0;
     }
   }
   bool transposed() const { return var[ROLE_STEP_LAYOUT] == 0 && t_ok; }
+  bool tungsten() const { return model_kind == PFC_HIP_MODEL_TUNGSTEN; }
+  int step_passes() const { return tungsten() ? 9 : 5; }
   bool t_ok = false; // transposed step supported for these extents
 };
 
@@ -181,6 +186,18 @@ template <typename T> const cplx<T> *twp(pfc_hip_plan_s *p, int n, int pid) {
   return it == p->tw.end() ? nullptr : (const cplx<T> *)it->second;
 }
 
+template <typename T> TungNl<T> tung_nonlin(const pfc_hip_plan_s *p) {
+  return TungNl<T>{(T)p->tq[0], (T)p->tq[1], (T)p->tq[2], (T)p->tq[3], (T)p->tq[4]};
+}
+
+// Transposed-step kernel kind of a role, or -1.
+int t_kind_of_role(int role) {
+  if (role >= ROLE_T_X_FWD && role <= ROLE_T_X_INV) return role - ROLE_T_X_FWD;
+  if (role == ROLE_T_X_R2C) return TK_X_R2C;
+  if (role == ROLE_T_X_TUNG) return TK_X_TUNG;
+  return -1;
+}
+
 template <typename T> Nonlin<T> nonlin(const pfc_hip_plan_s *p) {
   Nonlin<T> n{};
   n.split = p->mean_split ? 1 : 0;
@@ -204,14 +221,14 @@ void free_all(pfc_hip_plan_s *p) {
 template <typename T> int variant_count(pfc_hip_plan_s *p, int role) {
   const int n = p->role_len(role);
   if (role == ROLE_STEP_LAYOUT) return p->t_ok ? 2 : 1;
-  if (role >= ROLE_T_X_FWD && role <= ROLE_T_X_INV)
-    return (int)t_variants<T>(role - ROLE_T_X_FWD, n).size();
+  if (t_kind_of_role(role) >= 0) return (int)t_variants<T>(t_kind_of_role(role), n).size();
   switch (role) {
   case ROLE_X_R2C: return (int)xr2c_variants<T>(n).size();
   case ROLE_X_C2R: return (int)xc2r_variants<T>(n).size();
   case ROLE_COL_Y: case ROLE_COL_Z: return (int)col_variants<T>(n).size();
   case ROLE_PFC_X: return (int)pfcx_variants<T>(n).size();
   case ROLE_PFC_Z: return (int)pfcz_variants<T>(n).size();
+  case ROLE_X_TUNG: return (int)tungx_variants<T>(n).size();
   default: return 0;
   }
 }
@@ -221,14 +238,14 @@ template <typename T> const std::string *variant_name(pfc_hip_plan_s *p, int rol
   const int n = p->role_len(role);
   if (v < 0 || v >= variant_count<T>(p, role)) return nullptr;
   if (role == ROLE_STEP_LAYOUT) return &layouts[p->t_ok ? v : 1];
-  if (role >= ROLE_T_X_FWD && role <= ROLE_T_X_INV)
-    return &t_variants<T>(role - ROLE_T_X_FWD, n)[v].name;
+  if (t_kind_of_role(role) >= 0) return &t_variants<T>(t_kind_of_role(role), n)[v].name;
   switch (role) {
   case ROLE_X_R2C: return &xr2c_variants<T>(n)[v].name;
   case ROLE_X_C2R: return &xc2r_variants<T>(n)[v].name;
   case ROLE_COL_Y: case ROLE_COL_Z: return &col_variants<T>(n)[v].name;
   case ROLE_PFC_X: return &pfcx_variants<T>(n)[v].name;
   case ROLE_PFC_Z: return &pfcz_variants<T>(n)[v].name;
+  case ROLE_X_TUNG: return &tungx_variants<T>(n)[v].name;
   default: return nullptr;
   }
 }
@@ -321,6 +338,23 @@ int c2r_impl(pfc_hip_plan_s *p, const void *in, void *out, hipStream_t s) {
   return 0;
 }
 
+// z pass of the natural layout on the plan's spectra A and B (ZMode `mode`).
+template <typename T> hipError_t z_natural(pfc_hip_plan_s *p, int mode, hipStream_t s) {
+  const auto &v = pfcz_variants<T>(p->nz)[p->var[ROLE_PFC_Z]];
+  PfcZArgs<T> a{};
+  a.zero_k0 = (mode == ZM_PFC && p->mean_split) ? 1 : 0;
+  a.mode = mode;
+  a.A = (cplx<T> *)p->A;
+  a.B = (cplx<T> *)p->B;
+  a.tw = twp<T>(p, p->nz, v.plan_id);
+  a.op = p->op;
+  a.top = p->top;
+  a.s_pos = (long)p->P * p->ny;
+  a.pitch = p->P;
+  a.W = p->W;
+  return v.fn(a, p->ny, s);
+}
+
 // One fused PFC step: 5 launches.  `ev` (optional) receives 6 events
 // bracketing the 5 passes.
 template <typename T>
@@ -349,20 +383,7 @@ int step_natural(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
     PFC_HIP_TRY(e, "pfc forward y pass");
   }
   mark(2);
-  {
-    const auto &v = pfcz_variants<T>(p->nz)[p->var[ROLE_PFC_Z]];
-    PfcZArgs<T> a{};
-    a.zero_k0 = p->mean_split ? 1 : 0;
-    a.A = (cplx<T> *)p->A;
-    a.B = (const cplx<T> *)p->B;
-    a.tw = twp<T>(p, p->nz, v.plan_id);
-    a.op = p->op;
-    a.s_pos = (long)p->P * p->ny;
-    a.pitch = p->P;
-    a.W = p->W;
-    e = v.fn(a, p->ny, s);
-    PFC_HIP_TRY(e, "pfc z pass");
-  }
+  PFC_HIP_TRY(z_natural<T>(p, ZM_PFC, s), "pfc z pass");
   mark(3);
   if (p->ny > 1) {
     e = col_pass<T>(p, 1, +1, p->A, p->A, nullptr, nullptr, 1, p->P, s);
@@ -409,6 +430,8 @@ int step_transposed(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev)
   a.op = p->op;
   a.nl = nonlin<T>(p);
   a.zero_k0 = p->mean_split ? 1 : 0;
+  a.mode = ZM_PFC;
+  a.nfields = 2;
   a.nx = p->nx;
   a.ny = p->ny;
   a.nz = p->nz;
@@ -428,9 +451,140 @@ int step_transposed(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev)
   return 0;
 }
 
+// One Tungsten mean-field step (cpu/tungsten_model.hpp:306-366) in the
+// natural layout: 9 launches, A holding Psi and B the mean field, then N.
+// `mf` (optional) receives psi_MF of the input field.
 template <typename T>
-int step_impl(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
+int step_tung_natural(pfc_hip_plan_s *p, void *psi, void *mf, hipStream_t s,
+                      hipEvent_t *ev) {
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], s);
+  };
+  const int m = p->nx / 2;
+  const long rows = (long)p->ny * p->nz;
+  hipError_t e;
+  mark(0);
+  {
+    const auto &v = xr2c_variants<T>(m)[p->var[ROLE_X_R2C]];
+    XArgs<T> a{};
+    a.in = (const T *)psi;
+    a.out = (T *)p->A;
+    a.tw = twp<T>(p, m, v.plan_id);
+    a.wn = (const cplx<T> *)p->wn;
+    a.rows = rows;
+    a.pitch = p->P;
+    a.scale = (T)1;
+    PFC_HIP_TRY(v.fn(a, s), "tungsten forward x");
+  }
+  mark(1);
+  if (p->ny > 1) {
+    e = col_pass<T>(p, 1, -1, p->A, p->A, nullptr, nullptr, 1, p->P, s);
+    PFC_HIP_TRY(e, "tungsten forward y");
+  }
+  mark(2);
+  PFC_HIP_TRY(z_natural<T>(p, ZM_TMF, s), "tungsten mean-field z pass");
+  mark(3);
+  if (p->ny > 1) {
+    e = col_pass<T>(p, 1, +1, p->B, p->B, nullptr, nullptr, 1, p->P, s);
+    PFC_HIP_TRY(e, "tungsten mean-field inverse y");
+  }
+  mark(4);
+  {
+    const auto &v = tungx_variants<T>(m)[p->var[ROLE_X_TUNG]];
+    TungXArgs<T> a{};
+    a.nl = tung_nonlin<T>(p);
+    a.psi = (const T *)psi;
+    a.spec = (cplx<T> *)p->B;
+    a.mf_out = (T *)mf;
+    a.tw = twp<T>(p, m, v.plan_id);
+    a.wn = (const cplx<T> *)p->wn;
+    a.rows = rows;
+    a.pitch = p->P;
+    PFC_HIP_TRY(v.fn(a, s), "tungsten fused x pass");
+  }
+  mark(5);
+  if (p->ny > 1) {
+    e = col_pass<T>(p, 1, -1, p->B, p->B, nullptr, nullptr, 1, p->P, s);
+    PFC_HIP_TRY(e, "tungsten forward y (N)");
+  }
+  mark(6);
+  PFC_HIP_TRY(z_natural<T>(p, ZM_TCOMB, s), "tungsten combine z pass");
+  mark(7);
+  if (p->ny > 1) {
+    e = col_pass<T>(p, 1, +1, p->A, p->A, nullptr, nullptr, 1, p->P, s);
+    PFC_HIP_TRY(e, "tungsten inverse y");
+  }
+  mark(8);
+  PFC_HIP_TRY(x_c2r_pass<T>(p, p->A, psi, p->P, 1.0, s), "tungsten inverse x");
+  mark(9);
+  return 0;
+}
+
+// The Tungsten step on the transposed layouts: 9 launches (T1/T2 buffers A
+// for Psi, B for the mean field and then N).
+template <typename T>
+int step_tung_transposed(pfc_hip_plan_s *p, void *psi, void *mf, hipStream_t s,
+                         hipEvent_t *ev) {
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], s);
+  };
+  TStepArgs<T> a{};
+  a.mode = ZM_PFC;
+  a.nfields = 1;
+  a.top = p->top;
+  a.tnl = tung_nonlin<T>(p);
+  a.mf_out = (T *)mf;
+  a.psi_in = (const T *)psi;
+  a.psi_out = (T *)psi;
+  a.A1 = (cplx<T> *)p->A1;
+  a.B1 = (cplx<T> *)p->B1;
+  a.A2 = (cplx<T> *)p->A2;
+  a.B2 = (cplx<T> *)p->B2;
+  a.wn = (const cplx<T> *)p->wn;
+  a.op = p->op;
+  a.nx = p->nx;
+  a.ny = p->ny;
+  a.nz = p->nz;
+  a.W = p->W;
+  struct TPass { int role, mode; bool on_b; const char *what; };
+  // on_b: the pass reads and writes the B buffers (passed as A1/A2)
+  static const TPass passes[9] = {
+      {ROLE_T_X_R2C, ZM_PFC, false, "tungsten x forward"},
+      {ROLE_T_Y_FWD, ZM_PFC, false, "tungsten y forward"},
+      {ROLE_T_Z, ZM_TMF, false, "tungsten mean-field z pass"},
+      {ROLE_T_Y_INV, ZM_PFC, true, "tungsten mean-field y inverse"},
+      {ROLE_T_X_TUNG, ZM_PFC, false, "tungsten fused x pass"},
+      {ROLE_T_Y_FWD, ZM_PFC, true, "tungsten y forward (N)"},
+      {ROLE_T_Z, ZM_TCOMB, false, "tungsten combine z pass"},
+      {ROLE_T_Y_INV, ZM_PFC, false, "tungsten y inverse"},
+      {ROLE_T_X_INV, ZM_PFC, false, "tungsten x inverse"}};
+  mark(0);
+  for (int k = 0; k < 9; ++k) {
+    const TPass &ps = passes[k];
+    const int n = p->role_len(ps.role);
+    const auto &v = t_variants<T>(t_kind_of_role(ps.role), n)[p->var[ps.role]];
+    TStepArgs<T> b = a;
+    b.mode = ps.mode;
+    if (ps.on_b) {
+      b.A1 = a.B1;
+      b.A2 = a.B2;
+    }
+    b.tw_x = b.tw_xm = b.tw_y = b.tw_z = twp<T>(p, n, v.plan_id);
+    PFC_HIP_TRY(v.fn(b, s), ps.what);
+    mark(k + 1);
+  }
+  return 0;
+}
+
+// One step of the plan's model; `ev` (optional) receives step_passes()+1
+// events bracketing the passes; `mf` (Tungsten, optional) receives psi_MF.
+template <typename T>
+int step_impl(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev,
+              void *mf = nullptr) {
   if (int rc = step_buffers(p)) return rc;
+  if (p->tungsten())
+    return p->transposed() ? step_tung_transposed<T>(p, psi, mf, s, ev)
+                           : step_tung_natural<T>(p, psi, mf, s, ev);
   return p->transposed() ? step_transposed<T>(p, psi, s, ev)
                          : step_natural<T>(p, psi, s, ev);
 }
@@ -503,7 +657,8 @@ int pfc_hip_plan_create(int nx, int ny, int nz, int precision, int flags,
   if (!rc) rc = upload_table(p, &p->wn, real_twiddles(nx), "real twiddles");
   if (!rc) {
     bool ok = true;
-    for (int r = ROLE_T_X_FWD; r <= ROLE_T_X_INV; ++r) {
+    for (int r = 0; r < ROLE_COUNT; ++r) {
+      if (t_kind_of_role(r) < 0) continue;
       int c = precision == PFC_HIP_F64 ? variant_count<double>(p, r)
                                        : variant_count<float>(p, r);
       ok = ok && c > 0;
@@ -561,41 +716,99 @@ int pfc_hip_c2r(pfc_hip_plan_t plan, const void *in, void *out, void *stream) {
                                    : c2r_impl<float>(plan, in, out, s);
 }
 
+// Host-side scalars of the Tungsten operators, in the reference's order
+// (tungsten_spectral.hpp:91-101), without contraction into FMAs.
+static TungOp tungsten_scalars(const double *q) {
+#pragma clang fp contract(off)
+  // q = {alpha2, lambda2, farTol, highOrd, Bx, T, T0, stabP, p2_bar, q2_bar}
+  TungOp t{};
+  t.alpha2 = q[0];
+  t.lambda2 = q[1];
+  t.rTol = ((-q[0]) * std::log(q[2])) - 1.0;
+  t.highOrd = (int)q[3];
+  t.c_peak = q[4] * std::exp((-q[5]) / q[6]);
+  t.sp2 = q[7] + q[8];
+  t.q2 = q[9];
+  return t;
+}
+
 int pfc_hip_pfc_set_operators(pfc_hip_plan_t plan, int model_kind,
                               const double *params, int nparams, double dt) {
   if (int rc = check_plan(plan)) return rc;
-  if (model_kind != PFC_HIP_MODEL_PFC)
+  if (model_kind != PFC_HIP_MODEL_PFC && model_kind != PFC_HIP_MODEL_TUNGSTEN)
     return fail(PFC_HIP_EUNSUPPORTED, "pfc_hip_pfc_set_operators: unknown model " +
                                           std::to_string(model_kind));
-  if (!params || nparams != 4)
+  if (model_kind == PFC_HIP_MODEL_PFC && (!params || nparams != 4))
     return fail(PFC_HIP_EINVAL,
                 "pfc_hip_pfc_set_operators: PFC needs params {dx, dy, dz, eps}");
-  if (!supported_len(plan->nx))
+  if (model_kind == PFC_HIP_MODEL_TUNGSTEN && (!params || nparams != PFC_HIP_TUNGSTEN_NPARAMS))
+    return fail(PFC_HIP_EINVAL,
+                "pfc_hip_pfc_set_operators: Tungsten needs " +
+                    std::to_string(PFC_HIP_TUNGSTEN_NPARAMS) + " params");
+  if (model_kind == PFC_HIP_MODEL_PFC && !supported_len(plan->nx))
     return fail(PFC_HIP_EUNSUPPORTED,
                 "pfc_hip_pfc_set_operators: nx unsupported for the fused step");
-  const double dx = params[0], dy = params[1], dz = params[2], eps = params[3];
-  if (!(dx > 0) || !(dy > 0) || !(dz > 0) || !std::isfinite(eps) ||
-      !std::isfinite(dt))
-    return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_set_operators: bad spacing/eps/dt");
+  const double dx = params[0], dy = params[1], dz = params[2];
+  if (!(dx > 0) || !(dy > 0) || !(dz > 0) || !std::isfinite(dt))
+    return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_set_operators: bad spacing or dt");
+  for (int i = 3; i < nparams; ++i)
+    if (!std::isfinite(params[i]))
+      return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_set_operators: parameter " +
+                                      std::to_string(i) + " is not finite");
+  if (model_kind == PFC_HIP_MODEL_TUNGSTEN) {
+    const double *q = params + 3;
+    if (!(q[0] > 0) || !(q[1] > 0) || !(q[2] > 0) || q[6] == 0.0 || q[3] < 0 ||
+        q[3] != std::floor(q[3]) || q[3] > 64)
+      return fail(PFC_HIP_EINVAL,
+                  "pfc_hip_pfc_set_operators: Tungsten needs alpha2 > 0, lambda2 > 0, "
+                  "alpha_farTol > 0, T0 != 0 and an integer alpha_highOrd in [0, 64]");
+  }
   if (int rc = set_device(plan)) return rc;
-  if (int rc = step_buffers(plan)) return rc;
-  // kspace.hpp:97-103: f = 2*pi/(d*n); opL/opN are evaluated per mode in the
-  // z pass, scaled by the 1/(nx*ny*nz) of the backward transform.
+  // kspace.hpp:97-103: f = 2*pi/(d*n); the multipliers are evaluated per
+  // mode in the z passes, scaled by the 1/(nx*ny*nz) of the backward transform.
   const double two_pi = 6.283185307179586;
   PfcOp o{};
   o.fx = two_pi / (dx * plan->nx);
   o.fy = two_pi / (dy * plan->ny);
   o.fz = two_pi / (dz * plan->nz);
-  o.eps = eps;
+  o.eps = model_kind == PFC_HIP_MODEL_PFC ? params[3] : 0.0;
   o.dt = dt;
   o.scale = 1.0 / ((double)plan->nx * (double)plan->ny * (double)plan->nz);
   o.nx = plan->nx;
   o.ny = plan->ny;
   o.nz = plan->nz;
   plan->op = o;
+  if (model_kind == PFC_HIP_MODEL_TUNGSTEN) {
+    plan->top = tungsten_scalars(params + 3);
+    for (int i = 0; i < 4; ++i) plan->tq[i] = params[13 + i]; // p3, p4, q3, q4
+    plan->tq[4] = params[10];                                  // stabP
+  }
   plan->model_kind = model_kind;
+  if (int rc = step_buffers(plan)) return rc;
   plan->dt = dt;
   plan->ops_ready = true;
+  return 0;
+}
+
+int pfc_hip_tungsten_operators(pfc_hip_plan_t plan, void *filterMF, void *expL,
+                               void *nW, void *stream) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!filterMF || !expL || !nW)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_tungsten_operators: null buffer");
+  if (!plan->ops_ready || !plan->tungsten())
+    return fail(PFC_HIP_ESTATE, "pfc_hip_tungsten_operators: Tungsten operators not set");
+  if (int rc = set_device(plan)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned blocks = ic_blocks((long)plan->outbox());
+  if (plan->prec == PFC_HIP_F64)
+    hipLaunchKernelGGL(k_tung_operators<double>, dim3(blocks), dim3(256), 0, s,
+                       plan->op, plan->top, plan->W, (double *)filterMF,
+                       (double *)expL, (double *)nW);
+  else
+    hipLaunchKernelGGL(k_tung_operators<float>, dim3(blocks), dim3(256), 0, s,
+                       plan->op, plan->top, plan->W, (float *)filterMF,
+                       (float *)expL, (float *)nW);
+  PFC_HIP_TRY(hipGetLastError(), "tungsten operator kernel launch");
   return 0;
 }
 
@@ -603,8 +816,8 @@ int pfc_hip_pfc_operators(pfc_hip_plan_t plan, void *opL, void *opN,
                           void *stream) {
   if (int rc = check_plan(plan)) return rc;
   if (!opL || !opN) return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_operators: null buffer");
-  if (!plan->ops_ready)
-    return fail(PFC_HIP_ESTATE, "pfc_hip_pfc_operators: operators not set");
+  if (!plan->ops_ready || plan->tungsten())
+    return fail(PFC_HIP_ESTATE, "pfc_hip_pfc_operators: PFC operators not set");
   if (int rc = set_device(plan)) return rc;
   hipStream_t s = (hipStream_t)stream;
   const unsigned blocks = ic_blocks((long)plan->outbox());
@@ -618,14 +831,21 @@ int pfc_hip_pfc_operators(pfc_hip_plan_t plan, void *opL, void *opN,
   return 0;
 }
 
-int pfc_hip_pfc_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *stream) {
+static int step_checks(pfc_hip_plan_t plan, void *psi, int nsteps, const char *fn) {
   if (int rc = check_plan(plan)) return rc;
-  if (!psi) return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_step: null psi");
-  if (nsteps < 0) return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_step: nsteps < 0");
+  if (!psi) return fail(PFC_HIP_EINVAL, std::string(fn) + ": null psi");
+  if (nsteps < 0) return fail(PFC_HIP_EINVAL, std::string(fn) + ": nsteps < 0");
   if (!plan->ops_ready)
-    return fail(PFC_HIP_ESTATE,
-                "pfc_hip_pfc_step: operators not set (call pfc_hip_pfc_set_operators)");
-  if (int rc = set_device(plan)) return rc;
+    return fail(PFC_HIP_ESTATE, std::string(fn) +
+                                    ": operators not set (call pfc_hip_pfc_set_operators)");
+  if (plan->tungsten() && plan->mean_split)
+    return fail(PFC_HIP_EUNSUPPORTED,
+                std::string(fn) + ": the mean-split mode applies to the PFC model only");
+  return set_device(plan);
+}
+
+int pfc_hip_pfc_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *stream) {
+  if (int rc = step_checks(plan, psi, nsteps, "pfc_hip_pfc_step")) return rc;
   hipStream_t s = (hipStream_t)stream;
   for (int i = 0; i < nsteps; ++i) {
     int rc = plan->prec == PFC_HIP_F64 ? step_impl<double>(plan, psi, s, nullptr)
@@ -635,17 +855,34 @@ int pfc_hip_pfc_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *stream) {
   return 0;
 }
 
+int pfc_hip_tungsten_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *psiMF,
+                          void *stream) {
+  if (int rc = step_checks(plan, psi, nsteps, "pfc_hip_tungsten_step")) return rc;
+  if (!plan->tungsten())
+    return fail(PFC_HIP_ESTATE, "pfc_hip_tungsten_step: Tungsten operators not set");
+  if (psiMF && psiMF == psi)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_tungsten_step: psiMF must not alias psi");
+  hipStream_t s = (hipStream_t)stream;
+  for (int i = 0; i < nsteps; ++i) {
+    void *mf = (i == nsteps - 1) ? psiMF : nullptr;
+    int rc = plan->prec == PFC_HIP_F64 ? step_impl<double>(plan, psi, s, nullptr, mf)
+                                       : step_impl<float>(plan, psi, s, nullptr, mf);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
 int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,
                            void *stream, double *kernel_ms, int nkernels) {
-  if (int rc = check_plan(plan)) return rc;
-  if (!psi || !kernel_ms || nkernels < 5)
-    return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_step_timed: bad arguments");
+  if (int rc = step_checks(plan, psi, nsteps, "pfc_hip_pfc_step_timed")) return rc;
+  const int np = plan->step_passes();
+  if (!kernel_ms || nkernels < np)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_step_timed: kernel_ms needs " +
+                                    std::to_string(np) + " entries");
   if (nsteps < 1) return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_step_timed: nsteps < 1");
-  if (!plan->ops_ready)
-    return fail(PFC_HIP_ESTATE, "pfc_hip_pfc_step_timed: operators not set");
-  if (int rc = set_device(plan)) return rc;
   hipStream_t s = (hipStream_t)stream;
-  std::vector<hipEvent_t> ev(6 * (size_t)nsteps, nullptr);
+  const int ne = np + 1;
+  std::vector<hipEvent_t> ev((size_t)ne * nsteps, nullptr);
   int rc = 0;
   for (auto &e : ev) {
     hipError_t he = hipEventCreate(&e);
@@ -656,18 +893,18 @@ int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,
     }
   }
   for (int i = 0; i < nsteps && !rc; ++i)
-    rc = plan->prec == PFC_HIP_F64 ? step_impl<double>(plan, psi, s, &ev[6 * i])
-                                   : step_impl<float>(plan, psi, s, &ev[6 * i]);
+    rc = plan->prec == PFC_HIP_F64 ? step_impl<double>(plan, psi, s, &ev[(size_t)ne * i])
+                                   : step_impl<float>(plan, psi, s, &ev[(size_t)ne * i]);
   if (!rc) {
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) rc = fail(PFC_HIP_EDEVICE, hipGetErrorString(e));
   }
   if (!rc) {
-    for (int k = 0; k < 5; ++k) kernel_ms[k] = 0;
+    for (int k = 0; k < np; ++k) kernel_ms[k] = 0;
     for (int i = 0; i < nsteps; ++i)
-      for (int k = 0; k < 5; ++k) {
+      for (int k = 0; k < np; ++k) {
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, ev[6 * i + k], ev[6 * i + k + 1]);
+        (void)hipEventElapsedTime(&ms, ev[(size_t)ne * i + k], ev[(size_t)ne * i + k + 1]);
         kernel_ms[k] += ms / nsteps;
       }
   }
@@ -816,7 +1053,8 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
   const size_t rbytes = plan->inbox() * plan->esize;
   PFC_HIP_TRY(hipMemcpyAsync(plan->work, psi, rbytes, hipMemcpyDeviceToDevice, s),
               "autotune backup");
-  std::vector<hipEvent_t> ev(6, nullptr);
+  const int np = plan->step_passes();
+  std::vector<hipEvent_t> ev((size_t)np + 1, nullptr);
   int rc = 0;
   for (auto &e : ev) {
     hipError_t he = hipEventCreate(&e);
@@ -826,33 +1064,47 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
       break;
     }
   }
-  // Tuned roles in slot order, with the pass intervals (event pairs) their
-  // kernels occupy; the last two slots receive each layout's step time.
-  struct RoleSpan { int role, layout, p0, p1; };
-  const RoleSpan spans[9] = {
-      {ROLE_PFC_X, 1, 0, -1},   {ROLE_COL_Y, 1, 1, 3},   {ROLE_PFC_Z, 1, 2, -1},
-      {ROLE_X_C2R, 1, 4, -1},   {ROLE_T_X_FWD, 0, 0, -1}, {ROLE_T_Y_FWD, 0, 1, -1},
-      {ROLE_T_Z, 0, 2, -1},     {ROLE_T_Y_INV, 0, 3, -1}, {ROLE_T_X_INV, 0, 4, -1}};
+  // Tuned roles in slot order, with the passes (bit k = pass k of the step)
+  // their kernels occupy; the last two slots receive each layout's step time.
+  struct RoleSpan { int role, layout; unsigned passes; };
+  static const RoleSpan pfc_spans[] = {
+      {ROLE_PFC_X, 1, 0x1},   {ROLE_COL_Y, 1, 0xA},    {ROLE_PFC_Z, 1, 0x4},
+      {ROLE_X_C2R, 1, 0x10},  {ROLE_T_X_FWD, 0, 0x1},  {ROLE_T_Y_FWD, 0, 0x2},
+      {ROLE_T_Z, 0, 0x4},     {ROLE_T_Y_INV, 0, 0x8},  {ROLE_T_X_INV, 0, 0x10}};
+  static const RoleSpan tung_spans[] = {
+      {ROLE_X_R2C, 1, 0x1},   {ROLE_COL_Y, 1, 0xAA},   {ROLE_PFC_Z, 1, 0x44},
+      {ROLE_X_TUNG, 1, 0x10}, {ROLE_X_C2R, 1, 0x100},  {ROLE_T_X_R2C, 0, 0x1},
+      {ROLE_T_Y_FWD, 0, 0x22}, {ROLE_T_Z, 0, 0x44},    {ROLE_T_Y_INV, 0, 0x88},
+      {ROLE_T_X_TUNG, 0, 0x10}, {ROLE_T_X_INV, 0, 0x100}};
+  const RoleSpan *spans = plan->tungsten() ? tung_spans : pfc_spans;
+  const int nspans = plan->tungsten() ? (int)(sizeof(tung_spans) / sizeof(RoleSpan))
+                                      : (int)(sizeof(pfc_spans) / sizeof(RoleSpan));
   auto run_step = [&](double *pass_ms) -> int {
     int r = plan->prec == PFC_HIP_F64 ? step_impl<double>(plan, psi, s, ev.data())
                                       : step_impl<float>(plan, psi, s, ev.data());
     if (r) return r;
-    hipError_t he = hipEventSynchronize(ev[5]);
+    hipError_t he = hipEventSynchronize(ev[np]);
     if (he != hipSuccess) return fail(PFC_HIP_EDEVICE, hipGetErrorString(he));
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < np; ++k) {
       float ms = 0;
       (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
       pass_ms[k] = ms;
     }
     return 0;
   };
+  auto span_ms = [&](const double *pm, unsigned passes) {
+    double t = 0;
+    for (int k = 0; k < np; ++k)
+      if (passes & (1u << k)) t += pm[k];
+    return t;
+  };
   int slot = 0;
   double layout_ms[2] = {0, 0};
   const int layouts = plan->t_ok ? 2 : 1;
   for (int li = 0; li < layouts && !rc; ++li) {
-    const int layout = plan->t_ok ? li : 1;
+    const int layout = plan->t_ok ? 1 - li : 1; // natural first (slot order)
     rc = pfc_hip_variant_set(plan, ROLE_STEP_LAYOUT, plan->t_ok ? layout : 0);
-    for (int si = 0; si < 9 && !rc; ++si) {
+    for (int si = 0; si < nspans && !rc; ++si) {
       const RoleSpan &sp = spans[si];
       if (sp.layout != layout) continue;
       int count = 0;
@@ -861,10 +1113,10 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
       int best_v = plan->var[sp.role];
       for (int v = 0; v < count && !rc; ++v) {
         plan->var[sp.role] = v;
-        double total = 0, pm[5];
+        double total = 0, pm[9];
         for (int r = 0; r <= reps && !rc; ++r) { // r == 0 warms up
           rc = run_step(pm);
-          if (!rc && r > 0) total += pm[sp.p0] + (sp.p1 >= 0 ? pm[sp.p1] : 0.0);
+          if (!rc && r > 0) total += span_ms(pm, sp.passes);
         }
         const double ms = total / reps;
         const int idx = slot + v;
@@ -877,10 +1129,10 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
       plan->var[sp.role] = best_v;
       slot += count;
     }
-    double total = 0, pm[5];
+    double total = 0, pm[9];
     for (int r = 0; r <= reps && !rc; ++r) {
       rc = run_step(pm);
-      if (!rc && r > 0) total += pm[0] + pm[1] + pm[2] + pm[3] + pm[4];
+      if (!rc && r > 0) total += span_ms(pm, ~0u);
     }
     layout_ms[layout] = total / reps;
   }
@@ -904,6 +1156,9 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
 
 int pfc_hip_pfc_set_mean(pfc_hip_plan_t plan, int enable, double mean) {
   if (int rc = check_plan(plan)) return rc;
+  if (enable && plan->tungsten())
+    return fail(PFC_HIP_EUNSUPPORTED,
+                "pfc_hip_pfc_set_mean: the mean-split mode applies to the PFC model only");
   if (!std::isfinite(mean)) return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_set_mean: mean not finite");
   plan->mean_split = enable != 0;
   plan->mean = enable ? mean : 0.0;

@@ -14,6 +14,14 @@
 //   k_ty_inv  T2 -> T1:               transposed load, y lines of T1
 //   k_tx_inv  T1 -> psi:              transposed load, C2R rows of psi
 //
+// The Tungsten mean-field step runs nine passes on the same layouts (A for
+// Psi, B for the mean field and then N):
+//   k_tx_r2c psi -> A1 ; k_ty_fwd A1 -> A2 ; k_tz(ZM_TMF) A2 -> A2, B2 ;
+//   k_ty_inv B2 -> B1 ; k_tx_tung B1 -> B1 (inverse x, N(psi, psi_MF),
+//   forward x) ; k_ty_fwd B1 -> B2 ; k_tz(ZM_TCOMB) A2, B2 -> A2 ;
+//   k_ty_inv A2 -> A1 ; k_tx_inv A1 -> psi
+// for 3R + 18C bytes (R a real field, C a half spectrum).
+//
 // A block handles G lines that are adjacent in the transposed dimension, so
 // the strided side of each pass moves G*sizeof(complex) contiguous bytes
 // (128 bytes for G = 8 in double).  Algorithmic traffic is the same as the
@@ -62,7 +70,12 @@ __device__ __forceinline__ void fetch_partner(const cplx<T> (&v)[E], cplx<T> (&p
 template <typename T> struct TStepArgs {
   Nonlin<T> nl;     // nonlinearity (plain or mean-split)
   int zero_k0;      // mean-split step: keep the k = 0 mode of the deviation at 0
-  const T *psi_in;  // k_tx_fwd input
+  int mode;         // k_tz kind (ZMode)
+  int nfields;      // k_ty_fwd: 2 (A and B) or 1 (A only)
+  TungOp top;       // Tungsten operator scalars
+  TungNl<T> tnl;    // Tungsten nonlinearity
+  T *mf_out;        // k_tx_tung: optional copy of psi_MF, or null
+  const T *psi_in;  // k_tx_fwd / k_tx_r2c / k_tx_tung input
   T *psi_out;       // k_tx_inv output
   cplx<T> *A1, *B1; // T1 buffers (Psi, N)
   cplx<T> *A2, *B2; // T2 buffers (Psi, N)
@@ -197,7 +210,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd(TStepArgs<T> a) {
 // ----------------------------------------------------------------------------
 // T2 z pass, in place: G consecutive z lines (kx*ny + ky) per block.
 // ----------------------------------------------------------------------------
-template <typename T, typename Plan, int G, bool SPLIT>
+template <typename T, typename Plan, int G, bool SPLIT, int MODE>
 __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
   constexpr int PL = t_plane<Plan, G>();
@@ -210,37 +223,68 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
   const long l = active ? li : l0;
   const int kx = (int)(l / a.ny), ky = (int)(l % a.ny);
   const LdsLine<T> ll{lds, g * RP, 1, PL};
-
-  // operators first, while nothing else is live (exp and division need
-  // many temporaries); then both lines are loaded together
-  const double kxy = pfc_kxy(a.op, kx, ky);
-  T opl[E], opn[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    double oL, oN;
-    pfc_op_mode(a.op, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), oL, oN);
-    opl[e] = (T)oL;
-    opn[e] = (T)oN;
-  }
   cplx<T> *__restrict__ la = a.A2 + l * N;
-  const cplx<T> *__restrict__ lb = a.B2 + l * N;
-  cplx<T> acc[E], v[E];
+  cplx<T> *__restrict__ lb = a.B2 + l * N;
+  const double kxy = pfc_kxy(a.op, kx, ky);
+
+  cplx<T> acc[E];
+  if constexpr (MODE == ZM_TMF) {
+    // A = Fz(A) (kept for the combine); B = Fz^-1(fMF * A)
 #pragma unroll
-  for (int e = 0; e < E; ++e) {
-    acc[e] = la[t + TL * e];
-    v[e] = lb[t + TL * e];
+    for (int e = 0; e < E; ++e) acc[e] = la[t + TL * e];
+    LineFft<T, Plan, -1, SPLIT, true>::run(acc, t, ll, a.tw_z);
+    if (active) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) la[t + TL * e] = acc[e];
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      double f, unused;
+      z_mode_ops<ZM_TMF>(a.op, a.top, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), f,
+                         unused);
+      acc[e] = c_scale(acc[e], (T)f);
+    }
+  } else {
+    // multipliers first, while nothing else is live (exp and division need
+    // many temporaries); then the lines are loaded
+    T opl[E], opn[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      double ma, mb;
+      z_mode_ops<MODE>(a.op, a.top, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), ma, mb);
+      opl[e] = (T)ma;
+      opn[e] = (T)mb;
+    }
+    cplx<T> v[E];
+    if constexpr (MODE == ZM_PFC) {
+      // A = Fz^-1(opL*Fz(A) + opN*Fz(B))
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        acc[e] = la[t + TL * e];
+        v[e] = lb[t + TL * e];
+      }
+      LineFft<T, Plan, -1, SPLIT, true>::run(acc, t, ll, a.tw_z);
+#pragma unroll
+      for (int e = 0; e < E; ++e) acc[e] = c_scale(acc[e], opl[e]);
+      LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_z);
+#pragma unroll
+      for (int e = 0; e < E; ++e) acc[e] = c_add(acc[e], c_scale(v[e], opn[e]));
+      if (a.zero_k0 && l == 0 && t == 0) acc[0] = cplx<T>{0, 0};
+    } else {
+      // A = Fz^-1(expL*A + nW*Fz(B)), A already a full spectrum
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = lb[t + TL * e];
+      LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_z);
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        acc[e] = c_add(c_scale(la[t + TL * e], opl[e]), c_scale(v[e], opn[e]));
+    }
   }
-  LineFft<T, Plan, -1, SPLIT, true>::run(acc, t, ll, a.tw_z);
-#pragma unroll
-  for (int e = 0; e < E; ++e) acc[e] = c_scale(acc[e], opl[e]);
-  LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_z);
-#pragma unroll
-  for (int e = 0; e < E; ++e) acc[e] = c_add(acc[e], c_scale(v[e], opn[e]));
-  if (a.zero_k0 && l == 0 && t == 0) acc[0] = cplx<T>{0, 0};
   LineFft<T, Plan, +1, SPLIT, true>::run(acc, t, ll, a.tw_z);
   if (active) {
+    cplx<T> *__restrict__ out = MODE == ZM_TMF ? lb : la;
 #pragma unroll
-    for (int e = 0; e < E; ++e) la[t + TL * e] = acc[e];
+    for (int e = 0; e < E; ++e) out[t + TL * e] = acc[e];
   }
 }
 
@@ -359,6 +403,148 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
 #pragma unroll
     for (int e = 0; e < E; ++e) out[t + TL * e] = v[e];
   }
+}
+
+// ----------------------------------------------------------------------------
+// Half-spectrum rows of G consecutive y rows of plane z, transposed store into
+// T1 (out[(z*W + kx)*ny + y]): v[r] holds X[TL*r + t] of row y0 + g, and
+// xm = X[M] (thread t == 0).  Through a TL x G LDS tile when TILE.
+// ----------------------------------------------------------------------------
+template <typename T, int E, int TL, int G, bool TILE>
+__device__ __forceinline__ void t1_store_rows(const cplx<T> (&v)[E], cplx<T> xm,
+                                              cplx<T> *__restrict__ out, T *lds,
+                                              int t, int g, int z, int y0,
+                                              int ny, int W) {
+  constexpr int M = E * TL;
+  const int y = y0 + g;
+  if (y < ny && t == 0) out[((long)z * W + M) * ny + y] = xm;
+  if constexpr (!TILE) {
+    if (y < ny) {
+#pragma unroll
+      for (int r = 0; r < E; ++r) out[((long)z * W + TL * r + t) * ny + y] = v[r];
+    }
+  } else {
+    T *tr = lds, *ti = lds + TL * (G + 1);
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+      tr[tile_idx<G>(t, g)] = v[r].x;
+      ti[tile_idx<G>(t, g)] = v[r].y;
+      __syncthreads();
+      const int j = threadIdx.x / G, gg = threadIdx.x % G;
+      if (y0 + gg < ny)
+        out[((long)z * W + TL * r + j) * ny + y0 + gg] =
+            cplx<T>{tr[tile_idx<G>(j, gg)], ti[tile_idx<G>(j, gg)]};
+      __syncthreads();
+    }
+  }
+}
+
+// Forward half-length transform of the G rows held as v (z[m] = x[2m] +
+// i*x[2m+1]) and the R2C split; on return v[e] = X[t + TL*e] and xm = X[M]
+// (valid on t == 0).
+template <typename T, typename Plan, int G>
+__device__ __forceinline__ void row_r2c(cplx<T> (&v)[Plan::E], cplx<T> &xm, T *lds,
+                                        int t, int g, const cplx<T> *tw,
+                                        const cplx<T> *wn) {
+  constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N, RP = row_padded(M);
+  constexpr int PL = t_plane<Plan, G>();
+  const LdsLine<T> ll{lds, g * RP, 1, PL};
+  LineFft<T, Plan, -1, false, true>::run(v, t, ll, tw);
+  cplx<T> pz[E];
+  row_fetch_partner<T, E, TL>(v, pz, t, lds + g * RP, lds + PL + g * RP,
+                              [](int k) { return k == 0 ? 0 : M - k; });
+  xm = cplx<T>{v[0].x - v[0].y, (T)0};
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = r2c_split(v[e], pz[e], wn[t + TL * e]);
+}
+
+// ----------------------------------------------------------------------------
+// Tungsten, psi -> T1 (A1): R2C of G consecutive rows of plane z with the
+// nx/2-point plan, transposed store.
+// ----------------------------------------------------------------------------
+template <typename T, typename Plan, int G, bool TILE>
+__global__ void __launch_bounds__(G *Plan::TL) k_tx_r2c(TStepArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N;
+  __shared__ T lds[2 * t_plane<Plan, G>()];
+  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
+  const int ngy = (a.ny + G - 1) / G;
+  const int z = blockIdx.x / ngy, y0 = (blockIdx.x % ngy) * G;
+  const int y = y0 + g < a.ny ? y0 + g : y0;
+  const cplx<T> *__restrict__ row =
+      reinterpret_cast<const cplx<T> *>(a.psi_in + ((long)z * a.ny + y) * (2L * M));
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = row[t + TL * e];
+  cplx<T> xm;
+  row_r2c<T, Plan, G>(v, xm, lds, t, g, a.tw_xm, a.wn);
+  t1_store_rows<T, E, TL, G, TILE>(v, xm, a.A1, lds, t, g, z, y0, a.ny, a.W);
+}
+
+// ----------------------------------------------------------------------------
+// Tungsten fused x pass on T1 (B1, in place): for G consecutive rows of plane
+// z, the mean-field half spectra (inverse y and z done, 1/(nx ny nz) folded
+// into fMF) are loaded transposed, brought to real space (nx/2-point
+// inverse), combined with the rows of psi into N(psi, psi_MF), transformed
+// forward and stored transposed.  psi_MF goes to HBM only when mf_out is set.
+// ----------------------------------------------------------------------------
+template <typename T, typename Plan, int G, bool TILE>
+__global__ void __launch_bounds__(G *Plan::TL) k_tx_tung(TStepArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N, RP = row_padded(M);
+  constexpr int PL = t_plane<Plan, G>();
+  constexpr int W = M + 1;
+  __shared__ T lds[tx_inv_lds<T, Plan, G, TILE>()];
+  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
+  const int ngy = (a.ny + G - 1) / G;
+  const int z = blockIdx.x / ngy, y0 = (blockIdx.x % ngy) * G;
+  const int y = y0 + g;
+  const int yy = y < a.ny ? y : y0;
+  cplx<T> *__restrict__ spec = a.B1;
+
+  cplx<T> v[E];
+  if constexpr (!TILE) {
+    const cplx<T> *__restrict__ col = spec + (long)z * a.W * a.ny + yy;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = t + TL * e;
+      v[e] = c2r_merge(col[(long)k * a.ny], col[(long)(M - k) * a.ny], a.wn[k], k == 0);
+    }
+  } else {
+    T *sr = lds, *si = lds + W * (G + 1);
+    for (int i = threadIdx.x; i < W * G; i += G * TL) {
+      const int kx = i / G, gg = i % G;
+      cplx<T> w{0, 0};
+      if (y0 + gg < a.ny) w = spec[((long)z * a.W + kx) * a.ny + y0 + gg];
+      sr[tile_idx<G>(kx, gg)] = w.x;
+      si[tile_idx<G>(kx, gg)] = w.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = t + TL * e;
+      const cplx<T> xa{sr[tile_idx<G>(k, g)], si[tile_idx<G>(k, g)]};
+      const cplx<T> xb{sr[tile_idx<G>(M - k, g)], si[tile_idx<G>(M - k, g)]};
+      v[e] = c2r_merge(xa, xb, a.wn[k], k == 0);
+    }
+    __syncthreads(); // the staging area becomes the FFT exchange area
+  }
+  const LdsLine<T> ll{lds, g * RP, 1, PL};
+  LineFft<T, Plan, +1, false, true>::run(v, t, ll, a.tw_xm);
+  // v[e] = (psi_MF[2k], psi_MF[2k+1]), k = t + TL*e
+  const long ro = ((long)z * a.ny + yy) * (2L * M);
+  if (a.mf_out && y < a.ny) {
+    cplx<T> *__restrict__ mo = reinterpret_cast<cplx<T> *>(a.mf_out + ro);
+#pragma unroll
+    for (int e = 0; e < E; ++e) mo[t + TL * e] = v[e];
+  }
+  const cplx<T> *__restrict__ pr = reinterpret_cast<const cplx<T> *>(a.psi_in + ro);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const cplx<T> u = pr[t + TL * e];
+    v[e] = cplx<T>{a.tnl(u.x, v[e].x), a.tnl(u.y, v[e].y)};
+  }
+  cplx<T> xm;
+  row_r2c<T, Plan, G>(v, xm, lds, t, g, a.tw_xm, a.wn);
+  t1_store_rows<T, E, TL, G, TILE>(v, xm, spec, lds, t, g, z, y0, a.ny, a.W);
 }
 
 } // namespace pfc_amd

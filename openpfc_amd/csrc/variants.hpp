@@ -28,11 +28,17 @@ enum Role {
   ROLE_T_Z = 9,         // T2 z pass
   ROLE_T_Y_INV = 10,    // T2 -> T1
   ROLE_T_X_INV = 11,    // T1 -> psi
-  ROLE_COUNT = 12
+  ROLE_X_TUNG = 12,     // Tungsten natural step: fused inverse x, N, forward x
+  ROLE_T_X_R2C = 13,    // Tungsten transposed step: psi -> T1 (psi only)
+  ROLE_T_X_TUNG = 14,   // Tungsten transposed step: fused x pass on T1
+  ROLE_COUNT = 15
 };
 
 // Kinds of transposed-step kernels (one variant list per kind and length).
-enum TKind { TK_X_FWD = 0, TK_Y_FWD = 1, TK_Z = 2, TK_Y_INV = 3, TK_X_INV = 4 };
+enum TKind {
+  TK_X_FWD = 0, TK_Y_FWD = 1, TK_Z = 2, TK_Y_INV = 3, TK_X_INV = 4,
+  TK_X_R2C = 5, TK_X_TUNG = 6, TK_COUNT = 7
+};
 
 template <typename T> struct ColV {
   std::string name;
@@ -56,6 +62,12 @@ template <typename T> struct PXV {
   hipError_t (*fn)(PfcXArgs<T> a, hipStream_t s);
 };
 
+template <typename T> struct TXV {
+  std::string name;
+  int plan_id;
+  hipError_t (*fn)(TungXArgs<T> a, hipStream_t s);
+};
+
 template <typename T> struct TV {
   std::string name;
   int plan_id;
@@ -68,6 +80,7 @@ template <typename T> const std::vector<ZV<T>> &pfcz_variants(int n);
 template <typename T> const std::vector<XV<T>> &xr2c_variants(int m);
 template <typename T> const std::vector<XV<T>> &xc2r_variants(int m);
 template <typename T> const std::vector<PXV<T>> &pfcx_variants(int n);
+template <typename T> const std::vector<TXV<T>> &tungx_variants(int m);
 template <typename T> const std::vector<TV<T>> &t_variants(int kind, int n);
 #define PFC_AMD_DECLARE_LISTS(T)                                               \
   template <> const std::vector<ColV<T>> &col_variants<T>(int);                \
@@ -75,6 +88,7 @@ template <typename T> const std::vector<TV<T>> &t_variants(int kind, int n);
   template <> const std::vector<XV<T>> &xr2c_variants<T>(int);                 \
   template <> const std::vector<XV<T>> &xc2r_variants<T>(int);                 \
   template <> const std::vector<PXV<T>> &pfcx_variants<T>(int);               \
+  template <> const std::vector<TXV<T>> &tungx_variants<T>(int);               \
   template <> const std::vector<TV<T>> &t_variants<T>(int, int);
 PFC_AMD_DECLARE_LISTS(double)
 PFC_AMD_DECLARE_LISTS(float)

@@ -71,7 +71,19 @@ template <typename T, typename Plan, int L, bool S>
 hipError_t pfcz_launch(PfcZArgs<T> a, long ny, hipStream_t s) {
   a.ntx = (a.W + L - 1) / L;
   dim3 grid((unsigned)(a.ntx * ny)), block(L * Plan::TL);
-  hipLaunchKernelGGL((k_pfc_z<T, Plan, L, S>), grid, block, 0, s, a);
+  switch (a.mode) {
+  case ZM_PFC:
+    hipLaunchKernelGGL((k_pfc_z<T, Plan, L, S, ZM_PFC>), grid, block, 0, s, a);
+    break;
+  case ZM_TMF:
+    hipLaunchKernelGGL((k_pfc_z<T, Plan, L, S, ZM_TMF>), grid, block, 0, s, a);
+    break;
+  case ZM_TCOMB:
+    hipLaunchKernelGGL((k_pfc_z<T, Plan, L, S, ZM_TCOMB>), grid, block, 0, s, a);
+    break;
+  default:
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -93,6 +105,13 @@ template <typename T, typename Plan, int LPB>
 hipError_t pfcx_launch(PfcXArgs<T> a, hipStream_t s) {
   dim3 grid((unsigned)((a.rows + LPB - 1) / LPB)), block(LPB * Plan::TL);
   hipLaunchKernelGGL((k_pfc_x<T, Plan, LPB>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T, typename Plan, int LPB>
+hipError_t tungx_launch(TungXArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)((a.rows + LPB - 1) / LPB)), block(LPB * Plan::TL);
+  hipLaunchKernelGGL((k_x_tung<T, Plan, LPB>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 
@@ -222,6 +241,18 @@ template <typename T, int N> std::vector<XV<T>> make_xc2r() {
   });
   return v;
 }
+template <typename T, int N> std::vector<TXV<T>> make_tungx() {
+  std::vector<TXV<T>> v;
+  for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+    using P = decltype(p);
+    rows_for_plan<T, P, sweep_len<N>()>(v, pid, [&](auto r) {
+      constexpr int R = decltype(r)::value;
+      v.push_back({plan_name<P>() + "/rows" + std::to_string(R), pid,
+                   &tungx_launch<T, P, R>});
+    });
+  });
+  return v;
+}
 template <typename T, int N> std::vector<PXV<T>> make_pfcx() {
   std::vector<PXV<T>> v;
   for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
@@ -244,7 +275,9 @@ hipError_t tx_fwd_launch(TStepArgs<T> a, hipStream_t s) {
 }
 template <typename T, typename Plan, int G, bool S, bool TILE>
 hipError_t ty_fwd_launch(TStepArgs<T> a, hipStream_t s) {
-  dim3 grid((unsigned)(a.W * ((a.nz + G - 1) / G)), 2), block(G * Plan::TL);
+  if (a.nfields != 1 && a.nfields != 2) return hipErrorInvalidValue;
+  dim3 grid((unsigned)(a.W * ((a.nz + G - 1) / G)), (unsigned)a.nfields),
+      block(G * Plan::TL);
   hipLaunchKernelGGL((k_ty_fwd<T, Plan, G, S, TILE>), grid, block, 0, s, a);
   return hipGetLastError();
 }
@@ -252,7 +285,19 @@ template <typename T, typename Plan, int G, bool S, bool TILE>
 hipError_t tz_launch(TStepArgs<T> a, hipStream_t s) {
   const long lines = (long)a.W * a.ny;
   dim3 grid((unsigned)((lines + G - 1) / G)), block(G * Plan::TL);
-  hipLaunchKernelGGL((k_tz<T, Plan, G, S>), grid, block, 0, s, a);
+  switch (a.mode) {
+  case ZM_PFC:
+    hipLaunchKernelGGL((k_tz<T, Plan, G, S, ZM_PFC>), grid, block, 0, s, a);
+    break;
+  case ZM_TMF:
+    hipLaunchKernelGGL((k_tz<T, Plan, G, S, ZM_TMF>), grid, block, 0, s, a);
+    break;
+  case ZM_TCOMB:
+    hipLaunchKernelGGL((k_tz<T, Plan, G, S, ZM_TCOMB>), grid, block, 0, s, a);
+    break;
+  default:
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 template <typename T, typename Plan, int G, bool S, bool TILE>
@@ -268,24 +313,40 @@ hipError_t tx_inv_launch(TStepArgs<T> a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename T, typename Plan, int G, bool S, bool TILE>
+hipError_t tx_r2c_launch(TStepArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
+  hipLaunchKernelGGL((k_tx_r2c<T, Plan, G, TILE>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+template <typename T, typename Plan, int G, bool S, bool TILE>
+hipError_t tx_tung_launch(TStepArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
+  hipLaunchKernelGGL((k_tx_tung<T, Plan, G, TILE>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
 template <typename T, typename Plan, int G, bool S, bool TILE, int KIND>
 constexpr bool t_fits() {
   if (G * Plan::TL > kMaxThreads || G * Plan::TL < 64) return false;
   if (KIND == TK_Z && !TILE) return false; // the z pass has no transposition
   constexpr int fft = (S ? 1 : 2) * G * row_padded(Plan::N) * (int)sizeof(T);
-  if (KIND == TK_X_INV) {
+  // row kernels on the nx/2-point plan use the full exchange only
+  if (KIND == TK_X_INV || KIND == TK_X_TUNG) {
     constexpr int stage = TILE ? 2 * (Plan::N + 1) * (G + 1) * (int)sizeof(T) : 0;
     constexpr int full = 2 * G * row_padded(Plan::N) * (int)sizeof(T);
     return !S && (stage > full ? stage : full) <= kMaxLds;
   }
+  if (KIND == TK_X_R2C) return !S && 2 * G * row_padded(Plan::N) * (int)sizeof(T) <= kMaxLds;
   return fft <= kMaxLds;
 }
 
 template <typename T, typename Plan, int G, bool S, bool TILE, int KIND>
 void add_t(std::vector<TV<T>> &v, int pid) {
   if constexpr (t_fits<T, Plan, G, S, TILE, KIND>()) {
+    constexpr bool rows_m = KIND == TK_X_INV || KIND == TK_X_R2C || KIND == TK_X_TUNG;
     const std::string nm = plan_name<Plan>() + "/G" + std::to_string(G) +
-                           (KIND == TK_X_INV ? "" : (S ? "/split" : "/full")) +
+                           (rows_m ? "" : (S ? "/split" : "/full")) +
                            (TILE ? "" : "/direct");
     if constexpr (KIND == TK_X_FWD)
       v.push_back({nm, pid, &tx_fwd_launch<T, Plan, G, S, TILE>});
@@ -296,6 +357,10 @@ void add_t(std::vector<TV<T>> &v, int pid) {
       v.push_back({nm, pid, &ty_inv_launch<T, Plan, G, S, TILE>});
     if constexpr (KIND == TK_X_INV)
       v.push_back({nm, pid, &tx_inv_launch<T, Plan, G, S, TILE>});
+    if constexpr (KIND == TK_X_R2C)
+      v.push_back({nm, pid, &tx_r2c_launch<T, Plan, G, S, TILE>});
+    if constexpr (KIND == TK_X_TUNG)
+      v.push_back({nm, pid, &tx_tung_launch<T, Plan, G, S, TILE>});
   }
 }
 
@@ -340,6 +405,8 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
   default: break;                                                              \
   }
 
+// Lists of the column- and row-shaped passes (one translation unit per
+// precision) and of the transposed-step kernels (another one).
 #define PFC_AMD_DEFINE_LISTS(T)                                                \
   template <> const std::vector<ColV<T>> &col_variants<T>(int n) {             \
     static const std::vector<ColV<T>> empty;                                   \
@@ -366,13 +433,21 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
     PFC_AMD_SWITCH_LEN(n, static const auto v = make_pfcx<T, N_>(); return v;) \
     return empty;                                                              \
   }                                                                            \
+  template <> const std::vector<TXV<T>> &tungx_variants<T>(int n) {            \
+    static const std::vector<TXV<T>> empty;                                    \
+    PFC_AMD_SWITCH_LEN(n, static const auto v = make_tungx<T, N_>(); return v;) \
+    return empty;                                                              \
+  }
+
+#define PFC_AMD_DEFINE_T_LISTS(T)                                              \
   template <> const std::vector<TV<T>> &t_variants<T>(int kind, int n) {       \
     static const std::vector<TV<T>> empty;                                     \
-    if (kind < 0 || kind > 4) return empty;                                    \
-    PFC_AMD_SWITCH_LEN(n, static const std::vector<TV<T>> v[5] = {             \
+    if (kind < 0 || kind >= TK_COUNT) return empty;                            \
+    PFC_AMD_SWITCH_LEN(n, static const std::vector<TV<T>> v[TK_COUNT] = {      \
         make_t_kind<T, N_, 0>(), make_t_kind<T, N_, 1>(),                      \
         make_t_kind<T, N_, 2>(), make_t_kind<T, N_, 3>(),                      \
-        make_t_kind<T, N_, 4>()};                                              \
+        make_t_kind<T, N_, 4>(), make_t_kind<T, N_, 5>(),                      \
+        make_t_kind<T, N_, 6>()};                                              \
                           return v[kind];)                                     \
     return empty;                                                              \
   }

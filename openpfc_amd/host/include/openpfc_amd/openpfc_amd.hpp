@@ -6,6 +6,7 @@
 #include "field_modifier.hpp"
 #include "model.hpp"
 #include "pfc_model_hip.hpp"
+#include "tungsten_model_hip.hpp"
 #include "simulator.hpp"
 #include "time.hpp"
 #include "world.hpp"

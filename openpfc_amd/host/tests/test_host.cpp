@@ -6,6 +6,10 @@
 //   test_host gpu-sim N STEPS OUT.bin  PFCModelHIP driven by Simulator from
 //                                      Constant(-0.2) + UniformNoise(1234, +-0.01);
 //                                      writes the final field (raw f64)
+//   test_host gpu-tung STEPS OUT MFOUT TungstenHIP<double> on the reference's
+//                                      CPU-vs-HIP recipe (32^3, spacing 1,
+//                                      T = 0.5, dt = 0.01, -0.4 + 0.1 sin(2 pi i/n));
+//                                      writes psi and psi_MF (raw f64)
 #include <openpfc_amd/openpfc_amd.hpp>
 
 #include <cmath>
@@ -261,6 +265,34 @@ static int run_gpu_sim(int n, int steps, const char *out) {
   return g_fail;
 }
 
+// apps/tungsten/tests/test_tungsten_cpu_vs_hip.cpp:30-90 through the C++
+// drop-in: TungstenHIP<double> over a HipFft, psi set through the host field.
+static int run_gpu_tung(int steps, const char *out, const char *mfout) {
+  World w = domain::create(GridSize({32, 32, 32}), PhysicalOrigin({0.0, 0.0, 0.0}),
+                           GridSpacing({1.0, 1.0, 1.0}));
+  fft::HipFft f(w);
+  TungstenHIP<double> model(f, w);
+  model.params.set_n0(-0.4);
+  model.params.set_T(0.5);
+  initialize(model, 0.01);
+  RealField &psi = get_real_field(model, "psi");
+  for (size_t i = 0; i < psi.size(); ++i)
+    psi[i] = -0.4 + 0.1 * std::sin(2.0 * M_PI * (double)i / (double)psi.size());
+  model.finalize_after_field_modifiers();
+  for (int i = 0; i < steps; ++i) model.step(0.0);
+  const RealField &res = model.get_psi_for_writer();
+  const auto mf = model.get_psiMF().to_host();
+  std::ofstream o(out, std::ios::binary);
+  o.write(reinterpret_cast<const char *>(res.data()), (std::streamsize)(res.size() * sizeof(double)));
+  std::ofstream m(mfout, std::ios::binary);
+  m.write(reinterpret_cast<const char *>(mf.data()), (std::streamsize)(mf.size() * sizeof(double)));
+  CHECK(model.get_allocated_memory_bytes() > 0);
+  // a float plan does not fit TungstenHIP<double>
+  fft::HipFft f32(w, PFC_HIP_F32);
+  CHECK(throws<std::invalid_argument>([&] { TungstenHIP<double> bad(f32, w); }));
+  return g_fail;
+}
+
 int main(int argc, char **argv) {
   const std::string mode = argc > 1 ? argv[1] : "cpu";
   int rc = 0;
@@ -268,8 +300,10 @@ int main(int argc, char **argv) {
     if (mode == "cpu") rc = run_cpu();
     else if (mode == "gpu-fft") rc = run_gpu_fft();
     else if (mode == "gpu-sim" && argc == 5) rc = run_gpu_sim(std::atoi(argv[2]), std::atoi(argv[3]), argv[4]);
+    else if (mode == "gpu-tung" && argc == 5) rc = run_gpu_tung(std::atoi(argv[2]), argv[3], argv[4]);
     else {
-      std::fprintf(stderr, "usage: test_host cpu | gpu-fft | gpu-sim N STEPS OUT\n");
+      std::fprintf(stderr, "usage: test_host cpu | gpu-fft | gpu-sim N STEPS OUT | "
+                           "gpu-tung STEPS OUT MFOUT\n");
       return 2;
     }
   } catch (const std::exception &e) {

@@ -562,10 +562,11 @@ void orc_tungsten_operators(int nx, int ny, int nz, double dx, double dy,
   const double fx = two_pi / (dx * nx), fy = two_pi / (dy * ny),
                fz = two_pi / (dz * nz);
   const int w = nx / 2 + 1;
-  size_t idx = 0;
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < w; ++i) {
+        const size_t idx = ((size_t)k * ny + j) * w + i;
         double ki = orc_k_component(i, nx, fx);
         double kj = orc_k_component(j, ny, fy);
         double kk = orc_k_component(k, nz, fz);
@@ -577,7 +578,6 @@ void orc_tungsten_operators(int nx, int ny, int nz, double dx, double dy,
         filterMF[idx] = fMF;
         expL[idx] = e;
         nW[idx] = kLap * phi1;
-        ++idx;
       }
 }
 

@@ -52,3 +52,25 @@ def test_host_pfc_model_simulator_matches_oracle(torch_gpu, tmp_path, n, steps):
     ref = O.pfc_step(psi0, opL, opN, steps)
     err = np.abs(got - ref).max() / np.abs(ref).max()
     assert err <= 1e-10, err
+
+
+@pytest.mark.gpu
+def test_host_tungsten_hip_cpu_vs_hip_recipe(torch_gpu, tmp_path):
+    """TungstenHIP<double> (C++ drop-in) on the reference's CPU-vs-HIP recipe
+    (apps/tungsten/tests/test_tungsten_cpu_vs_hip.cpp:30-90): every point
+    within 1e-10 of the oracle after 10 steps; psi_MF of the last step too."""
+    from openpfc_amd.tungsten import TungstenParams
+    out, mfout = tmp_path / "psi.bin", tmp_path / "mf.bin"
+    r = _run("gpu-tung", "10", str(out), str(mfout))
+    assert r.returncode == 0, r.stderr
+    n = 32
+    got = np.fromfile(out, dtype=np.float64).reshape(n, n, n)
+    mf = np.fromfile(mfout, dtype=np.float64).reshape(n, n, n)
+    prm = TungstenParams(n0=-0.4, T=0.5)
+    fMF, expL, nW = O.tungsten_operators((n, n, n), (1.0, 1.0, 1.0), prm.operator_params(),
+                                         0.01)
+    N = n ** 3
+    psi0 = (-0.4 + 0.1 * np.sin(2.0 * math.pi * np.arange(N) / N)).reshape(n, n, n)
+    ref, ref_mf = O.tungsten_step(psi0, fMF, expL, nW, prm.nonlinear_params(), 10)
+    assert float(np.abs(got - ref).max()) < 1e-10
+    assert float(np.abs(mf - ref_mf).max()) < 1e-10

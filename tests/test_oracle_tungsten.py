@@ -93,3 +93,20 @@ def test_tungsten_conserves_mean():
     psi = initial_field(n, dx)
     out, _ = O.tungsten_step(psi, fMF, expL, nW, nl, 5)
     assert abs(out.mean() - psi.mean()) <= 1e-12
+
+
+def test_host_tungsten_params_match_golden_derivation():
+    """openpfc_amd.TungstenParams (the host mirror of tungsten_params.hpp) gives
+    the parameter vectors that reproduce the golden norms above."""
+    from openpfc_amd.tungsten import TungstenParams
+    prm = TungstenParams.from_dict({**{k: v for k, v in PARAMS.items() if k != "lambda_"},
+                                    "lambda": PARAMS["lambda_"]})
+    op, nl = derived(PARAMS)
+    assert prm.operator_params() == op
+    assert prm.nonlinear_params() == nl
+    assert prm.abi_params((1.0, 2.0, 3.0)) == [1.0, 2.0, 3.0] + op + nl[:4]
+    # defaults are the tungsten_single_seed.json values
+    assert TungstenParams() == TungstenParams.from_dict(
+        {k: v for k, v in PARAMS.items() if k != "lambda_"} | {"lambda": 0.22})
+    with pytest.raises(KeyError):
+        TungstenParams.from_dict({"no_such": 1.0})
