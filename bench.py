@@ -73,6 +73,8 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
     `mode` is the z-pass kind (0 PFC, 1 Tungsten mean field, 2 Tungsten combine)."""
     import re
     parts = variant.split("/")
+    if kernel == "k_tx_fwd" and "r2c2" in parts:
+        kernel, n = "k_tx_fwd2", n // 2
     radices = parts[0][1:]
     e = parts[1][1:]
     nn = n

@@ -39,7 +39,8 @@ enum {
   PFC_HIP_EUNSUPPORTED = 2, /* extent or model not supported */
   PFC_HIP_EDEVICE = 3,      /* HIP runtime error (launch, copy, ...) */
   PFC_HIP_ENOMEM = 4,       /* device allocation failed */
-  PFC_HIP_ESTATE = 5        /* call out of order (e.g. step before operators) */
+  PFC_HIP_ESTATE = 5,       /* call out of order (e.g. step before operators) */
+  PFC_HIP_EIO = 6           /* file could not be opened, read or written */
 };
 
 enum { PFC_HIP_F64 = 0, PFC_HIP_F32 = 1 };
@@ -271,6 +272,41 @@ int pfc_hip_variant_set(pfc_hip_plan_t plan, int role, int index);
  */
 int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
                          double *times_ms, int max_times);
+
+/*
+ * Raw binary field files (the reference's BinaryWriter / BinaryReader:
+ * include/openpfc/frontend/io/binary_writer.hpp:76-205,
+ * include/openpfc/kernel/simulation/binary_reader.hpp:79-200).  A file holds
+ * the global array of global3[0]*global3[1]*global3[2] values, x fastest
+ * (the MPI_ORDER_FORTRAN subarray layout), each a native little-endian f64
+ * (kind PFC_HIP_IO_COMPLEX: f64 re, f64 im), no header.  `field` holds the
+ * brick local3 at offset3 of that array, x fastest, in `precision` (FP32
+ * values are widened to f64 on write and rounded on read).  The domain
+ * must satisfy 0 <= offset and offset + local <= global with positive extents
+ * (domain_geometry.hpp:36-60), else PFC_HIP_EINVAL.  truncate != 0 empties
+ * the file first (the reference's single-writer MPI_File_set_size(0));
+ * bricks of one global array written by several calls use truncate = 0
+ * after the first.  A file shorter than the domain needs gives PFC_HIP_EIO
+ * ("Short read ...").
+ *
+ * pfc_hip_field_write / _read: `field` is device memory; the data are staged
+ * through pinned chunks, copies on `stream` overlapping the file IO; the
+ * call returns when the file (or the device buffer) is complete.
+ * pfc_hip_field_write_host / _read_host: `field` is host memory; no device
+ * is needed.
+ */
+enum { PFC_HIP_IO_REAL = 0, PFC_HIP_IO_COMPLEX = 1 };
+int pfc_hip_field_write(const char *path, const void *field, int precision, int kind,
+                        const int *global3, const int *local3, const int *offset3,
+                        int truncate, void *stream);
+int pfc_hip_field_read(const char *path, void *field, int precision, int kind,
+                       const int *global3, const int *local3, const int *offset3,
+                       void *stream);
+int pfc_hip_field_write_host(const char *path, const void *field, int precision, int kind,
+                             const int *global3, const int *local3, const int *offset3,
+                             int truncate);
+int pfc_hip_field_read_host(const char *path, void *field, int precision, int kind,
+                            const int *global3, const int *local3, const int *offset3);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,9 @@ PFC_HIP_EUNSUPPORTED = 2
 PFC_HIP_EDEVICE = 3
 PFC_HIP_ENOMEM = 4
 PFC_HIP_ESTATE = 5
+PFC_HIP_EIO = 6
+IO_REAL = 0
+IO_COMPLEX = 1
 
 F64 = 0
 F32 = 1
@@ -28,6 +31,7 @@ TUNGSTEN_NPARAMS = 17
 # Every symbol include/pfc_hip.h declares, with its ctypes signature.
 _vp = ctypes.c_void_p
 _sz = ctypes.POINTER(ctypes.c_size_t)
+_ip3 = ctypes.POINTER(ctypes.c_int)
 SIGNATURES = {
     "pfc_hip_abi_version": (ctypes.c_int, []),
     "pfc_hip_last_error": (ctypes.c_char_p, []),
@@ -55,6 +59,15 @@ SIGNATURES = {
                                             ctypes.c_double, _vp]),
     "pfc_hip_pfc_operators": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "pfc_hip_tungsten_operators": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "pfc_hip_field_write": (ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.c_int, ctypes.c_int,
+                                           _ip3, _ip3, _ip3, ctypes.c_int, _vp]),
+    "pfc_hip_field_read": (ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.c_int, ctypes.c_int,
+                                          _ip3, _ip3, _ip3, _vp]),
+    "pfc_hip_field_write_host": (ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.c_int,
+                                                ctypes.c_int, _ip3, _ip3, _ip3,
+                                                ctypes.c_int]),
+    "pfc_hip_field_read_host": (ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.c_int,
+                                               ctypes.c_int, _ip3, _ip3, _ip3]),
     "pfc_hip_tungsten_step": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
     "pfc_hip_pfc_set_mean": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
     "pfc_hip_variant_count": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),

@@ -16,9 +16,14 @@
 
 using namespace pfc_amd;
 
+namespace pfc_amd {
+// message of the last failed ABI call on this thread (field_io.hip sets it too)
+thread_local std::string g_last_error;
+} // namespace pfc_amd
+
 namespace {
 
-thread_local std::string g_last_error;
+using pfc_amd::g_last_error;
 
 int fail(int code, const std::string &msg) {
   g_last_error = msg;
@@ -443,7 +448,7 @@ int step_transposed(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev)
   for (int k = 0; k < 5; ++k) {
     const int n = p->role_len(roles[k]);
     const auto &v = t_variants<T>(k, n)[p->var[roles[k]]];
-    const cplx<T> *tw = twp<T>(p, n, v.plan_id);
+    const cplx<T> *tw = twp<T>(p, v.half ? n / 2 : n, v.plan_id);
     a.tw_x = a.tw_xm = a.tw_y = a.tw_z = tw;
     PFC_HIP_TRY(v.fn(a, s), what[k]);
     mark(k + 1);
@@ -569,7 +574,7 @@ int step_tung_transposed(pfc_hip_plan_s *p, void *psi, void *mf, hipStream_t s,
       b.A1 = a.B1;
       b.A2 = a.B2;
     }
-    b.tw_x = b.tw_xm = b.tw_y = b.tw_z = twp<T>(p, n, v.plan_id);
+    b.tw_x = b.tw_xm = b.tw_y = b.tw_z = twp<T>(p, v.half ? n / 2 : n, v.plan_id);
     PFC_HIP_TRY(v.fn(b, s), ps.what);
     mark(k + 1);
   }

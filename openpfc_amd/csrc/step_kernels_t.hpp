@@ -9,6 +9,7 @@
 //   T2   [kx][ky][z] complex, z contiguous        (after forward y; z pass)
 //
 //   k_tx_fwd  psi -> T1 (Psi and N):  rows of psi, transposed store
+//             (k_tx_fwd2: the same with two nx/2-point R2Cs)
 //   k_ty_fwd  T1 -> T2 (both fields): y lines of T1, transposed store
 //   k_tz      T2 -> T2 (in place):    z lines; forward both, ETD combine, inverse
 //   k_ty_inv  T2 -> T1:               transposed load, y lines of T1
@@ -545,6 +546,37 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_tung(TStepArgs<T> a) {
   cplx<T> xm;
   row_r2c<T, Plan, G>(v, xm, lds, t, g, a.tw_xm, a.wn);
   t1_store_rows<T, E, TL, G, TILE>(v, xm, spec, lds, t, g, z, y0, a.ny, a.W);
+}
+
+// ----------------------------------------------------------------------------
+// PFC forward x as two half-length R2Cs: psi -> T1 (A1: Psi, B1: N).  The G
+// rows of plane z are read once as nx/2 complex pairs; the transform of psi
+// and then that of N(psi) run on the nx/2-point plan, each followed by its
+// transposed store.  Same bytes as k_tx_fwd, half the transform length.
+// ----------------------------------------------------------------------------
+template <typename T, typename Plan, int G, bool TILE>
+__global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd2(TStepArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N;
+  __shared__ T lds[2 * t_plane<Plan, G>()];
+  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
+  const int ngy = (a.ny + G - 1) / G;
+  const int z = blockIdx.x / ngy, y0 = (blockIdx.x % ngy) * G;
+  const int y = y0 + g < a.ny ? y0 + g : y0;
+  const cplx<T> *__restrict__ row =
+      reinterpret_cast<const cplx<T> *>(a.psi_in + ((long)z * a.ny + y) * (2L * M));
+  cplx<T> u[E], v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    u[e] = row[t + TL * e];
+    v[e] = u[e];
+  }
+  cplx<T> xm;
+  row_r2c<T, Plan, G>(v, xm, lds, t, g, a.tw_xm, a.wn);
+  t1_store_rows<T, E, TL, G, TILE>(v, xm, a.A1, lds, t, g, z, y0, a.ny, a.W);
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = cplx<T>{a.nl(u[e].x), a.nl(u[e].y)};
+  row_r2c<T, Plan, G>(v, xm, lds, t, g, a.tw_xm, a.wn);
+  t1_store_rows<T, E, TL, G, TILE>(v, xm, a.B1, lds, t, g, z, y0, a.ny, a.W);
 }
 
 } // namespace pfc_amd

@@ -72,6 +72,7 @@ template <typename T> struct TV {
   std::string name;
   int plan_id;
   hipError_t (*fn)(TStepArgs<T> a, hipStream_t s);
+  int half = 0; // 1: the kernel runs the n/2-point plan `plan_id` of a length-n role
 };
 
 // Variant lists for a line length n (empty when n is unsupported).

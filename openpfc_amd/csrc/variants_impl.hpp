@@ -326,6 +326,13 @@ hipError_t tx_tung_launch(TStepArgs<T> a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename T, typename Plan, int G, bool S, bool TILE>
+hipError_t tx_fwd2_launch(TStepArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
+  hipLaunchKernelGGL((k_tx_fwd2<T, Plan, G, TILE>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
 template <typename T, typename Plan, int G, bool S, bool TILE, int KIND>
 constexpr bool t_fits() {
   if (G * Plan::TL > kMaxThreads || G * Plan::TL < 64) return false;
@@ -379,12 +386,45 @@ void t_for_plan(std::vector<TV<T>> &v, int pid) {
   add_t<T, Plan, G0 / 2, false, false, KIND>(v, pid);
 }
 
+// PFC forward x on the nx/2-point plans (k_tx_fwd2), appended to the
+// forward-x list of length N with half = 1.
+template <typename T, typename Plan, int G, bool TILE>
+void add_t_fwd2(std::vector<TV<T>> &v, int pid) {
+  if constexpr (t_fits<T, Plan, G, false, TILE, TK_X_R2C>()) {
+    TV<T> e{plan_name<Plan>() + "/G" + std::to_string(G) + (TILE ? "" : "/direct") + "/r2c2",
+            pid, &tx_fwd2_launch<T, Plan, G, false, TILE>};
+    e.half = 1;
+    v.push_back(e);
+  }
+}
+
 template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
   std::vector<TV<T>> v;
   if constexpr (N >= 128) {
     for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
       t_for_plan<T, decltype(p), KIND>(v, pid);
     });
+    if constexpr (KIND == TK_X_FWD) {
+      for_plans<T, N / 2>(typename Plans<N / 2>::L{}, [&](auto p, int pid) {
+        using P = decltype(p);
+        constexpr int G0 = 128 / (2 * (int)sizeof(T));
+        add_t_fwd2<T, P, G0, true>(v, pid);
+        add_t_fwd2<T, P, G0 / 2, true>(v, pid);
+        add_t_fwd2<T, P, 2 * G0, true>(v, pid);
+        add_t_fwd2<T, P, G0, false>(v, pid);
+      });
+    }
+    if constexpr (KIND == TK_Z) {
+      // one or two lines per block: waves of different blocks drift apart, so
+      // loads of one overlap the transforms of another
+      for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+        using P = decltype(p);
+        add_t<T, P, 1, true, true, KIND>(v, pid);
+        add_t<T, P, 1, false, true, KIND>(v, pid);
+        add_t<T, P, 2, true, true, KIND>(v, pid);
+        add_t<T, P, 2, false, true, KIND>(v, pid);
+      });
+    }
   }
   return v;
 }

@@ -4,6 +4,7 @@
 #include "core.hpp"
 #include "fft.hpp"
 #include "field_modifier.hpp"
+#include "io.hpp"
 #include "model.hpp"
 #include "pfc_model_hip.hpp"
 #include "tungsten_model_hip.hpp"

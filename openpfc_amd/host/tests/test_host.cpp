@@ -22,6 +22,7 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <unistd.h>
 
 using namespace pfc;
 
@@ -141,6 +142,50 @@ static int run_cpu() {
     CHECK(throws<std::invalid_argument>(
         [&] { field::apply(bad, w, f, [](const Real3 &) { return 0.0; }); }, "inbox"));
     CHECK(throws<std::invalid_argument>([&] { m.get_real_field("nope"); }));
+  }
+  // Binary field IO on host buffers (test_binary_writer.cpp recipes): two
+  // x bricks of a {4,2,2} array, round trip, layout, FileReader, errors
+  {
+    const std::string tmpl = std::string("/tmp/openpfc_amd_host_io_") +
+                             std::to_string((long)getpid()) + "_%d.bin";
+    const std::array<int, 3> g{4, 2, 2}, l{2, 2, 2};
+    std::vector<double> whole(16);
+    for (int r = 0; r < 2; ++r) {
+      RealField d(8);
+      for (int i = 0; i < 8; ++i) d[i] = i + 1000.0 * r + 0.25 * (2 * r);
+      BinaryWriter w(tmpl);
+      w.set_domain(g, l, {2 * r, 0, 0});
+      if (r == 0) w.write(0, d);
+      else
+        CHECK(pfc_hip_field_write_host(utils::format_with_number(tmpl, 0).c_str(), d.data(),
+                                       PFC_HIP_F64, PFC_HIP_IO_REAL, g.data(), l.data(),
+                                       std::array<int, 3>{2, 0, 0}.data(), 0) == 0);
+      for (int i = 0; i < 8; ++i) whole[(i / 2) * 4 + 2 * r + (i % 2)] = d[i];
+    }
+    const std::string path = utils::format_with_number(tmpl, 0);
+    RealField all(16);
+    BinaryReader rd;
+    rd.set_domain(g, g, {0, 0, 0});
+    rd.read(path, all);
+    CHECK(all == whole);
+    // FileReader ("from_file") into a model's field
+    World w = domain::create(GridSize({4, 2, 2}), PhysicalOrigin({0, 0, 0}), GridSpacing({1, 1, 1}));
+    BoxOnlyFft f(w.size);
+    CountingModel m(f, w);
+    m.initialize(1.0);
+    FileReader fr(path);
+    fr.apply(m, 0.0);
+    CHECK(m.psi == whole);
+    FileReader missing("/nonexistent/dir/none.bin");
+    CHECK(throws<std::runtime_error>([&] { missing.apply(m, 0.0); }, "FileReader failed"));
+    RealField wrong(15);
+    CHECK(throws<std::runtime_error>([&] { rd.read(path, wrong); }, "buffer size mismatch"));
+    BinaryWriter nodomain(tmpl);
+    CHECK(throws<std::runtime_error>([&] { nodomain.write(0, whole); }, "set_domain"));
+    CHECK(throws<std::invalid_argument>([&] { rd.set_domain({8, 8, 8}, {8, 8, 8}, {4, 0, 0}); }));
+    CHECK(utils::format_with_number("frame_%04d.vti", 3) == "frame_0003.vti");
+    CHECK(throws<std::invalid_argument>([] { utils::format_with_number("a_%s", 1); }));
+    std::remove(path.c_str());
   }
   // backend names (backend_from_string.hpp)
   CHECK(fft::backend_from_string("hip") == fft::Backend::HIP);
