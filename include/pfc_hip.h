@@ -199,6 +199,17 @@ int pfc_hip_ic_add_noise(pfc_hip_plan_t plan, void *psi, unsigned long long seed
                          double lo, double hi, void *stream);
 
 /*
+ * Fixed boundary condition on the device.  With x = x0 + i*dx and
+ * xpos = nx*dx - xwidth, every point with |x - xpos| < xwidth becomes
+ * rho_low*S + rho_high*(1 - S), S = 1/(1 + exp(-alpha*(x - xpos))); other
+ * points are unchanged.  Replaces FixedBC::apply
+ * (include/openpfc/kernel/simulation/boundary_conditions/fixed_bc.hpp; the
+ * reference uses xwidth = 20, alpha = 1).
+ */
+int pfc_hip_bc_fixed(pfc_hip_plan_t plan, void *psi, double x0, double dx, double rho_low,
+                     double rho_high, double xwidth, double alpha, void *stream);
+
+/*
  * Operator tables of the fused step, written in the outbox layout (W = nx/2+1
  * pitch) to opL and opN (device, outbox real values each), scaled by
  * 1/(nx*ny*nz).  The step itself evaluates them per mode on the fly with the

@@ -666,6 +666,30 @@ __global__ void k_ic_add_noise(T *psi, long n, unsigned long long seed,
   }
 }
 
+// FixedBC (include/openpfc/kernel/simulation/boundary_conditions/fixed_bc.hpp):
+// at x = x0 + i*dx within |x - xpos| < xwidth the field becomes
+// rho_low*S + rho_high*(1 - S), S = 1/(1 + exp(-alpha*(x - xpos))); elsewhere
+// it is left untouched.  Only the band's points are written.
+struct FixedBcArgs {
+  double x0, dx, xpos, xwidth, alpha, rho_low, rho_high;
+  int nx;
+  long rows; // ny*nz
+};
+
+template <typename T> __global__ void k_bc_fixed(T *psi, FixedBcArgs a) {
+#pragma clang fp contract(off)
+  const long n = (long)a.nx * a.rows;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(idx % a.nx);
+    const double x = (a.x0 + ((double)i * a.dx));
+    if (fabs(x - a.xpos) < a.xwidth) {
+      const double S = 1.0 / (1.0 + exp((-a.alpha) * (x - a.xpos)));
+      psi[idx] = (T)((a.rho_low * S) + (a.rho_high * (1.0 - S)));
+    }
+  }
+}
+
 // SingleSeed (include/openpfc/kernel/simulation/initial_conditions/single_seed.hpp:51-83):
 // inside x^2+y^2+z^2 < radius^2 the value is rho + sum_q 2*amp*cos(q.x) over
 // the six <110>/sqrt(2) vectors; outside it writes 0.0 (keep_outside = 0, the

@@ -985,6 +985,39 @@ int pfc_hip_ic_add_noise(pfc_hip_plan_t plan, void *psi, unsigned long long seed
   return 0;
 }
 
+int pfc_hip_bc_fixed(pfc_hip_plan_t plan, void *psi, double x0, double dx, double rho_low,
+                     double rho_high, double xwidth, double alpha, void *stream) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!psi) return fail(PFC_HIP_EINVAL, "pfc_hip_bc_fixed: null psi");
+  if (!std::isfinite(x0) || !std::isfinite(dx) || !std::isfinite(xwidth) ||
+      !std::isfinite(alpha))
+    return fail(PFC_HIP_EINVAL, "pfc_hip_bc_fixed: non-finite geometry");
+  if (int rc = set_device(plan)) return rc;
+  FixedBcArgs a{};
+  a.x0 = x0;
+  a.dx = dx;
+  a.xwidth = xwidth;
+  a.alpha = alpha;
+  a.rho_low = rho_low;
+  a.rho_high = rho_high;
+  {
+#pragma clang fp contract(off)
+    a.xpos = ((double)plan->nx * dx) - xwidth; // fixed_bc.hpp: (Lx * dx) - xwidth
+  }
+  a.nx = plan->nx;
+  a.rows = (long)plan->ny * plan->nz;
+  const long n = (long)plan->inbox();
+  hipStream_t s = (hipStream_t)stream;
+  if (plan->prec == PFC_HIP_F64)
+    hipLaunchKernelGGL(k_bc_fixed<double>, dim3(ic_blocks(n)), dim3(256), 0, s,
+                       (double *)psi, a);
+  else
+    hipLaunchKernelGGL(k_bc_fixed<float>, dim3(ic_blocks(n)), dim3(256), 0, s,
+                       (float *)psi, a);
+  PFC_HIP_TRY(hipGetLastError(), "bc_fixed launch");
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Launch variants and autotuning.
 // ---------------------------------------------------------------------------

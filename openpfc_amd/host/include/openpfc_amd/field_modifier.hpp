@@ -21,6 +21,10 @@ class FieldModifier {
 public:
   virtual ~FieldModifier() = default;
   virtual void apply(Model &m, double t) = 0;
+  // Device path: applies the modifier to the model's device field when the
+  // model has one and the modifier has a device implementation; returns
+  // false (nothing done) otherwise.
+  virtual bool apply_device(Model &, double) { return false; }
   void set_field_name(const std::string &n) { m_field = n; }
   const std::string &get_field_name() const noexcept { return m_field; }
 
@@ -37,6 +41,13 @@ public:
                  [n0](const Real3 &) { return n0; });
   }
   double get_density() const noexcept { return m_n0; }
+  bool apply_device(Model &m, double) override {
+    DeviceField d;
+    if (!m.get_device_field(get_field_name(), d)) return false;
+    pfc::detail::check_abi(pfc_hip_ic_constant(d.plan, d.ptr, m_n0, d.stream), "Constant");
+    m.device_field_modified(get_field_name());
+    return true;
+  }
 
 private:
   double m_n0;
@@ -73,6 +84,18 @@ public:
                  });
   }
 
+  bool apply_device(Model &m, double) override {
+    DeviceField d;
+    if (!m.get_device_field(get_field_name(), d)) return false;
+    const World &w = m.get_world();
+    pfc::detail::check_abi(pfc_hip_ic_single_seed(d.plan, d.ptr, w.origin.data(),
+                                                  w.spacing.data(), m_rho, m_amp, m_radius, 0,
+                                                  d.stream),
+                           "SingleSeed");
+    m.device_field_modified(get_field_name());
+    return true;
+  }
+
 private:
   double m_amp = 0.0, m_rho = 0.0, m_radius = 64.0;
 };
@@ -90,6 +113,15 @@ public:
     }
   }
 
+  bool apply_device(Model &m, double) override {
+    DeviceField d;
+    if (!m.get_device_field(get_field_name(), d)) return false;
+    pfc::detail::check_abi(pfc_hip_ic_add_noise(d.plan, d.ptr, m_seed, m_lo, m_hi, d.stream),
+                           "UniformNoise");
+    m.device_field_modified(get_field_name());
+    return true;
+  }
+
 private:
   static std::uint64_t mix(std::uint64_t z) {
     z += 0x9E3779B97F4A7C15ULL;
@@ -99,6 +131,50 @@ private:
   }
   std::uint64_t m_seed;
   double m_lo, m_hi;
+};
+
+// FixedBC (include/openpfc/kernel/simulation/boundary_conditions/fixed_bc.hpp):
+// a sigmoid from rho_high to rho_low across a band of half-width xwidth = 20
+// around xpos = Lx*dx - xwidth, in world x coordinates.
+class FixedBC : public FieldModifier {
+public:
+  FixedBC() = default;
+  FixedBC(double rho_low, double rho_high) : m_rho_low(rho_low), m_rho_high(rho_high) {}
+  void set_rho_low(double v) { m_rho_low = v; }
+  void set_rho_high(double v) { m_rho_high = v; }
+  double get_rho_low() const { return m_rho_low; }
+  double get_rho_high() const { return m_rho_high; }
+
+  void apply(Model &m, double) override {
+    const World &w = m.get_world();
+    const double xpos = ((double)w.size[0] * w.spacing[0]) - m_xwidth;
+    RealField &f = m.get_real_field(get_field_name());
+    const Box3i in = m.get_fft().get_inbox_bounds();
+    size_t idx = 0;
+    for (int k = in.low[2]; k <= in.high[2]; ++k)
+      for (int j = in.low[1]; j <= in.high[1]; ++j)
+        for (int i = in.low[0]; i <= in.high[0]; ++i, ++idx) {
+          const double x = domain::to_coords(w, {i, j, k})[0];
+          if (std::abs(x - xpos) < m_xwidth) {
+            const double S = 1.0 / (1.0 + std::exp(-m_alpha * (x - xpos)));
+            f[idx] = (m_rho_low * S) + (m_rho_high * (1.0 - S));
+          }
+        }
+  }
+  bool apply_device(Model &m, double) override {
+    DeviceField d;
+    if (!m.get_device_field(get_field_name(), d)) return false;
+    const World &w = m.get_world();
+    pfc::detail::check_abi(pfc_hip_bc_fixed(d.plan, d.ptr, w.origin[0], w.spacing[0],
+                                            m_rho_low, m_rho_high, m_xwidth, m_alpha, d.stream),
+                           "FixedBC");
+    m.device_field_modified(get_field_name());
+    return true;
+  }
+
+private:
+  double m_xwidth = 20.0, m_alpha = 1.0;
+  double m_rho_low = 0.0, m_rho_high = 0.0;
 };
 
 } // namespace pfc

@@ -178,6 +178,25 @@ public:
     }
   }
 
+  // device path: the inbox brick goes straight from the file to the model's
+  // device field through the library's pinned staging
+  bool apply_device(Model &m, double) override {
+    DeviceField d;
+    if (!m.get_device_field(get_field_name(), d)) return false;
+    const World &w = m.get_world();
+    const auto inbox = fft::get_inbox(m.get_fft());
+    const int g[3] = {w.size[0], w.size[1], w.size[2]};
+    const int l[3] = {inbox.size[0], inbox.size[1], inbox.size[2]};
+    const int o[3] = {inbox.low[0], inbox.low[1], inbox.low[2]};
+    const int rc = pfc_hip_field_read(m_filename.c_str(), d.ptr, d.precision, PFC_HIP_IO_REAL,
+                                      g, l, o, d.stream);
+    if (rc != PFC_HIP_OK)
+      throw std::runtime_error("FileReader failed to read \"" + m_filename + "\" into field \"" +
+                               get_field_name() + "\": " + pfc_hip_last_error());
+    m.device_field_modified(get_field_name());
+    return true;
+  }
+
 private:
   std::string m_filename;
 };

@@ -18,6 +18,16 @@ namespace pfc {
 
 using RealField = std::vector<double>;
 
+// A field held in device memory by a device-resident model, with the plan
+// and stream that work on it (field modifiers and writers use it to stay on
+// the device).
+struct DeviceField {
+  void *ptr = nullptr;
+  int precision = PFC_HIP_F64;
+  pfc_hip_plan_t plan = nullptr;
+  hipStream_t stream = nullptr;
+};
+
 class Model {
 public:
   Model(fft::IFFT &fft, const World &world) : m_fft(fft), m_world(world) {}
@@ -32,6 +42,11 @@ public:
   virtual void prepare_for_field_modifiers() {}
   virtual void finalize_after_field_modifiers() {}
   virtual size_t get_allocated_memory_bytes() const { return 0; }
+  // Device-resident models: the device copy of a named field (false when the
+  // field lives on the host only).  After editing it on the device, callers
+  // report the edit so the host mirror is refreshed before host access.
+  virtual bool get_device_field(const std::string &, DeviceField &) { return false; }
+  virtual void device_field_modified(const std::string &) {}
 
   fft::IFFT &get_fft() noexcept { return m_fft; }
   const fft::IFFT &get_fft() const noexcept { return m_fft; }

@@ -60,6 +60,16 @@ public:
     return m_density.size() * sizeof(double) + m_psi.size() * sizeof(double);
   }
 
+  bool get_device_field(const std::string &name, DeviceField &d) override {
+    if (name != "density" && name != "default") return false;
+    d.ptr = m_psi.data();
+    d.precision = PFC_HIP_F64;
+    d.plan = m_hfft.plan();
+    d.stream = m_hfft.stream();
+    return true;
+  }
+  void device_field_modified(const std::string &) override { m_host_stale = true; }
+
   double get_epsilon() const { return m_epsilon; }
   core::DataBuffer<backend::HipTag, double> &device_density() { return m_psi; }
 

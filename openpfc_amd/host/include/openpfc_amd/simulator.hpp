@@ -1,9 +1,11 @@
 // simulator.hpp -- the time loop (include/openpfc/kernel/simulation/simulator.hpp:549-553,
 // simulator_integrator.hpp:43-61): on the first call initial conditions and
 // boundary conditions are applied, then every call does time.next(),
-// boundary conditions, model.step(time.get_current()).  Field modifiers run
+// boundary conditions, model.step(time.get_current()).  A field modifier
+// with a device path runs on the model's device field; the others run
 // between the model's prepare/finalize residency hooks, so device-resident
-// models see host edits.
+// models see host edits.  Results writers get the result counter
+// (simulator_integrator.hpp:31-36: file number, incremented per save).
 #pragma once
 
 #include "field_modifier.hpp"
@@ -24,8 +26,14 @@ public:
   void add_boundary_conditions(std::unique_ptr<FieldModifier> bc) { m_bcs.push_back(std::move(bc)); }
   // Called with the model whenever Time::do_save() holds (results writers).
   void add_results_writer(std::function<void(Model &, const Time &)> w) {
+    m_writers.push_back([w = std::move(w)](Model &m, const Time &t, int) { w(m, t); });
+  }
+  void add_results_writer(std::function<void(Model &, const Time &, int)> w) {
     m_writers.push_back(std::move(w));
   }
+  void set_result_counter(int n) { m_result_counter = n; }
+  int get_result_counter() const { return m_result_counter; }
+  void apply_initial_conditions() { apply(m_ics); }
 
   void step() {
     if (m_time.get_increment() == 0) {
@@ -45,21 +53,41 @@ public:
 
 private:
   void apply(std::vector<std::unique_ptr<FieldModifier>> &mods) {
-    if (mods.empty()) return;
-    m_model.prepare_for_field_modifiers();
-    for (auto &f : mods) f->apply(m_model, m_time.get_current());
-    m_model.finalize_after_field_modifiers();
+    bool host_ready = false, host_dirty = false;
+    for (auto &f : mods) {
+      if (host_dirty) {
+        // host edits go to the device before any device-side modifier
+        DeviceField d;
+        if (m_model.get_device_field(f->get_field_name(), d)) {
+          m_model.finalize_after_field_modifiers();
+          host_dirty = false;
+          host_ready = false;
+        }
+      }
+      if (!host_dirty && f->apply_device(m_model, m_time.get_current())) {
+        host_ready = false;
+        continue;
+      }
+      if (!host_ready) {
+        m_model.prepare_for_field_modifiers();
+        host_ready = true;
+      }
+      f->apply(m_model, m_time.get_current());
+      host_dirty = true;
+    }
+    if (host_dirty) m_model.finalize_after_field_modifiers();
   }
   void write() {
     if (m_writers.empty()) return;
-    m_model.prepare_for_field_modifiers();
-    for (auto &w : m_writers) w(m_model, m_time);
+    for (auto &w : m_writers) w(m_model, m_time, m_result_counter);
+    ++m_result_counter;
   }
 
   Model &m_model;
   Time &m_time;
   std::vector<std::unique_ptr<FieldModifier>> m_ics, m_bcs;
-  std::vector<std::function<void(Model &, const Time &)>> m_writers;
+  std::vector<std::function<void(Model &, const Time &, int)>> m_writers;
+  int m_result_counter = 0;
 };
 
 inline bool done(const Simulator &s) { return const_cast<Simulator &>(s).done(); }

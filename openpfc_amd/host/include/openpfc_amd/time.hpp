@@ -44,6 +44,14 @@ public:
     return (std::fmod(get_current() + 1.0e-9, m_saveat) < 1.e-6) || done() || (m_increment == 0);
   }
   operator double() const { return get_current(); }
+  // time.hpp:654-661: restart at a given step count
+  void set_increment(int increment) {
+    if (increment < 0)
+      throw std::invalid_argument("Time increment cannot be negative: " +
+                                  std::to_string(increment));
+    m_increment = increment;
+    m_accepted = std::min(m_t0 + static_cast<double>(increment) * m_dt, m_t1);
+  }
 
 private:
   double m_t0, m_t1, m_dt;

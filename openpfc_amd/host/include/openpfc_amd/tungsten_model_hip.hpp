@@ -136,6 +136,18 @@ public:
     m_host_stale = false;
   }
 
+  bool get_device_field(const std::string &name, DeviceField &d) override {
+    if (name != "psi" && name != "psiMF") return false;
+    d.ptr = name == "psi" ? m_psi.data() : m_psiMF.data();
+    d.precision = std::is_same_v<RealType, double> ? PFC_HIP_F64 : PFC_HIP_F32;
+    d.plan = m_hfft.plan();
+    d.stream = m_hfft.stream();
+    return true;
+  }
+  void device_field_modified(const std::string &name) override {
+    if (name == "psi") m_host_stale = true;
+  }
+
   core::DataBuffer<backend::HipTag, RealType> &get_psi() { return m_psi; }
   core::DataBuffer<backend::HipTag, RealType> &get_psiMF() { return m_psiMF; }
   RealField &get_psi_for_writer() {

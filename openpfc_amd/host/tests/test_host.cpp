@@ -10,6 +10,7 @@
 //                                      CPU-vs-HIP recipe (32^3, spacing 1,
 //                                      T = 0.5, dt = 0.01, -0.4 + 0.1 sin(2 pi i/n));
 //                                      writes psi and psi_MF (raw f64)
+#include <openpfc_amd/app.hpp>
 #include <openpfc_amd/openpfc_amd.hpp>
 
 #include <cmath>
@@ -338,6 +339,30 @@ static int run_gpu_tung(int steps, const char *out, const char *mfout) {
   return g_fail;
 }
 
+// Settings files: canonical JSON of a parsed .json/.toml file (checked
+// against Python's json and tomli by the tests), and World/Time wiring.
+static int run_parse(const char *path) {
+  std::printf("%s\n", ui::load_settings_file(path).dump().c_str());
+  return 0;
+}
+
+static int run_config(const char *path) {
+  const json::Value s = ui::load_settings_file(path);
+  World w = ui::world_from_json(s);
+  Time t = ui::time_from_json(s);
+  std::printf("world %d %d %d spacing %.17g %.17g %.17g origin %.17g %.17g %.17g\n", w.size[0],
+              w.size[1], w.size[2], w.spacing[0], w.spacing[1], w.spacing[2], w.origin[0],
+              w.origin[1], w.origin[2]);
+  std::printf("time %.17g %.17g %.17g %.17g\n", t.get_t0(), t.get_t1(), t.get_dt(),
+              t.get_saveat());
+  if (const json::Value *ics = s.find("initial_conditions"))
+    for (const json::Value &j : ics->items()) (void)ui::field_modifier_from_json(j);
+  if (const json::Value *bcs = s.find("boundary_conditions"))
+    for (const json::Value &j : bcs->items()) (void)ui::field_modifier_from_json(j);
+  ui::check_backend(s);
+  return 0;
+}
+
 int main(int argc, char **argv) {
   const std::string mode = argc > 1 ? argv[1] : "cpu";
   int rc = 0;
@@ -346,9 +371,11 @@ int main(int argc, char **argv) {
     else if (mode == "gpu-fft") rc = run_gpu_fft();
     else if (mode == "gpu-sim" && argc == 5) rc = run_gpu_sim(std::atoi(argv[2]), std::atoi(argv[3]), argv[4]);
     else if (mode == "gpu-tung" && argc == 5) rc = run_gpu_tung(std::atoi(argv[2]), argv[3], argv[4]);
+    else if (mode == "parse" && argc == 3) return run_parse(argv[2]);
+    else if (mode == "config" && argc == 3) rc = run_config(argv[2]);
     else {
       std::fprintf(stderr, "usage: test_host cpu | gpu-fft | gpu-sim N STEPS OUT | "
-                           "gpu-tung STEPS OUT MFOUT\n");
+                           "gpu-tung STEPS OUT MFOUT | parse FILE | config FILE\n");
       return 2;
     }
   } catch (const std::exception &e) {

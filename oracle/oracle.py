@@ -54,6 +54,7 @@ def lib():
                                              + [_dp, ctypes.c_double, _dp, _dp, _dp])
         L.orc_tungsten_step.argtypes = [ctypes.c_int] * 3 + [_dp] * 6 + [ctypes.c_int]
         L.orc_ic_constant.argtypes = [_dp, ctypes.c_size_t, ctypes.c_double]
+        L.orc_bc_fixed.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 6 + [_dp]
         L.orc_ic_single_seed.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 9 + [_dp]
         L.orc_add_counter_noise.argtypes = [_dp, ctypes.c_size_t, ctypes.c_uint64,
                                             ctypes.c_double, ctypes.c_double]
@@ -186,6 +187,14 @@ def ic_constant(n, value: float) -> np.ndarray:
     a = np.empty(shape_real(n))
     lib().orc_ic_constant(_p(a), a.size, value)
     return a
+
+
+def bc_fixed(a: np.ndarray, x0: float, dx: float, rho_low: float, rho_high: float,
+             xwidth: float = 20.0, alpha: float = 1.0) -> None:
+    """FixedBC applied in place to a (nz, ny, nx) float64 array."""
+    nz, ny, nx = a.shape
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    lib().orc_bc_fixed(nx, ny, nz, x0, dx, rho_low, rho_high, xwidth, alpha, _p(a))
 
 
 def ic_single_seed(n, origin, spacing, rho: float, amp: float, radius: float = 64.0):

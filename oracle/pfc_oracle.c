@@ -25,6 +25,8 @@
  *       include/openpfc/kernel/simulation/initial_conditions/single_seed.hpp:51-83
  *       include/openpfc/kernel/field/operations.hpp:81-105 (x-fastest loop)
  *       include/openpfc/kernel/data/domain.hpp:185-189 (x = origin + i*dx)
+ *   - FixedBC boundary condition:
+ *       include/openpfc/kernel/simulation/boundary_conditions/fixed_bc.hpp
  *   - Tungsten mean-field PFC operators and step:
  *       apps/tungsten/include/tungsten/common/tungsten_spectral.hpp:83-104
  *       apps/tungsten/include/tungsten/cpu/tungsten_model.hpp:223-366
@@ -615,6 +617,24 @@ int orc_tungsten_step(int nx, int ny, int nz, const double *filterMF,
   if (psiMF_out) memcpy(psiMF_out, psiMF, sizeof(double) * nr);
   free(psiF); free(tF); free(wk); free(psiMF); free(psiN);
   return 0;
+}
+
+/* FixedBC (include/openpfc/kernel/simulation/boundary_conditions/fixed_bc.hpp):
+ * xpos = Lx*dx - xwidth; where |x - xpos| < xwidth (x = x0 + i*dx) the field
+ * becomes rho_low*S + rho_high*(1 - S), S = 1/(1 + exp(-alpha*(x - xpos))). */
+void orc_bc_fixed(int nx, int ny, int nz, double x0, double dx, double rho_low,
+                  double rho_high, double xwidth, double alpha, double *psi) {
+  const double xpos = ((double)nx * dx) - xwidth;
+  size_t idx = 0;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i, ++idx) {
+        const double x = x0 + (double)i * dx;
+        if (fabs(x - xpos) < xwidth) {
+          const double S = 1.0 / (1.0 + exp(-alpha * (x - xpos)));
+          psi[idx] = (rho_low * S) + (rho_high * (1.0 - S));
+        }
+      }
 }
 
 /* ------------------------------------------------------------------------ */
