@@ -126,9 +126,9 @@ def test_device_fixed_bc_matches_oracle(torch_gpu):
     import ctypes
     import openpfc_amd as P
     torch = torch_gpu
-    n = (96, 8, 4)
+    n = (128, 8, 4)
     f = P.HipFft(*n)
-    x0, dx = -40.0, 0.75
+    x0, dx = 0.0, 0.75  # xpos = 76: the band covers x in (56, 96)
     a = np.random.default_rng(3).standard_normal(n[0] * n[1] * n[2])
     d = torch.from_numpy(a).cuda()
     P.check(f._lib.pfc_hip_bc_fixed(f.handle, ctypes.c_void_p(d.data_ptr()), x0, dx, -0.464,
@@ -136,4 +136,5 @@ def test_device_fixed_bc_matches_oracle(torch_gpu):
     ref = a.reshape(n[2], n[1], n[0]).copy()
     O.bc_fixed(ref, x0, dx, -0.464, -0.1)
     got = d.cpu().numpy().reshape(n[2], n[1], n[0])
+    assert (ref != a.reshape(ref.shape)).sum() == 53 * n[1] * n[2]  # 53 band columns
     assert np.abs(got - ref).max() <= 4e-16
