@@ -75,6 +75,8 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
     parts = variant.split("/")
     if kernel == "k_tx_fwd" and "r2c2" in parts:
         kernel, n = "k_tx_fwd2", n // 2
+    if kernel == "k_tx_fwd" and "wide" in parts:
+        kernel = "k_tx_fwd_w"
     radices = parts[0][1:]
     e = parts[1][1:]
     nn = n
@@ -90,7 +92,7 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
         rest = [str(direction)] + rest
     if kernel in ("k_tz", "k_pfc_z"):
         rest.append(str(mode))
-    elif kernel.startswith("k_t"):
+    elif kernel.startswith("k_t") and kernel != "k_tx_fwd_w":
         rest.append("false" if "direct" in parts else "true")
     t = "double" if prec == "f64" else "float"
     return f"{kernel}<{t},N{nn}/E{e},r{radices}" + ("," + ",".join(rest) if rest else "") + ">"

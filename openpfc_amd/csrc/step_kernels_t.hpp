@@ -579,4 +579,75 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd2(TStepArgs<T> a) {
   t1_store_rows<T, E, TL, G, TILE>(v, xm, a.B1, lds, t, g, z, y0, a.ny, a.W);
 }
 
+// ----------------------------------------------------------------------------
+// k_tx_fwd with a wide transposed store: per field, all W half-spectrum
+// values of the G rows go to one W x G LDS tile, then the whole block stores
+// it (G contiguous y values per kx).  Two block-wide syncs per field instead
+// of two per E/2 rounds.
+// ----------------------------------------------------------------------------
+template <typename T, typename Plan, int G, bool SPLIT> constexpr int tx_fwd_w_lds() {
+  constexpr int stage = 2 * (Plan::N / 2 + 1) * (G + 1);
+  constexpr int fft = (SPLIT ? 1 : 2) * t_plane<Plan, G>();
+  return stage > fft ? stage : fft;
+}
+
+template <typename T, typename Plan, int G, bool SPLIT>
+__global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w(TStepArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
+  constexpr int PL = t_plane<Plan, G>();
+  constexpr int W = N / 2 + 1;
+  __shared__ T lds[tx_fwd_w_lds<T, Plan, G, SPLIT>()];
+  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
+  const int ngy = (a.ny + G - 1) / G;
+  const int z = blockIdx.x / ngy, y0 = (blockIdx.x % ngy) * G;
+  const int y = y0 + g;
+  const bool active = y < a.ny;
+  const T *__restrict__ row = a.psi_in + ((long)z * a.ny + (active ? y : y0)) * N;
+
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const T u = row[t + TL * e];
+    v[e] = cplx<T>{u, a.nl(u)};
+  }
+  T *re = lds + g * RP;
+  T *im = lds + (SPLIT ? 0 : PL) + g * RP;
+  const LdsLine<T> ll{lds, g * RP, 1, PL};
+  LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_x);
+  cplx<T> pz[E];
+  fetch_partner<T, E, TL, SPLIT>(v, pz, t, re, im,
+                                 [](int k) { return k == 0 ? 0 : N - k; });
+#pragma unroll
+  for (int e = 0; e <= E / 2 && e < E; ++e) {
+    const cplx<T> zk = v[e], zc = c_conj(pz[e]);
+    const cplx<T> sum = c_add(zk, zc), d = c_sub(zk, zc);
+    v[e] = c_scale(sum, (T)0.5);
+    pz[e] = cplx<T>{(T)0.5 * d.y, (T)-0.5 * d.x};
+  }
+  T *sr = lds, *si = lds + W * (G + 1);
+#pragma unroll
+  for (int f = 0; f < 2; ++f) {
+    __syncthreads(); // the previous users of the LDS are done
+#pragma unroll
+    for (int e = 0; e < E / 2; ++e) {
+      const cplx<T> w = f ? pz[e] : v[e];
+      sr[tile_idx<G>(t + TL * e, g)] = w.x;
+      si[tile_idx<G>(t + TL * e, g)] = w.y;
+    }
+    if (t == 0) { // k = N/2, element E/2 of thread 0
+      const cplx<T> w = f ? pz[E / 2] : v[E / 2];
+      sr[tile_idx<G>(N / 2, g)] = w.x;
+      si[tile_idx<G>(N / 2, g)] = w.y;
+    }
+    __syncthreads();
+    cplx<T> *__restrict__ out = f ? a.B1 : a.A1;
+    for (int i = threadIdx.x; i < W * G; i += G * TL) {
+      const int kx = i / G, gg = i % G;
+      if (y0 + gg < a.ny)
+        out[((long)z * a.W + kx) * a.ny + y0 + gg] =
+            cplx<T>{sr[tile_idx<G>(kx, gg)], si[tile_idx<G>(kx, gg)]};
+    }
+  }
+}
+
 } // namespace pfc_amd

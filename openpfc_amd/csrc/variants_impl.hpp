@@ -333,6 +333,13 @@ hipError_t tx_fwd2_launch(TStepArgs<T> a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename T, typename Plan, int G, bool S>
+hipError_t tx_fwd_w_launch(TStepArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
+  hipLaunchKernelGGL((k_tx_fwd_w<T, Plan, G, S>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
 template <typename T, typename Plan, int G, bool S, bool TILE, int KIND>
 constexpr bool t_fits() {
   if (G * Plan::TL > kMaxThreads || G * Plan::TL < 64) return false;
@@ -398,6 +405,16 @@ void add_t_fwd2(std::vector<TV<T>> &v, int pid) {
   }
 }
 
+// PFC forward x with the wide transposed store (k_tx_fwd_w).
+template <typename T, typename Plan, int G, bool S>
+void add_t_fwd_w(std::vector<TV<T>> &v, int pid) {
+  if constexpr (G >= 1 && G * Plan::TL <= kMaxThreads && G * Plan::TL >= 64 &&
+                tx_fwd_w_lds<T, Plan, G, S>() * (int)sizeof(T) <= kMaxLds)
+    v.push_back({plan_name<Plan>() + "/G" + std::to_string(G) + (S ? "/split" : "/full") +
+                     "/wide",
+                 pid, &tx_fwd_w_launch<T, Plan, G, S>});
+}
+
 template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
   std::vector<TV<T>> v;
   if constexpr (N >= 128) {
@@ -412,6 +429,17 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
         add_t_fwd2<T, P, G0 / 2, true>(v, pid);
         add_t_fwd2<T, P, 2 * G0, true>(v, pid);
         add_t_fwd2<T, P, G0, false>(v, pid);
+      });
+    }
+    if constexpr (KIND == TK_X_FWD) {
+      for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+        using P = decltype(p);
+        constexpr int G0 = 128 / (2 * (int)sizeof(T));
+        add_t_fwd_w<T, P, G0 / 2, true>(v, pid);
+        add_t_fwd_w<T, P, G0 / 2, false>(v, pid);
+        add_t_fwd_w<T, P, G0, true>(v, pid);
+        add_t_fwd_w<T, P, G0, false>(v, pid);
+        add_t_fwd_w<T, P, G0 / 4, true>(v, pid);
       });
     }
     if constexpr (KIND == TK_Z) {
