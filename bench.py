@@ -77,6 +77,9 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
         kernel, n = "k_tx_fwd2", n // 2
     if kernel == "k_tx_fwd" and "wide" in parts:
         kernel = "k_tx_fwd_w"
+    wide_nr = [p for p in parts if p.startswith("wide") and p != "wide"]
+    if kernel in ("k_ty_fwd", "k_ty_inv") and wide_nr:
+        kernel = kernel + "_w"
     radices = parts[0][1:]
     e = parts[1][1:]
     nn = n
@@ -92,6 +95,8 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
         rest = [str(direction)] + rest
     if kernel in ("k_tz", "k_pfc_z"):
         rest.append(str(mode))
+    elif kernel in ("k_ty_fwd_w", "k_ty_inv_w"):
+        rest.append(wide_nr[0][4:])
     elif kernel.startswith("k_t") and kernel != "k_tx_fwd_w":
         rest.append("false" if "direct" in parts else "true")
     t = "double" if prec == "f64" else "float"
@@ -216,20 +221,32 @@ def accuracy_run(P, torch, n, dx, dt, r, steps, every=100):
     mean0 = float(psi.mean().item())
     d = (psi - mean0).float()
     st32.set_mean(mean0)
+    # control: FP64 from the initial field rounded to FP32 (one FP32-sized
+    # perturbation of the state, no FP32 arithmetic afterwards)
+    fftc = P.HipFft(n, n, n, precision=P.F64)
+    stc = P.PfcStepper(fftc, (dx, dx, dx), r)
+    stc.initialize(dt)
+    pc = psi.float().double()
     rows = []
     done = 0
     while done < steps:
         k = min(every, steps - done)
         st64.step(psi, k)
         st32.step(d, k)
+        stc.step(pc, k)
         done += k
+        nrm = torch.linalg.vector_norm(psi)
         diff = d.double() + mean0 - psi
-        l2 = (torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(psi)).item()
+        l2 = (torch.linalg.vector_norm(diff) / nrm).item()
         mx = (diff.abs().max() / psi.abs().max()).item()
         drift = abs((d.double().mean() + mean0).item() - mean0)
-        rows.append({"step": done, "rel_l2": l2, "rel_max": mx, "mean_drift": drift})
-        log(f"accuracy step {done}: rel L2 {l2:.3e} rel max {mx:.3e}")
+        cl2 = (torch.linalg.vector_norm(pc - psi) / nrm).item()
+        del diff
+        rows.append({"step": done, "rel_l2": l2, "rel_max": mx, "mean_drift": drift,
+                     "fp64_from_fp32_rounded_start_rel_l2": cl2})
+        log(f"accuracy step {done}: rel L2 {l2:.3e} rel max {mx:.3e} (fp64 control {cl2:.3e})")
     return {"metric": "fp32-mixed error vs fp64", "grid": [n, n, n], "steps": steps,
+            "control": "fp64 run started from the fp32-rounded initial field",
             "rows": rows}
 
 
@@ -348,6 +365,62 @@ def tungsten_main(args, P, torch):
     return 0
 
 
+def adjoint_run(P, torch, n=256, nsteps=200, every=20):
+    """Configuration 5: gradients of J = ||psi_T - psi_target||^2 with respect
+    to psi0 and r through the libtorch autograd::Function (adjoint steps,
+    checkpoints every `every` steps), psi_target from the seeded run at
+    r = -0.30; checked against FP64 central differences (one random
+    direction for psi0, h = 1e-6 * |psi0|_max; h = 1e-6 for r)."""
+    import numpy as np
+
+    from openpfc_amd.autograd import pfc_trajectory
+    dx, dt = math.pi / 4, 0.5
+    fft = P.HipFft(n, n, n)
+    psi = torch.empty(n ** 3, dtype=torch.float64, device="cuda")
+    P.ic_constant(fft, psi, -0.2)
+    P.ic_add_noise(fft, psi, 1234, -0.01, 0.01)
+    psi0 = psi.view(n, n, n).clone()
+    del fft
+    sp = (dx, dx, dx)
+    with torch.no_grad():
+        target = pfc_trajectory(psi0, torch.tensor(-0.30, dtype=torch.float64), nsteps, dt, sp,
+                                every)
+    x0 = psi0.clone().requires_grad_(True)
+    r = torch.tensor(-0.25, dtype=torch.float64, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    psiT = pfc_trajectory(x0, r, nsteps, dt, sp, every)
+    J = ((psiT - target) ** 2).sum()
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    J.backward()
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    g, gr = x0.grad, float(r.grad.item())
+
+    def loss(p0, rv):
+        with torch.no_grad():
+            pt = pfc_trajectory(p0, torch.tensor(rv, dtype=torch.float64), nsteps, dt, sp, every)
+            return float(((pt - target) ** 2).sum().item())
+
+    rng = np.random.default_rng(11)
+    v = torch.from_numpy(rng.standard_normal((n, n, n))).cuda()
+    h = 1e-6 * float(psi0.abs().max().item())
+    fd = (loss(psi0 + h * v, -0.25) - loss(psi0 - h * v, -0.25)) / (2 * h)
+    ad = float((g * v).sum().item())
+    hr = 1e-6
+    fdr = (loss(psi0, -0.25 + hr) - loss(psi0, -0.25 - hr)) / (2 * hr)
+    return {"metric": "adjoint gradients vs FP64 central differences", "grid": [n, n, n],
+            "steps": nsteps, "checkpoint_every": every, "precision": "f64",
+            "loss": float(J.item()),
+            "psi0_direction": {"adjoint": ad, "central_difference": fd,
+                               "rel_err": abs(ad - fd) / abs(fd), "h": h},
+            "r": {"adjoint": gr, "central_difference": fdr, "rel_err": abs(gr - fdr) / abs(fdr),
+                  "h": hr},
+            "seconds": {"forward_with_checkpoints": t_fwd, "backward": t_bwd}}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -369,6 +442,9 @@ def main(argv=None):
     ap.add_argument("--accuracy", type=int, default=0,
                     help="run FP64 and FP32-mixed in lockstep for this many steps and "
                          "report the FP32-mixed error every 100 steps (no timing)")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="configuration 5: adjoint gradients at 256^3, 200 steps, against "
+                         "FP64 central differences (no timing line)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also time a device-to-device copy of 2 x 4 GiB (the plain "
                          "copy kernel whose PMC byte counts scale the traffic figure)")
@@ -386,6 +462,9 @@ def main(argv=None):
     torch.cuda.set_device(0)
     if args.model == "tungsten":
         return tungsten_main(args, P, torch)
+    if args.adjoint:
+        print(json.dumps(adjoint_run(P, torch)), flush=True)
+        return 0
     n, dx, dt, r = args.n, math.pi / 4, 0.5, -0.25
     if args.accuracy:
         print(json.dumps(accuracy_run(P, torch, n, dx, dt, r, args.accuracy)), flush=True)

@@ -148,6 +148,20 @@ int pfc_hip_tungsten_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *psiM
                           void *stream);
 
 /*
+ * Adjoint of one plain PFC ETD1 step taken from psi_n (device, inbox reals):
+ * lambda (device, inbox reals, the adjoint field after the step) becomes
+ * A lambda + 3 psi_n^2 (B lambda), A = F^-1 opL F and B = F^-1 opN F (both
+ * symmetric: opL and opN are real and even in k).  With dr != NULL,
+ * <lambda, d step(psi_n) / d eps> is added to *dr first (the stream is
+ * synchronised), the derivative taken through opL and opN.  Chained from the
+ * last step back, this gives the gradients of a loss on psi_T with respect
+ * to psi_0 and eps.  Requires the PFC_HIP_MODEL_PFC operators (not the
+ * mean-split mode); lambda must not alias psi_n.  No reference counterpart.
+ */
+int pfc_hip_pfc_adjoint_step(pfc_hip_plan_t plan, const void *psi_n, void *lambda,
+                             double *dr, void *stream);
+
+/*
  * Mean-split step (the FP32-mixed mode): with enable != 0 the field passed
  * to pfc_hip_pfc_step holds the deviation d = psi - mean, and `mean` (kept in
  * double by the plan) is the conserved average of psi.  The step then forms

@@ -69,6 +69,8 @@ SIGNATURES = {
     "pfc_hip_field_read_host": (ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.c_int,
                                                ctypes.c_int, _ip3, _ip3, _ip3]),
     "pfc_hip_tungsten_step": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp]),
+    "pfc_hip_pfc_adjoint_step": (ctypes.c_int, [_vp, _vp, _vp, ctypes.POINTER(ctypes.c_double),
+                                                _vp]),
     "pfc_hip_bc_fixed": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, _vp]),

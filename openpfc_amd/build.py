@@ -67,5 +67,34 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+TORCH_EXT = os.path.join(HERE, "openpfc_amd_torch.so")
+
+
+def build_torch_extension(verbose: bool = False) -> str:
+    """The libtorch autograd wrapper (csrc/torch_pfc.cpp): host C++ against
+    torch's headers and the C ABI, linked to libopenpfc_hip.so next to it
+    ($ORIGIN rpath).  Compiled directly with g++ (no JIT cache)."""
+    import sysconfig
+
+    import torch
+    import torch.utils.cpp_extension as ce
+    cxx = shutil.which("g++") or "g++"
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    cmd = [cxx, "-O2", "-std=c++17", "-shared", "-fPIC", "-w",
+           f"-D_GLIBCXX_USE_CXX11_ABI={abi}", "-DTORCH_EXTENSION_NAME=openpfc_amd_torch",
+           "-DTORCH_API_INCLUDE_EXTENSION_H"]
+    for d in ce.include_paths() + [sysconfig.get_paths()["include"]]:
+        cmd += ["-isystem", d]
+    cmd += [os.path.join(CSRC, "torch_pfc.cpp")]
+    for d in ce.library_paths():
+        cmd += [f"-L{d}", f"-Wl,-rpath,{d}"]
+    cmd += ["-lc10", "-ltorch", "-ltorch_cpu", "-ltorch_python", f"-L{HERE}", "-lopenpfc_hip",
+            "-Wl,-rpath,$ORIGIN", "-o", TORCH_EXT + ".tmp"]
+    _run(cmd, quiet=not verbose)
+    os.replace(TORCH_EXT + ".tmp", TORCH_EXT)
+    return TORCH_EXT
+
+
 if __name__ == "__main__":
     print(build_library(verbose="-v" in sys.argv))
+    print(build_torch_extension(verbose="-v" in sys.argv))

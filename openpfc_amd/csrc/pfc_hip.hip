@@ -1,6 +1,7 @@
 // pfc_hip.hip -- C-ABI implementation: plans, twiddle tables, size dispatch
 // and the launch sequences of the R2C/C2R transforms and the fused PFC step.
 #include "../../include/pfc_hip.h"
+#include "adjoint_kernels.hpp"
 #include "variants.hpp"
 
 #include <hip/hip_runtime.h>
@@ -106,6 +107,9 @@ struct pfc_hip_plan_s {
   // layouts T1 (A1, B1) and T2 (A2, B2), W*ny*nz complex each
   void *A = nullptr, *B = nullptr;
   void *A1 = nullptr, *B1 = nullptr, *A2 = nullptr, *B2 = nullptr;
+  // adjoint workspace: three outbox spectra, two real fields, block partials
+  void *adj_lam = nullptr, *adj_P = nullptr, *adj_N = nullptr;
+  void *adj_a = nullptr, *adj_b = nullptr, *adj_part = nullptr;
   bool ops_ready = false;
   int model_kind = -1;
   double dt = 0;
@@ -215,7 +219,9 @@ void free_all(pfc_hip_plan_s *p) {
   for (auto &kv : p->tw)
     if (kv.second) (void)hipFree(kv.second);
   p->tw.clear();
-  void **bufs[] = {&p->wn, &p->work, &p->A, &p->B, &p->A1, &p->B1, &p->A2, &p->B2};
+  void **bufs[] = {&p->wn,      &p->work,  &p->A,     &p->B,     &p->A1,
+                   &p->B1,      &p->A2,    &p->B2,    &p->adj_lam, &p->adj_P,
+                   &p->adj_N,   &p->adj_a, &p->adj_b, &p->adj_part};
   for (void **b : bufs) {
     if (*b) (void)hipFree(*b);
     *b = nullptr;
@@ -594,8 +600,59 @@ int step_impl(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev,
                          : step_natural<T>(p, psi, s, ev);
 }
 
+constexpr unsigned kAdjBlocks = 2048; // blocks of the spectral adjoint kernel
+
+// Adjoint of one plain PFC step (see adjoint_kernels.hpp).
+template <typename T>
+int adjoint_impl(pfc_hip_plan_s *p, const void *psi, void *lambda, double *dr, hipStream_t s);
+
 unsigned ic_blocks(long n) {
   return (unsigned)std::min<long>((n + 255) / 256, 1L << 20);
+}
+
+template <typename T>
+int adjoint_impl(pfc_hip_plan_s *p, const void *psi, void *lambda, double *dr, hipStream_t s) {
+  const size_t cb = p->outbox() * 2 * p->esize, rb = p->inbox() * p->esize;
+  int rc = 0;
+  if (!rc) rc = dalloc(p, &p->adj_lam, cb, "adjoint spectrum");
+  if (!rc) rc = dalloc(p, &p->adj_P, cb, "adjoint spectrum");
+  if (!rc) rc = dalloc(p, &p->adj_N, cb, "adjoint spectrum");
+  if (!rc) rc = dalloc(p, &p->adj_a, rb, "adjoint field");
+  if (!rc) rc = dalloc(p, &p->adj_b, rb, "adjoint field");
+  if (!rc) rc = dalloc(p, &p->adj_part, (kAdjBlocks + 1) * sizeof(double), "adjoint partials");
+  if (rc) return rc;
+  const long n = (long)p->inbox();
+  PfcOp o = p->op;
+  o.scale = 1.0; // the C2R below applies 1/(nx ny nz)
+  if (dr) {
+    if ((rc = r2c_impl<T>(p, psi, p->adj_P, s))) return rc;
+    hipLaunchKernelGGL(k_cube<T>, dim3(ic_blocks(n)), dim3(256), 0, s, (const T *)psi,
+                       (T *)p->adj_a, n);
+    PFC_HIP_TRY(hipGetLastError(), "adjoint cube");
+    if ((rc = r2c_impl<T>(p, p->adj_a, p->adj_N, s))) return rc;
+  }
+  if ((rc = r2c_impl<T>(p, lambda, p->adj_lam, s))) return rc;
+  hipLaunchKernelGGL(k_adj_spectral<T>, dim3(kAdjBlocks), dim3(256), 0, s, o, p->W,
+                     (const cplx<T> *)p->adj_lam, (cplx<T> *)p->adj_P, (cplx<T> *)p->adj_N,
+                     (double *)p->adj_part, dr ? 1 : 0);
+  PFC_HIP_TRY(hipGetLastError(), "adjoint spectral kernel");
+  if (dr) {
+    double *part = (double *)p->adj_part;
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, (const double *)part,
+                       (int)kAdjBlocks, part + kAdjBlocks);
+    PFC_HIP_TRY(hipGetLastError(), "adjoint reduction");
+    double h = 0;
+    PFC_HIP_TRY(hipMemcpyAsync(&h, part + kAdjBlocks, sizeof(double), hipMemcpyDeviceToHost, s),
+                "adjoint reduction copy");
+    PFC_HIP_TRY(hipStreamSynchronize(s), "adjoint sync");
+    *dr += h / ((double)p->nx * (double)p->ny * (double)p->nz);
+  }
+  if ((rc = c2r_impl<T>(p, p->adj_P, p->adj_a, s))) return rc;
+  if ((rc = c2r_impl<T>(p, p->adj_N, p->adj_b, s))) return rc;
+  hipLaunchKernelGGL(k_adj_combine<T>, dim3(ic_blocks(n)), dim3(256), 0, s, (const T *)psi,
+                     (const T *)p->adj_a, (const T *)p->adj_b, (T *)lambda, n);
+  PFC_HIP_TRY(hipGetLastError(), "adjoint combine");
+  return 0;
 }
 
 int check_plan(pfc_hip_plan_t p) {
@@ -875,6 +932,23 @@ int pfc_hip_tungsten_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *psiM
     if (rc) return rc;
   }
   return 0;
+}
+
+int pfc_hip_pfc_adjoint_step(pfc_hip_plan_t plan, const void *psi_n, void *lambda,
+                             double *dr, void *stream) {
+  if (int rc = check_plan(plan)) return rc;
+  if (!psi_n || !lambda) return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_adjoint_step: null field");
+  if (psi_n == lambda)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_adjoint_step: lambda must not alias psi_n");
+  if (!plan->ops_ready || plan->model_kind != PFC_HIP_MODEL_PFC)
+    return fail(PFC_HIP_ESTATE, "pfc_hip_pfc_adjoint_step: PFC operators not set");
+  if (plan->mean_split)
+    return fail(PFC_HIP_EUNSUPPORTED,
+                "pfc_hip_pfc_adjoint_step: the mean-split mode has no adjoint here");
+  if (int rc = set_device(plan)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  return plan->prec == PFC_HIP_F64 ? adjoint_impl<double>(plan, psi_n, lambda, dr, s)
+                                   : adjoint_impl<float>(plan, psi_n, lambda, dr, s);
 }
 
 int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,

@@ -650,4 +650,98 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w(TStepArgs<T> a) {
   }
 }
 
+// ----------------------------------------------------------------------------
+// y passes with few-round transpositions: the N positions of the G lines go
+// through the LDS in NR rounds of N/NR rows ([position][line] tile), one
+// block-wide store (forward) or load (inverse) per round instead of one per
+// value of E.
+// ----------------------------------------------------------------------------
+template <typename T, typename Plan, int G, bool SPLIT, int NR> constexpr int ty_w_lds() {
+  constexpr int stage = 2 * (Plan::N / NR) * (G + 1);
+  constexpr int fft = (SPLIT ? 1 : 2) * t_plane<Plan, G>();
+  return stage > fft ? stage : fft;
+}
+
+template <typename T, typename Plan, int G, bool SPLIT, int NR>
+__global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd_w(TStepArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
+  constexpr int PL = t_plane<Plan, G>();
+  constexpr int ROWS = N / NR, EPR = E / NR; // tile rows, values per thread per round
+  static_assert(E % NR == 0, "NR must divide E");
+  __shared__ T lds[ty_w_lds<T, Plan, G, SPLIT, NR>()];
+  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
+  const int ngz = (a.nz + G - 1) / G;
+  const int kx = blockIdx.x / ngz, z0 = (blockIdx.x % ngz) * G;
+  const int z = z0 + g;
+  const bool active = z < a.nz;
+  const cplx<T> *__restrict__ in = blockIdx.y ? a.B1 : a.A1;
+  cplx<T> *__restrict__ out = blockIdx.y ? a.B2 : a.A2;
+  const cplx<T> *line = in + ((long)(active ? z : z0) * a.W + kx) * N;
+
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) v[e] = line[t + TL * e];
+  const LdsLine<T> ll{lds, g * RP, 1, PL};
+  LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_y);
+  // ky = TL*e + t; round h holds e in [h*EPR, (h+1)*EPR), i.e. ky in [h*ROWS, (h+1)*ROWS)
+  T *tr = lds, *ti = lds + ROWS * (G + 1);
+#pragma unroll
+  for (int h = 0; h < NR; ++h) {
+    __syncthreads(); // previous users of the LDS are done
+#pragma unroll
+    for (int q = 0; q < EPR; ++q) {
+      const int row = TL * q + t; // ky - h*ROWS
+      tr[tile_idx<G>(row, g)] = v[h * EPR + q].x;
+      ti[tile_idx<G>(row, g)] = v[h * EPR + q].y;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < ROWS * G; i += G * TL) {
+      const int row = i / G, gg = i % G;
+      if (z0 + gg < a.nz)
+        out[((long)kx * N + h * ROWS + row) * a.nz + z0 + gg] =
+            cplx<T>{tr[tile_idx<G>(row, gg)], ti[tile_idx<G>(row, gg)]};
+    }
+  }
+}
+
+template <typename T, typename Plan, int G, bool SPLIT, int NR>
+__global__ void __launch_bounds__(G *Plan::TL) k_ty_inv_w(TStepArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
+  constexpr int PL = t_plane<Plan, G>();
+  constexpr int ROWS = N / NR, EPR = E / NR;
+  static_assert(E % NR == 0, "NR must divide E");
+  __shared__ T lds[ty_w_lds<T, Plan, G, SPLIT, NR>()];
+  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
+  const int ngz = (a.nz + G - 1) / G;
+  const int kx = blockIdx.x / ngz, z0 = (blockIdx.x % ngz) * G;
+  const int z = z0 + g;
+
+  cplx<T> v[E];
+  T *tr = lds, *ti = lds + ROWS * (G + 1);
+#pragma unroll
+  for (int h = 0; h < NR; ++h) {
+    for (int i = threadIdx.x; i < ROWS * G; i += G * TL) {
+      const int row = i / G, gg = i % G;
+      cplx<T> w{0, 0};
+      if (z0 + gg < a.nz) w = a.A2[((long)kx * N + h * ROWS + row) * a.nz + z0 + gg];
+      tr[tile_idx<G>(row, gg)] = w.x;
+      ti[tile_idx<G>(row, gg)] = w.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < EPR; ++q) {
+      const int row = TL * q + t;
+      v[h * EPR + q] = cplx<T>{tr[tile_idx<G>(row, g)], ti[tile_idx<G>(row, g)]};
+    }
+    __syncthreads(); // the tile is refilled (or becomes the FFT exchange area)
+  }
+  const LdsLine<T> ll{lds, g * RP, 1, PL};
+  LineFft<T, Plan, +1, SPLIT, true>::run(v, t, ll, a.tw_y);
+  if (z < a.nz) {
+    cplx<T> *__restrict__ line = a.A1 + ((long)z * a.W + kx) * N;
+#pragma unroll
+    for (int e = 0; e < E; ++e) line[t + TL * e] = v[e];
+  }
+}
+
 } // namespace pfc_amd

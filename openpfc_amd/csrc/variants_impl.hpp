@@ -340,6 +340,20 @@ hipError_t tx_fwd_w_launch(TStepArgs<T> a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename T, typename Plan, int G, bool S, int NR>
+hipError_t ty_fwd_w_launch(TStepArgs<T> a, hipStream_t s) {
+  if (a.nfields != 1 && a.nfields != 2) return hipErrorInvalidValue;
+  dim3 grid((unsigned)(a.W * ((a.nz + G - 1) / G)), (unsigned)a.nfields), block(G * Plan::TL);
+  hipLaunchKernelGGL((k_ty_fwd_w<T, Plan, G, S, NR>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+template <typename T, typename Plan, int G, bool S, int NR>
+hipError_t ty_inv_w_launch(TStepArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)(a.W * ((a.nz + G - 1) / G))), block(G * Plan::TL);
+  hipLaunchKernelGGL((k_ty_inv_w<T, Plan, G, S, NR>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
 template <typename T, typename Plan, int G, bool S, bool TILE, int KIND>
 constexpr bool t_fits() {
   if (G * Plan::TL > kMaxThreads || G * Plan::TL < 64) return false;
@@ -415,6 +429,18 @@ void add_t_fwd_w(std::vector<TV<T>> &v, int pid) {
                  pid, &tx_fwd_w_launch<T, Plan, G, S>});
 }
 
+// y passes with NR-round transpositions (k_ty_fwd_w / k_ty_inv_w).
+template <typename T, typename Plan, int G, bool S, int NR, int KIND>
+void add_t_y_w(std::vector<TV<T>> &v, int pid) {
+  if constexpr (Plan::E % NR == 0 && G * Plan::TL <= kMaxThreads && G * Plan::TL >= 64 &&
+                ty_w_lds<T, Plan, G, S, NR>() * (int)sizeof(T) <= kMaxLds) {
+    const std::string nm = plan_name<Plan>() + "/G" + std::to_string(G) +
+                           (S ? "/split" : "/full") + "/wide" + std::to_string(NR);
+    if constexpr (KIND == TK_Y_FWD) v.push_back({nm, pid, &ty_fwd_w_launch<T, Plan, G, S, NR>});
+    if constexpr (KIND == TK_Y_INV) v.push_back({nm, pid, &ty_inv_w_launch<T, Plan, G, S, NR>});
+  }
+}
+
 template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
   std::vector<TV<T>> v;
   if constexpr (N >= 128) {
@@ -440,6 +466,18 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
         add_t_fwd_w<T, P, G0, true>(v, pid);
         add_t_fwd_w<T, P, G0, false>(v, pid);
         add_t_fwd_w<T, P, G0 / 4, true>(v, pid);
+      });
+    }
+    if constexpr (KIND == TK_Y_FWD || KIND == TK_Y_INV) {
+      for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+        using P = decltype(p);
+        constexpr int G0 = 128 / (2 * (int)sizeof(T));
+        add_t_y_w<T, P, G0, true, 2, KIND>(v, pid);
+        add_t_y_w<T, P, G0, true, 4, KIND>(v, pid);
+        add_t_y_w<T, P, G0, false, 2, KIND>(v, pid);
+        add_t_y_w<T, P, G0 / 2, true, 1, KIND>(v, pid);
+        add_t_y_w<T, P, G0 / 2, true, 2, KIND>(v, pid);
+        add_t_y_w<T, P, 2 * G0, true, 4, KIND>(v, pid);
       });
     }
     if constexpr (KIND == TK_Z) {

@@ -55,6 +55,9 @@ def lib():
         L.orc_tungsten_step.argtypes = [ctypes.c_int] * 3 + [_dp] * 6 + [ctypes.c_int]
         L.orc_ic_constant.argtypes = [_dp, ctypes.c_size_t, ctypes.c_double]
         L.orc_bc_fixed.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 6 + [_dp]
+        L.orc_pfc_operator_derivs.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 5
+                                              + [_dp, _dp])
+        L.orc_pfc_adjoint_step.argtypes = [ctypes.c_int] * 3 + [_dp] * 6 + [_dp]
         L.orc_ic_single_seed.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 9 + [_dp]
         L.orc_add_counter_noise.argtypes = [_dp, ctypes.c_size_t, ctypes.c_uint64,
                                             ctypes.c_double, ctypes.c_double]
@@ -187,6 +190,29 @@ def ic_constant(n, value: float) -> np.ndarray:
     a = np.empty(shape_real(n))
     lib().orc_ic_constant(_p(a), a.size, value)
     return a
+
+
+def pfc_operator_derivs(n, spacing, eps: float, dt: float):
+    """d opL/d eps, d opN/d eps over the outbox (unscaled)."""
+    nx, ny, nz = n
+    a = np.empty(shape_complex(n))
+    b = np.empty(shape_complex(n))
+    lib().orc_pfc_operator_derivs(nx, ny, nz, *spacing, eps, dt, _p(a), _p(b))
+    return a, b
+
+
+def pfc_adjoint_step(psi, lam, opL, opN, dopL=None, dopN=None):
+    """Adjoint of one step from psi; returns (new lambda, dr contribution or None)."""
+    nz, ny, nx = psi.shape
+    p = np.ascontiguousarray(psi, dtype=np.float64)
+    l = np.ascontiguousarray(lam, dtype=np.float64).copy()
+    dr = ctypes.c_double(0.0)
+    want = dopL is not None
+    dl = dopL if want else opL
+    dn = dopN if want else opN
+    lib().orc_pfc_adjoint_step(nx, ny, nz, _p(opL), _p(opN), _p(dl), _p(dn), _p(p), _p(l),
+                               ctypes.cast(ctypes.pointer(dr), _dp) if want else None)
+    return l, (dr.value if want else None)
 
 
 def bc_fixed(a: np.ndarray, x0: float, dx: float, rho_low: float, rho_high: float,

@@ -25,6 +25,7 @@
  *       include/openpfc/kernel/simulation/initial_conditions/single_seed.hpp:51-83
  *       include/openpfc/kernel/field/operations.hpp:81-105 (x-fastest loop)
  *       include/openpfc/kernel/data/domain.hpp:185-189 (x = origin + i*dx)
+ *   - adjoint of the plain PFC step (its transpose; no reference code)
  *   - FixedBC boundary condition:
  *       include/openpfc/kernel/simulation/boundary_conditions/fixed_bc.hpp
  *   - Tungsten mean-field PFC operators and step:
@@ -524,6 +525,70 @@ int orc_pfc_step(int nx, int ny, int nz, const double *opL, const double *opN,
   free(psiF);
   free(NF);
   free(psiN);
+  return 0;
+}
+
+/* Adjoint of one plain PFC step (no reference counterpart; the mathematics
+ * of the step 10_complete_pfc_model.cpp:121-144 fixes it).  The step is
+ * psi' = A psi + B psi^3 with A = F^-1 opL F, B = F^-1 opN F (opL, opN real
+ * and even in k, so A and B are symmetric).  lambda <- A lambda +
+ * 3 psi^2 (B lambda); with dr != NULL, *dr += sum_x lambda_in(x) g(x),
+ * g = F^-1(dopL F(psi) + dopN F(psi^3)) the derivative of the step with
+ * respect to eps, summed in real space.  Tables are unscaled (the C2R
+ * applies 1/N). */
+void orc_pfc_operator_derivs(int nx, int ny, int nz, double dx, double dy, double dz,
+                             double eps, double dt, double *dopL, double *dopN) {
+  const double two_pi = 6.283185307179586;
+  const double fx = two_pi / (dx * nx), fy = two_pi / (dy * ny), fz = two_pi / (dz * nz);
+  const int w = nx / 2 + 1;
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < w; ++i) {
+        const size_t idx = ((size_t)k * ny + j) * w + i;
+        double kx = orc_k_component(i, nx, fx), ky = orc_k_component(j, ny, fy),
+               kz = orc_k_component(k, nz, fz);
+        double k2 = kx * kx + ky * ky + kz * kz;
+        double L = -k2 * (eps + (1.0 - k2) * (1.0 - k2));
+        double oL = exp(L * dt);
+        dopL[idx] = dt * (-k2) * oL;
+        dopN[idx] = fabs(L) > 1e-14 ? (k2 * k2) * (dt * oL * L - (oL - 1.0)) / (L * L) : 0.0;
+      }
+}
+
+int orc_pfc_adjoint_step(int nx, int ny, int nz, const double *opL, const double *opN,
+                         const double *dopL, const double *dopN, const double *psi,
+                         double *lambda, double *dr) {
+  const size_t nr = (size_t)nx * ny * nz;
+  const size_t nc = (size_t)(nx / 2 + 1) * ny * nz;
+  cdbl *L = (cdbl *)malloc(sizeof(cdbl) * nc);
+  cdbl *X = (cdbl *)malloc(sizeof(cdbl) * nc);
+  cdbl *Y = (cdbl *)malloc(sizeof(cdbl) * nc);
+  double *a = (double *)malloc(sizeof(double) * nr);
+  double *b = (double *)malloc(sizeof(double) * nr);
+  if (!L || !X || !Y || !a || !b) {
+    free(L); free(X); free(Y); free(a); free(b);
+    return -2;
+  }
+  if (dr) {
+    orc_r2c3d(nx, ny, nz, psi, (double *)X);
+    for (size_t i = 0; i < nr; ++i) a[i] = psi[i] * psi[i] * psi[i];
+    orc_r2c3d(nx, ny, nz, a, (double *)Y);
+    for (size_t i = 0; i < nc; ++i) X[i] = dopL[i] * X[i] + dopN[i] * Y[i];
+    orc_c2r3d(nx, ny, nz, (const double *)X, b, (double *)Y);
+    double s = 0;
+    for (size_t i = 0; i < nr; ++i) s += lambda[i] * b[i];
+    *dr += s;
+  }
+  orc_r2c3d(nx, ny, nz, lambda, (double *)L);
+  for (size_t i = 0; i < nc; ++i) {
+    X[i] = opL[i] * L[i];
+    Y[i] = opN[i] * L[i];
+  }
+  orc_c2r3d(nx, ny, nz, (const double *)X, a, (double *)L);
+  orc_c2r3d(nx, ny, nz, (const double *)Y, b, (double *)L);
+  for (size_t i = 0; i < nr; ++i) lambda[i] = a[i] + 3.0 * psi[i] * psi[i] * b[i];
+  free(L); free(X); free(Y); free(a); free(b);
   return 0;
 }
 
