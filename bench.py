@@ -95,10 +95,14 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
         rest = [str(direction)] + rest
     if kernel in ("k_tz", "k_pfc_z"):
         rest.append(str(mode))
+        if kernel == "k_tz":
+            rest.append("true" if "pre" in parts else "false")
     elif kernel in ("k_ty_fwd_w", "k_ty_inv_w"):
         rest.append(wide_nr[0][4:])
     elif kernel.startswith("k_t") and kernel != "k_tx_fwd_w":
         rest.append("false" if "direct" in parts else "true")
+        if kernel in ("k_tx_r2c", "k_tx_tung"):
+            rest.append("true" if "wide" in parts else "false")
     t = "double" if prec == "f64" else "float"
     return f"{kernel}<{t},N{nn}/E{e},r{radices}" + ("," + ",".join(rest) if rest else "") + ">"
 

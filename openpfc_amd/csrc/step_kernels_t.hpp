@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd(TStepArgs<T> a) {
 // ----------------------------------------------------------------------------
 // T2 z pass, in place: G consecutive z lines (kx*ny + ky) per block.
 // ----------------------------------------------------------------------------
-template <typename T, typename Plan, int G, bool SPLIT, int MODE>
+template <typename T, typename Plan, int G, bool SPLIT, int MODE, bool PRE = false>
 __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
   constexpr int PL = t_plane<Plan, G>();
@@ -246,24 +246,38 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
       acc[e] = c_scale(acc[e], (T)f);
     }
   } else {
-    // multipliers first, while nothing else is live (exp and division need
-    // many temporaries); then the lines are loaded
+    // PRE = false: multipliers first, while nothing else is live (exp and
+    // division need many temporaries), then the lines are loaded.  PRE =
+    // true: both lines are requested first and the multipliers computed
+    // while the loads are in flight.
     T opl[E], opn[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      double ma, mb;
-      z_mode_ops<MODE>(a.op, a.top, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), ma, mb);
-      opl[e] = (T)ma;
-      opn[e] = (T)mb;
-    }
     cplx<T> v[E];
-    if constexpr (MODE == ZM_PFC) {
-      // A = Fz^-1(opL*Fz(A) + opN*Fz(B))
+    auto multipliers = [&] {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        double ma, mb;
+        z_mode_ops<MODE>(a.op, a.top, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), ma, mb);
+        opl[e] = (T)ma;
+        opn[e] = (T)mb;
+      }
+    };
+    if constexpr (PRE) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         acc[e] = la[t + TL * e];
         v[e] = lb[t + TL * e];
       }
+      multipliers();
+    } else {
+      multipliers();
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        if constexpr (MODE == ZM_PFC) acc[e] = la[t + TL * e];
+        v[e] = lb[t + TL * e];
+      }
+    }
+    if constexpr (MODE == ZM_PFC) {
+      // A = Fz^-1(opL*Fz(A) + opN*Fz(B))
       LineFft<T, Plan, -1, SPLIT, true>::run(acc, t, ll, a.tw_z);
 #pragma unroll
       for (int e = 0; e < E; ++e) acc[e] = c_scale(acc[e], opl[e]);
@@ -273,12 +287,12 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
       if (a.zero_k0 && l == 0 && t == 0) acc[0] = cplx<T>{0, 0};
     } else {
       // A = Fz^-1(expL*A + nW*Fz(B)), A already a full spectrum
-#pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = lb[t + TL * e];
       LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_z);
 #pragma unroll
-      for (int e = 0; e < E; ++e)
-        acc[e] = c_add(c_scale(la[t + TL * e], opl[e]), c_scale(v[e], opn[e]));
+      for (int e = 0; e < E; ++e) {
+        const cplx<T> h = PRE ? acc[e] : la[t + TL * e];
+        acc[e] = c_add(c_scale(h, opl[e]), c_scale(v[e], opn[e]));
+      }
     }
   }
   LineFft<T, Plan, +1, SPLIT, true>::run(acc, t, ll, a.tw_z);
@@ -440,6 +454,32 @@ __device__ __forceinline__ void t1_store_rows(const cplx<T> (&v)[E], cplx<T> xm,
   }
 }
 
+// The same store through one (M+1) x G LDS tile: two block-wide syncs.
+template <typename T, int E, int TL, int G>
+__device__ __forceinline__ void t1_store_rows_wide(const cplx<T> (&v)[E], cplx<T> xm,
+                                                   cplx<T> *__restrict__ out, T *lds, int t,
+                                                   int g, int z, int y0, int ny, int W) {
+  constexpr int M = E * TL, WR = M + 1;
+  T *sr = lds, *si = lds + WR * (G + 1);
+  __syncthreads(); // previous users of the LDS are done
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    sr[tile_idx<G>(t + TL * e, g)] = v[e].x;
+    si[tile_idx<G>(t + TL * e, g)] = v[e].y;
+  }
+  if (t == 0) {
+    sr[tile_idx<G>(M, g)] = xm.x;
+    si[tile_idx<G>(M, g)] = xm.y;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < WR * G; i += G * TL) {
+    const int kx = i / G, gg = i % G;
+    if (y0 + gg < ny)
+      out[((long)z * W + kx) * ny + y0 + gg] =
+          cplx<T>{sr[tile_idx<G>(kx, gg)], si[tile_idx<G>(kx, gg)]};
+  }
+}
+
 // Forward half-length transform of the G rows held as v (z[m] = x[2m] +
 // i*x[2m+1]) and the R2C split; on return v[e] = X[t + TL*e] and xm = X[M]
 // (valid on t == 0).
@@ -463,10 +503,10 @@ __device__ __forceinline__ void row_r2c(cplx<T> (&v)[Plan::E], cplx<T> &xm, T *l
 // Tungsten, psi -> T1 (A1): R2C of G consecutive rows of plane z with the
 // nx/2-point plan, transposed store.
 // ----------------------------------------------------------------------------
-template <typename T, typename Plan, int G, bool TILE>
+template <typename T, typename Plan, int G, bool TILE, bool WIDE = false>
 __global__ void __launch_bounds__(G *Plan::TL) k_tx_r2c(TStepArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N;
-  __shared__ T lds[2 * t_plane<Plan, G>()];
+  __shared__ T lds[WIDE ? tx_inv_lds<T, Plan, G, true>() : 2 * t_plane<Plan, G>()];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngy = (a.ny + G - 1) / G;
   const int z = blockIdx.x / ngy, y0 = (blockIdx.x % ngy) * G;
@@ -478,7 +518,10 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_r2c(TStepArgs<T> a) {
   for (int e = 0; e < E; ++e) v[e] = row[t + TL * e];
   cplx<T> xm;
   row_r2c<T, Plan, G>(v, xm, lds, t, g, a.tw_xm, a.wn);
-  t1_store_rows<T, E, TL, G, TILE>(v, xm, a.A1, lds, t, g, z, y0, a.ny, a.W);
+  if constexpr (WIDE)
+    t1_store_rows_wide<T, E, TL, G>(v, xm, a.A1, lds, t, g, z, y0, a.ny, a.W);
+  else
+    t1_store_rows<T, E, TL, G, TILE>(v, xm, a.A1, lds, t, g, z, y0, a.ny, a.W);
 }
 
 // ----------------------------------------------------------------------------
@@ -488,7 +531,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_r2c(TStepArgs<T> a) {
 // inverse), combined with the rows of psi into N(psi, psi_MF), transformed
 // forward and stored transposed.  psi_MF goes to HBM only when mf_out is set.
 // ----------------------------------------------------------------------------
-template <typename T, typename Plan, int G, bool TILE>
+template <typename T, typename Plan, int G, bool TILE, bool WIDE = false>
 __global__ void __launch_bounds__(G *Plan::TL) k_tx_tung(TStepArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N, RP = row_padded(M);
   constexpr int PL = t_plane<Plan, G>();
@@ -545,7 +588,10 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_tung(TStepArgs<T> a) {
   }
   cplx<T> xm;
   row_r2c<T, Plan, G>(v, xm, lds, t, g, a.tw_xm, a.wn);
-  t1_store_rows<T, E, TL, G, TILE>(v, xm, spec, lds, t, g, z, y0, a.ny, a.W);
+  if constexpr (WIDE)
+    t1_store_rows_wide<T, E, TL, G>(v, xm, spec, lds, t, g, z, y0, a.ny, a.W);
+  else
+    t1_store_rows<T, E, TL, G, TILE>(v, xm, spec, lds, t, g, z, y0, a.ny, a.W);
 }
 
 // ----------------------------------------------------------------------------

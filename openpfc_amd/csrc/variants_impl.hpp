@@ -340,6 +340,38 @@ hipError_t tx_fwd_w_launch(TStepArgs<T> a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename T, typename Plan, int G, bool S>
+hipError_t tz_pre_launch(TStepArgs<T> a, hipStream_t s) {
+  const long lines = (long)a.W * a.ny;
+  dim3 grid((unsigned)((lines + G - 1) / G)), block(G * Plan::TL);
+  switch (a.mode) {
+  case ZM_PFC:
+    hipLaunchKernelGGL((k_tz<T, Plan, G, S, ZM_PFC, true>), grid, block, 0, s, a);
+    break;
+  case ZM_TMF:
+    hipLaunchKernelGGL((k_tz<T, Plan, G, S, ZM_TMF, true>), grid, block, 0, s, a);
+    break;
+  case ZM_TCOMB:
+    hipLaunchKernelGGL((k_tz<T, Plan, G, S, ZM_TCOMB, true>), grid, block, 0, s, a);
+    break;
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+template <typename T, typename Plan, int G>
+hipError_t tx_r2c_wide_launch(TStepArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
+  hipLaunchKernelGGL((k_tx_r2c<T, Plan, G, true, true>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+template <typename T, typename Plan, int G>
+hipError_t tx_tung_wide_launch(TStepArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
+  hipLaunchKernelGGL((k_tx_tung<T, Plan, G, true, true>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
 template <typename T, typename Plan, int G, bool S, int NR>
 hipError_t ty_fwd_w_launch(TStepArgs<T> a, hipStream_t s) {
   if (a.nfields != 1 && a.nfields != 2) return hipErrorInvalidValue;
@@ -441,6 +473,25 @@ void add_t_y_w(std::vector<TV<T>> &v, int pid) {
   }
 }
 
+// z pass with the loads issued before the multipliers (k_tz<..., PRE>)
+template <typename T, typename Plan, int G, bool S>
+void add_t_z_pre(std::vector<TV<T>> &v, int pid) {
+  if constexpr (t_fits<T, Plan, G, S, true, TK_Z>())
+    v.push_back({plan_name<Plan>() + "/G" + std::to_string(G) + (S ? "/split" : "/full") +
+                     "/pre",
+                 pid, &tz_pre_launch<T, Plan, G, S>});
+}
+// Tungsten row kernels with the wide transposed store
+template <typename T, typename Plan, int G, int KIND>
+void add_t_rows_wide(std::vector<TV<T>> &v, int pid) {
+  if constexpr (G * Plan::TL <= kMaxThreads && G * Plan::TL >= 64 &&
+                tx_inv_lds<T, Plan, G, true>() * (int)sizeof(T) <= kMaxLds) {
+    const std::string nm = plan_name<Plan>() + "/G" + std::to_string(G) + "/wide";
+    if constexpr (KIND == TK_X_R2C) v.push_back({nm, pid, &tx_r2c_wide_launch<T, Plan, G>});
+    if constexpr (KIND == TK_X_TUNG) v.push_back({nm, pid, &tx_tung_wide_launch<T, Plan, G>});
+  }
+}
+
 template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
   std::vector<TV<T>> v;
   if constexpr (N >= 128) {
@@ -489,6 +540,17 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
         add_t<T, P, 1, false, true, KIND>(v, pid);
         add_t<T, P, 2, true, true, KIND>(v, pid);
         add_t<T, P, 2, false, true, KIND>(v, pid);
+        add_t_z_pre<T, P, 1, true>(v, pid);
+        add_t_z_pre<T, P, 1, false>(v, pid);
+        add_t_z_pre<T, P, 2, true>(v, pid);
+      });
+    }
+    if constexpr (KIND == TK_X_R2C || KIND == TK_X_TUNG) {
+      for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+        using P = decltype(p);
+        constexpr int G0 = 128 / (2 * (int)sizeof(T));
+        add_t_rows_wide<T, P, G0 / 2, KIND>(v, pid);
+        add_t_rows_wide<T, P, G0, KIND>(v, pid);
       });
     }
   }
