@@ -30,6 +30,7 @@ FP32 mode (TungstenHIP<float>) beside it.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -425,6 +426,124 @@ def adjoint_run(P, torch, n=256, nsteps=200, every=20):
             "seconds": {"forward_with_checkpoints": t_fwd, "backward": t_bwd}}
 
 
+ENS_SIZES = [32, 48, 64, 96, 128, 192, 256]
+ENS_METRIC = "ensemble DOF-steps/s (64 uneven PFC runs, FP32-mixed, one graph launch per step)"
+
+
+def ensemble_draw(nruns=64, seed=4):
+    """Configuration 4: sizes from ENS_SIZES, r in U(-0.35, -0.15) and the
+    mean in U(-0.30, -0.15), all from one seeded numpy generator."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = [int(v) for v in rng.choice(ENS_SIZES, size=nruns)]
+    eps = [float(v) for v in rng.uniform(-0.35, -0.15, nruns)]
+    means = [float(v) for v in rng.uniform(-0.30, -0.15, nruns)]
+    return sizes, eps, means
+
+
+def ensemble_bytes(sizes, esize):
+    """Algorithmic bytes of one ensemble step: 2R + 12C per run."""
+    return sum(2 * esize * n ** 3 + 12 * 2 * esize * (n // 2 + 1) * n * n for n in sizes)
+
+
+def ensemble_run(P, torch, steps, warmup, nruns=64, seed=4, acc_steps=100):
+    """Configuration 4: nruns independent plain-PFC runs of uneven cubic
+    sizes (dx = pi/4, dt = 0.5), psi0 = mean + U(-0.01, 0.01) (torch's
+    device generator, seed 1000*seed + run), FP32-mixed (each field holds
+    psi - <psi> in FP32; the mean is kept in FP64).  Autotuned per size
+    group (untimed), then `steps` ensemble steps timed, each one hipGraph
+    launch; the same steps enqueued without the graph are timed beside it.
+    The FP32-mixed ensemble is compared with the FP64 ensemble (plain step)
+    from the same psi0 after acc_steps steps."""
+    dx, dt = math.pi / 4, 0.5
+    sizes, eps, means = ensemble_draw(nruns, seed)
+    runs, devs = [], []
+    for i, n in enumerate(sizes):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(1000 * seed + i)
+        noise = torch.rand(n ** 3, dtype=torch.float64, device="cuda", generator=g) * 0.02 - 0.01
+        m = float(noise.mean().item())
+        devs.append(noise - m)
+        runs.append(P.Run.cubic(n, dx, eps[i], dt, means[i] + m))
+    dof = sum(n ** 3 for n in sizes)
+    log(f"ensemble: {nruns} runs, {dof:.3e} DOF")
+    ens = P.Ensemble(runs, precision=P.F32, mean_split=True)
+    for i in range(nruns):
+        ens.field(i).view(-1).copy_(devs[i].float())
+    d0 = ens.buffer.clone()
+    t0 = time.perf_counter()
+    ens.autotune(reps=2)
+    torch.cuda.synchronize()
+    tune_s = time.perf_counter() - t0
+    log(f"ensemble: autotune {tune_s:.1f} s")
+
+    def timed(graph):
+        ens.buffer.copy_(d0)
+        ens.step(warmup, graph=graph)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        ens.step(steps, graph=graph)
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    eager_ms = timed(False)
+    graph_ms = timed(True)
+    log(f"ensemble: graph {graph_ms:.3f} ms/step, eager {eager_ms:.3f} ms/step")
+    alg = ensemble_bytes(sizes, 4)
+    # per-group pass times (eager, HIP events)
+    groups = ens.groups()
+    for g in groups:
+        plan, batch, off = ctypes.c_void_p(), ctypes.c_int(), ctypes.c_size_t()
+        gi = groups.index(g)
+        P.check(ens._lib.pfc_hip_ensemble_group(ens._h, gi, ctypes.byref(plan),
+                                                ctypes.byref(batch), ctypes.byref(off)))
+        out = (ctypes.c_double * 5)()
+        ptr = ctypes.c_void_p(ens.buffer.data_ptr() + 4 * off.value)
+        P.check(ens._lib.pfc_hip_pfc_step_timed(
+            plan, ptr, 5, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream), out, 5))
+        g["pass_ms"] = [round(v, 4) for v in out]
+        g["step_ms"] = round(sum(out), 4)
+        n = g["ext"][0]
+        g["hbm_frac"] = ensemble_bytes([n] * batch.value, 4) / (sum(out) * 1e-3) / HBM_PEAK
+    # FP32-mixed against the FP64 ensemble from the same psi0
+    ens.buffer.copy_(d0)
+    ref = P.Ensemble(runs, precision=P.F64, mean_split=False)
+    for i in range(nruns):
+        ref.field(i).view(-1).copy_(devs[i] + runs[i].mean)
+    ens.step(acc_steps)
+    ref.step(acc_steps)
+    torch.cuda.synchronize()
+    l2, mx, drift = [], [], []
+    for i in range(nruns):
+        a = ref.field(i).view(-1)
+        d = ens.field(i).view(-1).double()
+        diff = d + runs[i].mean - a
+        l2.append(float(diff.norm() / a.norm()))
+        mx.append(float(diff.abs().max() / a.abs().max()))
+        drift.append(abs(float(d.mean())))
+    ref.close()
+    ens.close()
+    return {
+        "metric": ENS_METRIC, "value": dof / (graph_ms * 1e-3), "unit": "DOF-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": graph_ms,
+        "higher_is_better": True, "dtype": "f32 (FP64 mean)", "data": "synthetic",
+        "config": {"workload": f"ensemble of {nruns} plain PFC runs, sizes from {ENS_SIZES}",
+                   "seed": seed, "dx": dx, "dt": dt, "dof": dof,
+                   "sizes": sizes, "r": [round(v, 6) for v in eps],
+                   "mean": [round(v, 6) for v in means]},
+        "eager_ms_per_step": eager_ms, "graph_speedup": eager_ms / graph_ms,
+        "autotune_s": tune_s,
+        "roofline": {"bound": "hbm", "achieved": alg / (graph_ms * 1e-3) / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": alg / (graph_ms * 1e-3) / HBM_PEAK, "alg_bytes_per_step": alg},
+        "groups": groups,
+        "fp32_mixed_vs_fp64": {"steps": acc_steps, "rel_l2_max": max(l2),
+                               "rel_max_max": max(mx), "mean_drift_max": max(drift)},
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -449,6 +568,9 @@ def main(argv=None):
     ap.add_argument("--adjoint", action="store_true",
                     help="configuration 5: adjoint gradients at 256^3, 200 steps, against "
                          "FP64 central differences (no timing line)")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="configuration 4: 64 uneven-size PFC runs (FP32-mixed) advanced "
+                         "together, one hipGraph launch per step; --steps K are timed")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also time a device-to-device copy of 2 x 4 GiB (the plain "
                          "copy kernel whose PMC byte counts scale the traffic figure)")
@@ -468,6 +590,9 @@ def main(argv=None):
         return tungsten_main(args, P, torch)
     if args.adjoint:
         print(json.dumps(adjoint_run(P, torch)), flush=True)
+        return 0
+    if args.ensemble:
+        print(json.dumps(ensemble_run(P, torch, args.steps, args.warmup)), flush=True)
         return 0
     n, dx, dt, r = args.n, math.pi / 4, 0.5, -0.25
     if args.accuracy:

@@ -19,8 +19,10 @@
  *   - Precision PFC_HIP_F64 uses double / double-complex buffers,
  *     PFC_HIP_F32 float / float-complex buffers.
  *
- * Supported extents: every axis length in {1,2,4,...,1024}; nx even.
- * Other lengths are rejected with PFC_HIP_EUNSUPPORTED.
+ * Supported extents: every axis length in {1,2,4,...,1024} or {12,24,48,96,192};
+ * nx even (nx/2 also in that set).  Other lengths are rejected with
+ * PFC_HIP_EUNSUPPORTED.  The transposed step layouts need power-of-two
+ * extents of at least 128; other extents step in the natural layout.
  */
 #ifndef PFC_HIP_H
 #define PFC_HIP_H
@@ -68,6 +70,20 @@ const char *pfc_hip_last_error(void);
  */
 int pfc_hip_plan_create(int nx, int ny, int nz, int precision, int flags,
                         pfc_hip_plan_t *plan);
+
+/*
+ * A plan over `batch` independent fields of one nx*ny*nz grid, stored one
+ * after another (field b starts at b*nx*ny*nz values), stepped together by
+ * pfc_hip_pfc_step / _step_timed / _autotune in the natural layout: one
+ * launch per pass covers every field.  Each field has its own eps and mean
+ * (pfc_hip_pfc_set_members).  Entry points that act on one field (r2c, c2r,
+ * initial and boundary conditions, operator tables, the adjoint, the
+ * Tungsten model, set_mean) return PFC_HIP_EUNSUPPORTED on it; sizes are
+ * per field.  No reference counterpart: the reference steps one Model per
+ * Simulator (docs/api/examples/10_complete_pfc_model.cpp:121-144).
+ */
+int pfc_hip_plan_create_batched(int nx, int ny, int nz, int batch, int precision, int flags,
+                                pfc_hip_plan_t *plan);
 
 /* Release a plan and every device buffer it owns. NULL is accepted. */
 int pfc_hip_plan_destroy(pfc_hip_plan_t plan);
@@ -172,6 +188,19 @@ int pfc_hip_pfc_adjoint_step(pfc_hip_plan_t plan, const void *psi_n, void *lambd
  * step.  No reference counterpart.
  */
 int pfc_hip_pfc_set_mean(pfc_hip_plan_t plan, int enable, double mean);
+
+/*
+ * Per-field eps and mean of a plan's fields (after pfc_hip_pfc_set_operators
+ * with the PFC model, whose spacing and dt all fields share).  eps: `batch`
+ * values, or NULL to keep the operators' eps for every field.  mean_split
+ * != 0 selects the mean-split step of pfc_hip_pfc_set_mean for every field,
+ * with mean[b] the conserved average of field b (then not NULL).  The values
+ * live in device memory read by the step kernels, so a captured graph of
+ * the step picks up later changes.  On a plan with batch 1 this sets the
+ * plan's eps and mean mode.  No reference counterpart.
+ */
+int pfc_hip_pfc_set_members(pfc_hip_plan_t plan, const double *eps, const double *mean,
+                            int mean_split);
 
 /*
  * Measurement variant of pfc_hip_pfc_step: same work, with HIP events
@@ -286,13 +315,14 @@ int pfc_hip_variant_set(pfc_hip_plan_t plan, int role, int index);
 
 /*
  * Time every variant of every step role on full steps of psi (`reps` timed
- * steps after one warm-up each), keep the fastest per role, then keep the
- * faster step layout.  psi is restored before return (the stream is
- * synchronised).  times_ms (may be NULL) receives the mean time per step of
- * the passes each variant runs, for every variant tried, roles in the order
+ * steps after one warm-up each; the three fastest of a role are timed twice
+ * more, round-robin), keep the fastest per role, then keep the faster step
+ * layout.  psi is restored before return (the stream is synchronised).
+ * times_ms (may be NULL) receives the fastest time per step of the passes
+ * each variant runs, for every variant tried, roles in the order
  * 4, 2, 5, 1, 7, 8, 9, 10, 11 (PFC) or 0, 2, 5, 12, 1, 13, 8, 9, 10, 14, 11
  * (Tungsten); roles of an unsupported layout contribute no entries.  Then
- * follow the step times of the transposed and of the natural layout (0 if
+ * follow the fastest step times of the transposed and of the natural layout (0 if
  * not run); up to max_times entries are written.
  */
 int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
@@ -332,6 +362,47 @@ int pfc_hip_field_write_host(const char *path, const void *field, int precision,
                              int truncate);
 int pfc_hip_field_read_host(const char *path, void *field, int precision, int kind,
                             const int *global3, const int *local3, const int *offset3);
+
+/*
+ * Ensembles: independent PFC runs of uneven sizes advanced together.  Run r
+ * has extents ext[3r..3r+2] and params[6r..6r+5] = {dx, dy, dz, eps, dt,
+ * mean}.  Runs with equal extents, spacing and dt form a group stepped by
+ * one batched plan; the fields of all runs live in one caller-owned device
+ * buffer, each group's fields consecutive (groups in order of their first
+ * run, runs in order within a group).  No reference counterpart: the
+ * reference advances one Model per Simulator
+ * (include/openpfc/frontend/simulator.hpp:549-553).
+ *
+ * pfc_hip_ensemble_layout: offsets[r] (may be NULL) receives the index of
+ * the first value of run r, *total (may be NULL) the values of the buffer.
+ * Pure host computation.
+ */
+typedef struct pfc_hip_ensemble_s *pfc_hip_ensemble_t;
+int pfc_hip_ensemble_layout(int nruns, const int *ext, const double *params, size_t *offsets,
+                            size_t *total);
+/*
+ * Create the groups' plans over `fields` (the layout above, precision
+ * PFC_HIP_F64 or PFC_HIP_F32).  mean_split != 0: every field holds the
+ * deviation from its run's mean (the mean-split step of
+ * pfc_hip_pfc_set_mean); 0: the plain step (means ignored).
+ */
+int pfc_hip_ensemble_create(int nruns, const int *ext, const double *params, int precision,
+                            int mean_split, void *fields, pfc_hip_ensemble_t *ens);
+int pfc_hip_ensemble_destroy(pfc_hip_ensemble_t ens);
+/* Number of groups; plan, batch and first value of group g (any may be NULL). */
+int pfc_hip_ensemble_groups(pfc_hip_ensemble_t ens, int *ngroups);
+int pfc_hip_ensemble_group(pfc_hip_ensemble_t ens, int group, pfc_hip_plan_t *plan, int *batch,
+                           size_t *offset);
+/* pfc_hip_pfc_autotune on every group (the fields are restored). */
+int pfc_hip_ensemble_autotune(pfc_hip_ensemble_t ens, int reps, void *stream);
+/*
+ * Advance every run by nsteps.  use_graph != 0: one step of all groups (on
+ * parallel branches) is captured once into a hipGraph -- again after an
+ * autotune -- and each step is one hipGraphLaunch on `stream`; 0: the same
+ * launches are enqueued directly.  Per-run eps and means may be changed
+ * between steps through pfc_hip_pfc_set_members on a group's plan.
+ */
+int pfc_hip_ensemble_step(pfc_hip_ensemble_t ens, int nsteps, int use_graph, void *stream);
 
 #ifdef __cplusplus
 }

@@ -17,10 +17,11 @@ import time
 from . import _lib
 from ._lib import (F32, F64, MODEL_PFC, MODEL_TUNGSTEN, TUNGSTEN_NPARAMS, PfcHipError,
                    check, load)
+from .ensemble import Ensemble, Run
 from .tungsten import TungstenParams
 
-__all__ = ["HipFft", "PfcStepper", "TungstenStepper", "TungstenParams", "PfcHipError",
-           "F32", "F64", "load"]
+__all__ = ["HipFft", "PfcStepper", "TungstenStepper", "TungstenParams", "Ensemble", "Run",
+           "PfcHipError", "F32", "F64", "load"]
 
 
 def _stream_ptr(stream) -> ctypes.c_void_p:

@@ -81,6 +81,23 @@ SIGNATURES = {
     "pfc_hip_variant_set": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "pfc_hip_pfc_autotune": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp,
                                             ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
+    "pfc_hip_plan_create_batched": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.POINTER(ctypes.c_void_p)]),
+    "pfc_hip_pfc_set_members": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double),
+                                               ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
+    "pfc_hip_ensemble_layout": (ctypes.c_int, [ctypes.c_int, _ip3,
+                                               ctypes.POINTER(ctypes.c_double), _sz, _sz]),
+    "pfc_hip_ensemble_create": (ctypes.c_int, [ctypes.c_int, _ip3,
+                                               ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                               ctypes.c_int, _vp,
+                                               ctypes.POINTER(ctypes.c_void_p)]),
+    "pfc_hip_ensemble_destroy": (ctypes.c_int, [_vp]),
+    "pfc_hip_ensemble_groups": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+    "pfc_hip_ensemble_group": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
+                                              ctypes.POINTER(ctypes.c_int), _sz]),
+    "pfc_hip_ensemble_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
+    "pfc_hip_ensemble_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
 }
 
 _lib = None

@@ -7,7 +7,7 @@
 // to HBM directly with coalesced accesses: for a column tile the lanes of a
 // wave are (column, t) pairs, for a row the lanes are consecutive t.
 //
-// Passes are Stockham radix-R steps (R in {2,3,4,5,8,16}); between passes
+// Passes are Stockham radix-R steps (R in {2,3,4,5,6,8,12,16}); between passes
 // the values are exchanged through LDS (re and im as separate planes, so an
 // exchange can also run one plane at a time in half the LDS).  The last pass
 // writes its butterfly outputs straight back into the owning registers, so a
@@ -64,14 +64,14 @@ __device__ __forceinline__ cplx<T> c_rot(cplx<T> a) {
 
 // ---------------------------------------------------------------------------
 // Constant twiddles exp(DIR * 2*pi*i*m/R) for the small in-register DFTs.
-// cos/sin of multiples of pi/8, pi/4 and pi/3, pi*2/5 to full double precision.
+// cos/sin of multiples of pi/8, pi/6 and 2*pi/5 to full double precision.
 // ---------------------------------------------------------------------------
 struct cs {
   double c, s;
 };
 
 constexpr cs unit_root(int m, int R) {
-  // returns (cos(2 pi m/R), sin(2 pi m/R)) for R in {2,3,4,5,8,16}
+  // returns (cos(2 pi m/R), sin(2 pi m/R)) for R in {2,3,4,5,6,8,12,16}
   m = ((m % R) + R) % R;
   constexpr double C8 = 0.92387953251128675613; // cos(pi/8)
   constexpr double S8 = 0.38268343236508977173; // sin(pi/8)
@@ -89,10 +89,14 @@ constexpr cs unit_root(int m, int R) {
   if (R == 8) return unit_root(2 * m, 16);
   if (R == 4) return unit_root(4 * m, 16);
   if (R == 2) return unit_root(8 * m, 16);
-  if (R == 3) {
-    constexpr cs t[3] = {{1, 0}, {-0.5, H3}, {-0.5, -H3}};
+  if (R == 12) {
+    constexpr cs t[12] = {{1, 0},     {H3, 0.5},   {0.5, H3},   {0, 1},
+                          {-0.5, H3}, {-H3, 0.5},  {-1, 0},     {-H3, -0.5},
+                          {-0.5, -H3}, {0, -1},    {0.5, -H3},  {H3, -0.5}};
     return t[m];
   }
+  if (R == 6) return unit_root(2 * m, 12);
+  if (R == 3) return unit_root(4 * m, 12);
   if (R == 5) {
     constexpr cs t[5] = {{1, 0}, {C5a, S5a}, {C5b, S5b}, {C5b, -S5b},
                          {C5a, -S5a}};
@@ -225,6 +229,16 @@ template <int DIR, typename T> struct Dft<8, DIR, T> {
 template <int DIR, typename T> struct Dft<16, DIR, T> {
   __device__ __forceinline__ static void run(cplx<T> *a) {
     DftPQ<4, 4, DIR, T>::run(a);
+  }
+};
+template <int DIR, typename T> struct Dft<6, DIR, T> {
+  __device__ __forceinline__ static void run(cplx<T> *a) {
+    DftPQ<2, 3, DIR, T>::run(a);
+  }
+};
+template <int DIR, typename T> struct Dft<12, DIR, T> {
+  __device__ __forceinline__ static void run(cplx<T> *a) {
+    DftPQ<4, 3, DIR, T>::run(a);
   }
 };
 

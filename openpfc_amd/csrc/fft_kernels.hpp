@@ -229,6 +229,8 @@ template <typename T> struct Nonlin {
 
 template <typename T> struct PfcXArgs {
   Nonlin<T> nl;
+  const T *mnl; // batched plans: per-field (3m, 3m^2) pairs overriding nl.c1/c2, or null
+  long mrows;   // rows per field (ny*nz) when mnl is set
   const T *psi;
   cplx<T> *outA; // Psi half spectrum
   cplx<T> *outB; // N half spectrum
@@ -247,12 +249,18 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_pfc_x(PfcXArgs<T> a) {
   const bool active = row < a.rows;
   const long r = active ? row : 0;
   const T *__restrict__ in = a.psi + r * (long)N;
+  Nonlin<T> nl = a.nl;
+  if (a.mnl) {
+    const long b = r / a.mrows;
+    nl.c1 = a.mnl[2 * b];
+    nl.c2 = a.mnl[2 * b + 1];
+  }
 
   cplx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const T u = in[t + TL * e];
-    v[e] = cplx<T>{u, a.nl(u)};
+    v[e] = cplx<T>{u, nl(u)};
   }
   T *re = lds + c * RP;
   T *im = lds + LPB * RP + c * RP;
@@ -480,6 +488,8 @@ template <typename T> struct PfcZArgs {
   const cplx<T> *tw;
   PfcOp op;
   TungOp top;
+  const double *meps; // batched plans: per-field eps overriding op.eps, or null
+  long mstride;       // distance between the spectra of consecutive fields (P*ny*nz)
   long s_pos; // P*ny
   int pitch;  // P
   int W;      // nx/2+1
@@ -493,12 +503,17 @@ __global__ void __launch_bounds__(L *Plan::TL) k_pfc_z(PfcZArgs<T> a) {
   __shared__ T lds[(SPLIT ? 1 : 2) * L * NP];
   const int c = threadIdx.x % L, t = threadIdx.x / L;
   const int tx = blockIdx.x % a.ntx;
-  const int y = blockIdx.x / a.ntx;
+  // blocks run over (field, y, x tile); one field unless the plan is batched
+  const long yb = blockIdx.x / a.ntx;
+  const int y = (int)(yb % a.op.ny);
+  const long b = yb / a.op.ny;
   const int x = tx * L + c;
   const bool active = x < a.W;
-  const long base = (long)y * a.pitch + x;
+  const long base = b * a.mstride + (long)y * a.pitch + x;
   const LdsLine<T> ll{lds, c, L, L * NP};
-  const double kxy = pfc_kxy(a.op, x, y);
+  PfcOp op = a.op;
+  if (a.meps) op.eps = a.meps[b];
+  const double kxy = pfc_kxy(op, x, y);
 
   cplx<T> acc[E];
   if constexpr (MODE == ZM_PFC) {
@@ -510,7 +525,7 @@ __global__ void __launch_bounds__(L *Plan::TL) k_pfc_z(PfcZArgs<T> a) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       double l, n;
-      pfc_op_mode(a.op, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), l, n);
+      pfc_op_mode(op, kxy, pfc_kc(t + TL * e, op.nz, op.fz), l, n);
       acc[e] = c_scale(acc[e], (T)l);
       opn[e] = (T)n;
     }
@@ -536,7 +551,7 @@ __global__ void __launch_bounds__(L *Plan::TL) k_pfc_z(PfcZArgs<T> a) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       double f, unused;
-      z_mode_ops<ZM_TMF>(a.op, a.top, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), f, unused);
+      z_mode_ops<ZM_TMF>(op, a.top, kxy, pfc_kc(t + TL * e, op.nz, op.fz), f, unused);
       acc[e] = c_scale(acc[e], (T)f);
     }
   } else {
@@ -545,7 +560,7 @@ __global__ void __launch_bounds__(L *Plan::TL) k_pfc_z(PfcZArgs<T> a) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       double l, n;
-      z_mode_ops<ZM_TCOMB>(a.op, a.top, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), l, n);
+      z_mode_ops<ZM_TCOMB>(op, a.top, kxy, pfc_kc(t + TL * e, op.nz, op.fz), l, n);
       opl[e] = (T)l;
       opn[e] = (T)n;
     }

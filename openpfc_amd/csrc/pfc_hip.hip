@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -120,6 +121,12 @@ struct pfc_hip_plan_s {
   double mean = 0.0;
   int var[ROLE_COUNT] = {};
   size_t bytes = 0;
+  // Batched plans step `batch` fields of these extents stored one after
+  // another (natural layout only); per-field eps and (3m, 3m^2) live in
+  // m_eps (double) and m_nl (real type) once `members` is set.
+  int batch = 1;
+  void *m_eps = nullptr, *m_nl = nullptr;
+  bool members = false;
 
   size_t inbox() const { return (size_t)nx * ny * nz; }
   size_t outbox() const { return (size_t)W * ny * nz; }
@@ -221,7 +228,8 @@ void free_all(pfc_hip_plan_s *p) {
   p->tw.clear();
   void **bufs[] = {&p->wn,      &p->work,  &p->A,     &p->B,     &p->A1,
                    &p->B1,      &p->A2,    &p->B2,    &p->adj_lam, &p->adj_P,
-                   &p->adj_N,   &p->adj_a, &p->adj_b, &p->adj_part};
+                   &p->adj_N,   &p->adj_a, &p->adj_b, &p->adj_part, &p->m_eps,
+                   &p->m_nl};
   for (void **b : bufs) {
     if (*b) (void)hipFree(*b);
     *b = nullptr;
@@ -280,7 +288,8 @@ hipError_t col_pass(pfc_hip_plan_s *p, int axis, int dir, const void *src0,
   a.W = p->W;
   a.s_pos = axis == 1 ? (long)pitch : (long)pitch * p->ny;
   a.s_outer = axis == 1 ? (long)pitch * p->ny : (long)pitch;
-  const long n_outer = axis == 1 ? p->nz : p->ny;
+  // y columns of a batched plan run over the z planes of every field
+  const long n_outer = axis == 1 ? (long)p->nz * p->batch : p->ny;
   return v.fn(dir, a, n_outer, nfields, s);
 }
 
@@ -294,7 +303,7 @@ hipError_t x_c2r_pass(pfc_hip_plan_s *p, const void *in, void *out, int pitch,
   a.out = (T *)out;
   a.tw = twp<T>(p, m, v.plan_id);
   a.wn = (const cplx<T> *)p->wn;
-  a.rows = (long)p->ny * p->nz;
+  a.rows = (long)p->ny * p->nz * p->batch;
   a.pitch = pitch;
   a.scale = (T)scale;
   return v.fn(a, s);
@@ -360,10 +369,12 @@ template <typename T> hipError_t z_natural(pfc_hip_plan_s *p, int mode, hipStrea
   a.tw = twp<T>(p, p->nz, v.plan_id);
   a.op = p->op;
   a.top = p->top;
+  a.meps = p->members ? (const double *)p->m_eps : nullptr;
+  a.mstride = (long)p->padded();
   a.s_pos = (long)p->P * p->ny;
   a.pitch = p->P;
   a.W = p->W;
-  return v.fn(a, p->ny, s);
+  return v.fn(a, (long)p->ny * p->batch, s);
 }
 
 // One fused PFC step: 5 launches.  `ev` (optional) receives 6 events
@@ -379,11 +390,13 @@ int step_natural(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
     const auto &v = pfcx_variants<T>(p->nx)[p->var[ROLE_PFC_X]];
     PfcXArgs<T> a{};
     a.nl = nonlin<T>(p);
+    a.mnl = p->members ? (const T *)p->m_nl : nullptr;
+    a.mrows = (long)p->ny * p->nz;
     a.psi = (const T *)psi;
     a.outA = (cplx<T> *)p->A;
     a.outB = (cplx<T> *)p->B;
     a.tw = twp<T>(p, p->nx, v.plan_id);
-    a.rows = (long)p->ny * p->nz;
+    a.rows = (long)p->ny * p->nz * p->batch;
     a.pitch = p->P;
     e = v.fn(a, s);
     PFC_HIP_TRY(e, "pfc forward x pass");
@@ -418,7 +431,7 @@ int step_buffers(pfc_hip_plan_s *p) {
     if (!rc) rc = dalloc(p, &p->A2, c, "step T2 psi");
     if (!rc) rc = dalloc(p, &p->B2, c, "step T2 N");
   } else {
-    const size_t c = p->padded() * 2 * p->esize;
+    const size_t c = p->padded() * 2 * p->esize * p->batch;
     if (!rc) rc = dalloc(p, &p->A, c, "step spectrum A");
     if (!rc) rc = dalloc(p, &p->B, c, "step spectrum B");
   }
@@ -660,6 +673,14 @@ int check_plan(pfc_hip_plan_t p) {
   return 0;
 }
 
+// Entry points that act on one field refuse batched plans.
+int check_single(pfc_hip_plan_t p, const char *fn) {
+  if (int rc = check_plan(p)) return rc;
+  if (p->batch > 1)
+    return fail(PFC_HIP_EUNSUPPORTED, std::string(fn) + ": not available on a batched plan");
+  return 0;
+}
+
 int set_device(pfc_hip_plan_t p) {
   int cur = -1;
   if (hipGetDevice(&cur) == hipSuccess && cur == p->device) return 0;
@@ -694,7 +715,8 @@ int pfc_hip_plan_create(int nx, int ny, int nz, int precision, int flags,
     return fail(PFC_HIP_EUNSUPPORTED,
                 "pfc_hip_plan_create: unsupported extents " + std::to_string(nx) +
                     "x" + std::to_string(ny) + "x" + std::to_string(nz) +
-                    " (each axis a power of two <= 1024, nx even and >= 2)");
+                    " (each axis in {1,2,4,...,1024} or {12,24,48,96,192}; nx even with nx/2 "
+                    "in the same set)");
   auto *p = new (std::nothrow) pfc_hip_plan_s();
   if (!p) return fail(PFC_HIP_ENOMEM, "pfc_hip_plan_create: host allocation");
   p->nx = nx;
@@ -736,6 +758,21 @@ int pfc_hip_plan_create(int nx, int ny, int nz, int precision, int flags,
   return 0;
 }
 
+int pfc_hip_plan_create_batched(int nx, int ny, int nz, int batch, int precision, int flags,
+                                pfc_hip_plan_t *plan) {
+  if (!plan) return fail(PFC_HIP_EINVAL, "pfc_hip_plan_create_batched: plan is null");
+  *plan = nullptr;
+  if (batch < 1 || batch > (1 << 16))
+    return fail(PFC_HIP_EINVAL, "pfc_hip_plan_create_batched: batch must be in [1, 65536]");
+  if ((double)nx * ny * nz * batch > 9.0e18 / 16)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_plan_create_batched: too many points");
+  if (int rc = pfc_hip_plan_create(nx, ny, nz, precision, flags, plan)) return rc;
+  pfc_hip_plan_s *p = *plan;
+  p->batch = batch;
+  if (batch > 1) p->t_ok = false; // the transposed layouts step one field
+  return 0;
+}
+
 int pfc_hip_plan_destroy(pfc_hip_plan_t plan) {
   if (!plan) return 0;
   (void)set_device(plan);
@@ -761,7 +798,7 @@ int pfc_hip_allocated_bytes(pfc_hip_plan_t plan, size_t *bytes) {
 }
 
 int pfc_hip_r2c(pfc_hip_plan_t plan, const void *in, void *out, void *stream) {
-  if (int rc = check_plan(plan)) return rc;
+  if (int rc = check_single(plan, "pfc_hip_r2c")) return rc;
   if (!in || !out) return fail(PFC_HIP_EINVAL, "pfc_hip_r2c: null buffer");
   if (int rc = set_device(plan)) return rc;
   hipStream_t s = (hipStream_t)stream;
@@ -770,7 +807,7 @@ int pfc_hip_r2c(pfc_hip_plan_t plan, const void *in, void *out, void *stream) {
 }
 
 int pfc_hip_c2r(pfc_hip_plan_t plan, const void *in, void *out, void *stream) {
-  if (int rc = check_plan(plan)) return rc;
+  if (int rc = check_single(plan, "pfc_hip_c2r")) return rc;
   if (!in || !out) return fail(PFC_HIP_EINVAL, "pfc_hip_c2r: null buffer");
   if (int rc = set_device(plan)) return rc;
   hipStream_t s = (hipStream_t)stream;
@@ -803,6 +840,9 @@ int pfc_hip_pfc_set_operators(pfc_hip_plan_t plan, int model_kind,
   if (model_kind == PFC_HIP_MODEL_PFC && (!params || nparams != 4))
     return fail(PFC_HIP_EINVAL,
                 "pfc_hip_pfc_set_operators: PFC needs params {dx, dy, dz, eps}");
+  if (model_kind == PFC_HIP_MODEL_TUNGSTEN && plan->batch > 1)
+    return fail(PFC_HIP_EUNSUPPORTED,
+                "pfc_hip_pfc_set_operators: batched plans run the PFC model only");
   if (model_kind == PFC_HIP_MODEL_TUNGSTEN && (!params || nparams != PFC_HIP_TUNGSTEN_NPARAMS))
     return fail(PFC_HIP_EINVAL,
                 "pfc_hip_pfc_set_operators: Tungsten needs " +
@@ -854,7 +894,7 @@ int pfc_hip_pfc_set_operators(pfc_hip_plan_t plan, int model_kind,
 
 int pfc_hip_tungsten_operators(pfc_hip_plan_t plan, void *filterMF, void *expL,
                                void *nW, void *stream) {
-  if (int rc = check_plan(plan)) return rc;
+  if (int rc = check_single(plan, "pfc_hip_tungsten_operators")) return rc;
   if (!filterMF || !expL || !nW)
     return fail(PFC_HIP_EINVAL, "pfc_hip_tungsten_operators: null buffer");
   if (!plan->ops_ready || !plan->tungsten())
@@ -876,7 +916,7 @@ int pfc_hip_tungsten_operators(pfc_hip_plan_t plan, void *filterMF, void *expL,
 
 int pfc_hip_pfc_operators(pfc_hip_plan_t plan, void *opL, void *opN,
                           void *stream) {
-  if (int rc = check_plan(plan)) return rc;
+  if (int rc = check_single(plan, "pfc_hip_pfc_operators")) return rc;
   if (!opL || !opN) return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_operators: null buffer");
   if (!plan->ops_ready || plan->tungsten())
     return fail(PFC_HIP_ESTATE, "pfc_hip_pfc_operators: PFC operators not set");
@@ -936,7 +976,7 @@ int pfc_hip_tungsten_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *psiM
 
 int pfc_hip_pfc_adjoint_step(pfc_hip_plan_t plan, const void *psi_n, void *lambda,
                              double *dr, void *stream) {
-  if (int rc = check_plan(plan)) return rc;
+  if (int rc = check_single(plan, "pfc_hip_pfc_adjoint_step")) return rc;
   if (!psi_n || !lambda) return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_adjoint_step: null field");
   if (psi_n == lambda)
     return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_adjoint_step: lambda must not alias psi_n");
@@ -994,7 +1034,7 @@ int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,
 
 int pfc_hip_ic_constant(pfc_hip_plan_t plan, void *psi, double value,
                         void *stream) {
-  if (int rc = check_plan(plan)) return rc;
+  if (int rc = check_single(plan, "pfc_hip_ic_constant")) return rc;
   if (!psi) return fail(PFC_HIP_EINVAL, "pfc_hip_ic_constant: null psi");
   if (int rc = set_device(plan)) return rc;
   const long n = (long)plan->inbox();
@@ -1012,7 +1052,7 @@ int pfc_hip_ic_constant(pfc_hip_plan_t plan, void *psi, double value,
 int pfc_hip_ic_single_seed(pfc_hip_plan_t plan, void *psi, const double *origin3,
                            const double *spacing3, double rho, double amp,
                            double radius, int keep_outside, void *stream) {
-  if (int rc = check_plan(plan)) return rc;
+  if (int rc = check_single(plan, "pfc_hip_ic_single_seed")) return rc;
   if (!psi || !origin3 || !spacing3)
     return fail(PFC_HIP_EINVAL, "pfc_hip_ic_single_seed: null argument");
   if (int rc = set_device(plan)) return rc;
@@ -1044,7 +1084,7 @@ int pfc_hip_ic_single_seed(pfc_hip_plan_t plan, void *psi, const double *origin3
 
 int pfc_hip_ic_add_noise(pfc_hip_plan_t plan, void *psi, unsigned long long seed,
                          double lo, double hi, void *stream) {
-  if (int rc = check_plan(plan)) return rc;
+  if (int rc = check_single(plan, "pfc_hip_ic_add_noise")) return rc;
   if (!psi) return fail(PFC_HIP_EINVAL, "pfc_hip_ic_add_noise: null psi");
   if (int rc = set_device(plan)) return rc;
   const long n = (long)plan->inbox();
@@ -1061,7 +1101,7 @@ int pfc_hip_ic_add_noise(pfc_hip_plan_t plan, void *psi, unsigned long long seed
 
 int pfc_hip_bc_fixed(pfc_hip_plan_t plan, void *psi, double x0, double dx, double rho_low,
                      double rho_high, double xwidth, double alpha, void *stream) {
-  if (int rc = check_plan(plan)) return rc;
+  if (int rc = check_single(plan, "pfc_hip_bc_fixed")) return rc;
   if (!psi) return fail(PFC_HIP_EINVAL, "pfc_hip_bc_fixed: null psi");
   if (!std::isfinite(x0) || !std::isfinite(dx) || !std::isfinite(xwidth) ||
       !std::isfinite(alpha))
@@ -1136,7 +1176,7 @@ int pfc_hip_variant_set(pfc_hip_plan_t plan, int role, int index) {
     void **drop[4] = {&plan->A1, &plan->B1, &plan->A2, &plan->B2};
     void **drop_nat[2] = {&plan->A, &plan->B};
     const size_t tb = plan->outbox() * 2 * plan->esize;
-    const size_t nb = plan->padded() * 2 * plan->esize;
+    const size_t nb = plan->padded() * 2 * plan->esize * plan->batch;
     if (index == 1) {
       for (void **b : drop)
         if (*b) { (void)hipFree(*b); *b = nullptr; plan->bytes -= tb; }
@@ -1160,9 +1200,10 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
   if (int rc = set_device(plan)) return rc;
   hipStream_t s = (hipStream_t)stream;
   // psi is restored at the end from the c2r workspace (outbox complex >= inbox real)
-  if (int rc = dalloc(plan, &plan->work, plan->outbox() * 2 * plan->esize, "c2r workspace"))
+  if (int rc = dalloc(plan, &plan->work, plan->outbox() * 2 * plan->esize * plan->batch,
+                      "c2r workspace"))
     return rc;
-  const size_t rbytes = plan->inbox() * plan->esize;
+  const size_t rbytes = plan->inbox() * plan->esize * plan->batch;
   PFC_HIP_TRY(hipMemcpyAsync(plan->work, psi, rbytes, hipMemcpyDeviceToDevice, s),
               "autotune backup");
   const int np = plan->step_passes();
@@ -1221,32 +1262,46 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
       if (sp.layout != layout) continue;
       int count = 0;
       pfc_hip_variant_count(plan, sp.role, &count);
-      double best = 1e300;
-      int best_v = plan->var[sp.role];
-      for (int v = 0; v < count && !rc; ++v) {
+      // Each variant is scored by its fastest rep (interference only adds
+      // time); the kTop best are then timed again, round-robin, so that a
+      // drift of the clock over the sweep cannot decide between close ones.
+      std::vector<double> score((size_t)count, 1e300);
+      auto time_variant = [&](int v) {
         plan->var[sp.role] = v;
-        double total = 0, pm[9];
+        double pm[9];
         for (int r = 0; r <= reps && !rc; ++r) { // r == 0 warms up
           rc = run_step(pm);
-          if (!rc && r > 0) total += span_ms(pm, sp.passes);
+          if (!rc && r > 0) score[(size_t)v] = std::min(score[(size_t)v], span_ms(pm, sp.passes));
         }
-        const double ms = total / reps;
+      };
+      for (int v = 0; v < count && !rc; ++v) time_variant(v);
+      constexpr int kTop = 3;
+      std::vector<int> order((size_t)count);
+      for (int v = 0; v < count; ++v) order[(size_t)v] = v;
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int a, int b) { return score[(size_t)a] < score[(size_t)b]; });
+      const int top = std::min(kTop, count);
+      for (int round = 0; round < 2 && top > 1 && !rc; ++round)
+        for (int i = 0; i < top && !rc; ++i) time_variant(order[(size_t)i]);
+      int best_v = plan->var[sp.role];
+      double best = 1e300;
+      for (int v = 0; v < count; ++v) {
         const int idx = slot + v;
-        if (times_ms && idx < max_times) times_ms[idx] = ms;
-        if (ms < best) {
-          best = ms;
+        if (times_ms && idx < max_times) times_ms[idx] = score[(size_t)v];
+        if (score[(size_t)v] < best) {
+          best = score[(size_t)v];
           best_v = v;
         }
       }
       plan->var[sp.role] = best_v;
       slot += count;
     }
-    double total = 0, pm[9];
+    double fastest = 1e300, pm[9];
     for (int r = 0; r <= reps && !rc; ++r) {
       rc = run_step(pm);
-      if (!rc && r > 0) total += span_ms(pm, ~0u);
+      if (!rc && r > 0) fastest = std::min(fastest, span_ms(pm, ~0u));
     }
-    layout_ms[layout] = total / reps;
+    layout_ms[layout] = fastest;
   }
   if (!rc && plan->t_ok) {
     rc = pfc_hip_variant_set(plan, ROLE_STEP_LAYOUT,
@@ -1266,8 +1321,63 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
   return rc;
 }
 
-int pfc_hip_pfc_set_mean(pfc_hip_plan_t plan, int enable, double mean) {
+int pfc_hip_pfc_set_members(pfc_hip_plan_t plan, const double *eps, const double *mean,
+                            int mean_split) {
   if (int rc = check_plan(plan)) return rc;
+  if (!plan->ops_ready || plan->model_kind != PFC_HIP_MODEL_PFC)
+    return fail(PFC_HIP_ESTATE, "pfc_hip_pfc_set_members: PFC operators not set");
+  if (mean_split && !mean)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_set_members: mean_split needs the means");
+  const int nb = plan->batch;
+  std::vector<double> e((size_t)nb, plan->op.eps), m((size_t)nb, 0.0);
+  for (int b = 0; b < nb; ++b) {
+    if (eps) e[(size_t)b] = eps[b];
+    if (mean_split) m[(size_t)b] = mean[b];
+    if (!std::isfinite(e[(size_t)b]) || !std::isfinite(m[(size_t)b]))
+      return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_set_members: value of field " +
+                                      std::to_string(b) + " not finite");
+  }
+  if (int rc = set_device(plan)) return rc;
+  if (nb == 1) { // a plain plan: the scalars are the members
+    plan->op.eps = e[0];
+    plan->mean_split = mean_split != 0;
+    plan->mean = m[0];
+    return 0;
+  }
+  if (int rc = dalloc(plan, &plan->m_eps, (size_t)nb * sizeof(double), "field eps")) return rc;
+  if (int rc = dalloc(plan, &plan->m_nl, (size_t)nb * 2 * plan->esize, "field means"))
+    return rc;
+  // (3m, 3m^2) rounded once to the field type, as for a single field
+  std::vector<double> c2d((size_t)2 * nb);
+  std::vector<float> c2f((size_t)2 * nb);
+  for (int b = 0; b < nb; ++b) {
+    const double mb = m[(size_t)b];
+    c2d[(size_t)2 * b] = 3.0 * mb;
+    c2d[(size_t)2 * b + 1] = 3.0 * mb * mb;
+    c2f[(size_t)2 * b] = (float)c2d[(size_t)2 * b];
+    c2f[(size_t)2 * b + 1] = (float)c2d[(size_t)2 * b + 1];
+  }
+  // the stream order of later steps is kept by the synchronous copies
+  PFC_HIP_TRY(hipDeviceSynchronize(), "pfc_hip_pfc_set_members: sync");
+  PFC_HIP_TRY(hipMemcpy(plan->m_eps, e.data(), (size_t)nb * sizeof(double),
+                        hipMemcpyHostToDevice),
+              "pfc_hip_pfc_set_members: eps upload");
+  if (plan->prec == PFC_HIP_F64)
+    PFC_HIP_TRY(hipMemcpy(plan->m_nl, c2d.data(), c2d.size() * sizeof(double),
+                          hipMemcpyHostToDevice),
+                "pfc_hip_pfc_set_members: mean upload");
+  else
+    PFC_HIP_TRY(hipMemcpy(plan->m_nl, c2f.data(), c2f.size() * sizeof(float),
+                          hipMemcpyHostToDevice),
+                "pfc_hip_pfc_set_members: mean upload");
+  plan->members = true;
+  plan->mean_split = mean_split != 0;
+  plan->mean = 0.0;
+  return 0;
+}
+
+int pfc_hip_pfc_set_mean(pfc_hip_plan_t plan, int enable, double mean) {
+  if (int rc = check_single(plan, "pfc_hip_pfc_set_mean")) return rc;
   if (enable && plan->tungsten())
     return fail(PFC_HIP_EUNSUPPORTED,
                 "pfc_hip_pfc_set_mean: the mean-split mode applies to the PFC model only");

@@ -33,6 +33,24 @@ template <> struct Plans<512> {
                      FftPlan<512, 16, RadixList<16, 16, 2>>,
                      FftPlan<512, 4, RadixList<4, 4, 4, 4, 2>>>;
 };
+// Mixed-radix lengths of the uneven-size ensembles (radix 12 = 4 x 3 and
+// 6 = 2 x 3 in registers, radix 3 as its own pass).
+template <> struct Plans<12> {
+  using L = PlanList<FftPlan<12, 12, RadixList<12>>, FftPlan<12, 12, RadixList<4, 3>>>;
+};
+template <> struct Plans<24> {
+  using L = PlanList<FftPlan<24, 12, RadixList<12, 2>>, FftPlan<24, 12, RadixList<6, 4>>>;
+};
+template <> struct Plans<48> {
+  using L = PlanList<FftPlan<48, 12, RadixList<12, 4>>, FftPlan<48, 12, RadixList<4, 4, 3>>>;
+};
+template <> struct Plans<96> {
+  using L = PlanList<FftPlan<96, 24, RadixList<12, 8>>, FftPlan<96, 12, RadixList<4, 4, 6>>>;
+};
+template <> struct Plans<192> {
+  using L = PlanList<FftPlan<192, 12, RadixList<12, 4, 4>>,
+                     FftPlan<192, 24, RadixList<8, 8, 3>>>;
+};
 template <> struct Plans<1024> {
   using L = PlanList<FftPlan<1024, 16, RadixList<16, 16, 4>>,
                      FftPlan<1024, 8, RadixList<8, 8, 4, 4>>,
@@ -201,7 +219,7 @@ void for_plans(PlanList<Ps...>, F &&f) {
 
 // Lengths with several plans and tile shapes get the full sweep; the rest a
 // single default variant.
-template <int N> constexpr bool sweep_len() { return N >= 128; }
+template <int N> constexpr bool sweep_len() { return N >= 48; }
 
 template <typename T, int N> std::vector<ColV<T>> make_col() {
   std::vector<ColV<T>> v;
@@ -492,9 +510,10 @@ void add_t_rows_wide(std::vector<TV<T>> &v, int pid) {
   }
 }
 
+// Transposed-step kernels exist for power-of-two lengths from 128 up.
 template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
   std::vector<TV<T>> v;
-  if constexpr (N >= 128) {
+  if constexpr (N >= 128 && (N & (N - 1)) == 0) {
     for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
       t_for_plan<T, decltype(p), KIND>(v, pid);
     });
@@ -566,7 +585,12 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
   case 16: { constexpr int N_ = 16; __VA_ARGS__ }                                    \
   case 32: { constexpr int N_ = 32; __VA_ARGS__ }                                    \
   case 64: { constexpr int N_ = 64; __VA_ARGS__ }                                    \
+  case 12: { constexpr int N_ = 12; __VA_ARGS__ }                                    \
+  case 24: { constexpr int N_ = 24; __VA_ARGS__ }                                    \
+  case 48: { constexpr int N_ = 48; __VA_ARGS__ }                                    \
+  case 96: { constexpr int N_ = 96; __VA_ARGS__ }                                    \
   case 128: { constexpr int N_ = 128; __VA_ARGS__ }                                  \
+  case 192: { constexpr int N_ = 192; __VA_ARGS__ }                                  \
   case 256: { constexpr int N_ = 256; __VA_ARGS__ }                                  \
   case 512: { constexpr int N_ = 512; __VA_ARGS__ }                                  \
   case 1024: { constexpr int N_ = 1024; __VA_ARGS__ }                                \

@@ -108,3 +108,15 @@ def test_fft1d_against_direct_sum():
         W = np.exp(-2j * np.pi * np.outer(k, k) / n)
         assert np.abs(O.fft1d(x) - W @ x).max() <= 1e-12 * max(1, n)
         assert np.abs(O.fft1d(x, +1) - W.conj() @ x).max() <= 1e-12 * max(1, n)
+
+
+@pytest.mark.parametrize("shape", [(48, 24, 12), (24, 12, 96), (192, 6, 3), (12, 48, 24)])
+def test_mixed_radix_lengths_match_numpy(shape):
+    """The lengths of the uneven-size ensembles (factors 3 and 2^k) against
+    numpy's rfftn (pocketfft) and back."""
+    nx, ny, nz = shape
+    a = np.random.default_rng(nx * ny + nz).standard_normal((nz, ny, nx))
+    ref = np.fft.rfftn(a)
+    got = O.r2c(a)
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert np.abs(O.c2r(got, nx) - a).max() <= 1e-13 * np.abs(a).max()
