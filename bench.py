@@ -416,6 +416,26 @@ def adjoint_run(P, torch, n=256, nsteps=200, every=20):
     ad = float((g * v).sum().item())
     hr = 1e-6
     fdr = (loss(psi0, -0.25 + hr) - loss(psi0, -0.25 - hr)) / (2 * hr)
+    # FP32-mixed: FP32 deviation field and FP32 adjoint, FP64 mean and r sum
+    xm = psi0.clone().requires_grad_(True)
+    rm = torch.tensor(-0.25, dtype=torch.float64, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    psiTm = pfc_trajectory(xm, rm, nsteps, dt, sp, every, mixed=True)
+    Jm = ((psiTm - target) ** 2).sum()
+    torch.cuda.synchronize()
+    tm_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Jm.backward()
+    torch.cuda.synchronize()
+    tm_bwd = time.perf_counter() - t0
+    adm = float((xm.grad * v).sum().item())
+    grm = float(rm.grad.item())
+    mixed = {"psi0_direction_rel_err": abs(adm - fd) / abs(fd),
+             "r_rel_err": abs(grm - fdr) / abs(fdr),
+             "psiT_rel_l2_vs_fp64": float(((psiTm.detach() - psiT.detach()).norm() /
+                                           psiT.detach().norm()).item()),
+             "seconds": {"forward_with_checkpoints": tm_fwd, "backward": tm_bwd}}
     return {"metric": "adjoint gradients vs FP64 central differences", "grid": [n, n, n],
             "steps": nsteps, "checkpoint_every": every, "precision": "f64",
             "loss": float(J.item()),
@@ -423,7 +443,8 @@ def adjoint_run(P, torch, n=256, nsteps=200, every=20):
                                "rel_err": abs(ad - fd) / abs(fd), "h": h},
             "r": {"adjoint": gr, "central_difference": fdr, "rel_err": abs(gr - fdr) / abs(fdr),
                   "h": hr},
-            "seconds": {"forward_with_checkpoints": t_fwd, "backward": t_bwd}}
+            "seconds": {"forward_with_checkpoints": t_fwd, "backward": t_bwd},
+            "fp32_mixed": mixed}
 
 
 ENS_SIZES = [32, 48, 64, 96, 128, 192, 256]

@@ -171,11 +171,26 @@ int pfc_hip_tungsten_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *psiM
  * <lambda, d step(psi_n) / d eps> is added to *dr first (the stream is
  * synchronised), the derivative taken through opL and opN.  Chained from the
  * last step back, this gives the gradients of a loss on psi_T with respect
- * to psi_0 and eps.  Requires the PFC_HIP_MODEL_PFC operators (not the
- * mean-split mode); lambda must not alias psi_n.  No reference counterpart.
+ * to psi_0 and eps.  Requires the PFC_HIP_MODEL_PFC operators; lambda must
+ * not alias psi_n.  In the mean-split mode (pfc_hip_pfc_set_mean) psi_n holds
+ * the deviation d and the adjoint is taken at psi = mean + d: the same step
+ * map, so lambda stays the gradient with respect to the full field.  No
+ * reference counterpart.
  */
 int pfc_hip_pfc_adjoint_step(pfc_hip_plan_t plan, const void *psi_n, void *lambda,
                              double *dr, void *stream);
+
+/*
+ * Free energy of psi under the plain PFC model of the plan's operators,
+ *   F = dV sum_x [ 1/2 psi (eps + (1 + lap)^2) psi + 1/4 psi^4 ],
+ * the functional whose conserved gradient flow is the step's PDE
+ * (L = -k^2 (eps + (1-k^2)^2), N = psi^3), evaluated spectrally and summed in
+ * double with a fixed order.  In the mean-split mode psi holds the deviation
+ * and the mean's share is formed in double from the plan's exact mean.  The
+ * stream is synchronised.  No reference counterpart (the reference computes
+ * no free energy).
+ */
+int pfc_hip_pfc_free_energy(pfc_hip_plan_t plan, const void *psi, double *F, void *stream);
 
 /*
  * Mean-split step (the FP32-mixed mode): with enable != 0 the field passed

@@ -274,6 +274,15 @@ class PfcStepper(_Stepper):
         check(self.fft._lib.pfc_hip_pfc_step(self.fft.handle, _ptr(psi), int(nsteps),
                                              _stream_ptr(stream)))
 
+    def free_energy(self, psi, stream=None) -> float:
+        """F = dV sum [psi (eps + (1+lap)^2) psi / 2 + psi^4 / 4], summed in
+        double (psi the deviation in the mean-split mode)."""
+        self._check_psi(psi, "free_energy")
+        out = ctypes.c_double()
+        check(self.fft._lib.pfc_hip_pfc_free_energy(self.fft.handle, _ptr(psi),
+                                                    ctypes.byref(out), _stream_ptr(stream)))
+        return out.value
+
     def operators(self, stream=None):
         """(opL, opN) as device tensors in the outbox layout, scaled by 1/(nx ny nz)."""
         import torch

@@ -98,6 +98,7 @@ SIGNATURES = {
                                               ctypes.POINTER(ctypes.c_int), _sz]),
     "pfc_hip_ensemble_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
     "pfc_hip_ensemble_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
+    "pfc_hip_pfc_free_energy": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_double), _vp]),
 }
 
 _lib = None

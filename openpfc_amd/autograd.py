@@ -1,13 +1,15 @@
 """Differentiable PFC trajectories: the libtorch autograd::Function built from
 csrc/torch_pfc.cpp (openpfc_amd/openpfc_amd_torch.so).
 
-    psiT = pfc_trajectory(psi0, eps, nsteps, dt, spacing, every=20)
+    psiT = pfc_trajectory(psi0, eps, nsteps, dt, spacing, every=20, mixed=False)
     loss = ((psiT - target) ** 2).sum(); loss.backward()
     # psi0.grad, eps.grad: adjoint gradients (pfc_hip_pfc_adjoint_step),
     # checkpoints every `every` steps, segments recomputed in the backward
 
 psi0: contiguous float64 device tensor (nz, ny, nx); eps: float64 scalar
-tensor.  Loading fails loudly when the extension was not built.
+tensor.  mixed=True steps and differentiates in the FP32-mixed mode (FP32
+deviation from the FP64 mean, conserved exactly); inputs and outputs stay
+float64.  Loading fails loudly when the extension was not built.
 """
 from __future__ import annotations
 
@@ -36,6 +38,8 @@ def load_extension():
     return mod
 
 
-def pfc_trajectory(psi0, eps, nsteps: int, dt: float, spacing, every: int = 20):
+def pfc_trajectory(psi0, eps, nsteps: int, dt: float, spacing, every: int = 20,
+                   mixed: bool = False):
     return load_extension().pfc_trajectory(psi0, eps, int(nsteps), float(dt),
-                                           [float(s) for s in spacing], int(every))
+                                           [float(s) for s in spacing], int(every),
+                                           bool(mixed))

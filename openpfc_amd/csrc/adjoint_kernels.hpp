@@ -13,6 +13,11 @@
 // Parseval over the half spectrum (weight 2 except on the kx = 0 and
 // kx = nx/2 planes),
 //   <lambda, dpsi'/dr> = (1/N) sum_k w_kx Re(conj(Lambda_k) X_k).
+// Mean-split fields (the FP32-mixed mode) hold d = psi - m with m exact:
+// the same step map written in (m, d), so its adjoint is the one above at
+// psi = m + d.  The cube is then formed as psi^3 - m^3 (as the forward step
+// does) and F(d) stands for F(psi): both differ from the plain terms only at
+// k = 0, where dopL = dopN = 0.
 #pragma once
 
 #include "fft_kernels.hpp"
@@ -43,13 +48,12 @@ __device__ __forceinline__ void pfc_op_deriv(const PfcOp &o, double kxy, double 
   dL = (o.dt * k2_lap) * oL;
 }
 
-// n3 = psi^3 (the reference's psi*psi*psi)
-template <typename T> __global__ void k_cube(const T *psi, T *out, long n) {
+// n3 = psi^3 (the reference's psi*psi*psi), or psi^3 - m^3 from the
+// deviation (Nonlin of the forward step)
+template <typename T> __global__ void k_cube(const T *psi, T *out, long n, Nonlin<T> nl) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
-    const T u = psi[i];
-    out[i] = (u * u) * u;
-  }
+       i += (long)gridDim.x * blockDim.x)
+    out[i] = nl(psi[i]);
 }
 
 // Spectral part of the adjoint step over the outbox (pitch W, x fastest):
@@ -107,12 +111,12 @@ __global__ void __launch_bounds__(256) k_sum_partials(const double *partial, int
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
-// lambda = a + 3 psi^2 b
+// lambda = a + 3 psi^2 b with psi = m + field (m = 0 for a plain field)
 template <typename T>
-__global__ void k_adj_combine(const T *psi, const T *a, const T *b, T *lambda, long n) {
+__global__ void k_adj_combine(const T *psi, const T *a, const T *b, T *lambda, long n, T m) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x) {
-    const T u = psi[i];
+    const T u = m + psi[i];
     lambda[i] = a[i] + (((T)3 * (u * u)) * b[i]);
   }
 }

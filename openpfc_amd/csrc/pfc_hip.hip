@@ -2,6 +2,7 @@
 // and the launch sequences of the R2C/C2R transforms and the fused PFC step.
 #include "../../include/pfc_hip.h"
 #include "adjoint_kernels.hpp"
+#include "energy_kernels.hpp"
 #include "variants.hpp"
 
 #include <hip/hip_runtime.h>
@@ -115,6 +116,7 @@ struct pfc_hip_plan_s {
   int model_kind = -1;
   double dt = 0;
   PfcOp op{};    // k-grid, dt, 1/(nx ny nz); eps of the PFC model
+  double spacing[3] = {}; // dx, dy, dz of the operators
   TungOp top{};  // Tungsten operator scalars
   double tq[5] = {}; // Tungsten nonlinearity: p3_bar, p4_bar, q3_bar, q4_bar, stabP
   bool mean_split = false; // field holds psi - mean; mean kept in double here
@@ -613,7 +615,9 @@ int step_impl(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev,
                          : step_natural<T>(p, psi, s, ev);
 }
 
-constexpr unsigned kAdjBlocks = 2048; // blocks of the spectral adjoint kernel
+constexpr unsigned kAdjBlocks = 2048; // blocks of the reduction kernels
+// partials of two reductions plus their two sums
+constexpr size_t kPartials = 2 * kAdjBlocks + 2;
 
 // Adjoint of one plain PFC step (see adjoint_kernels.hpp).
 template <typename T>
@@ -632,7 +636,7 @@ int adjoint_impl(pfc_hip_plan_s *p, const void *psi, void *lambda, double *dr, h
   if (!rc) rc = dalloc(p, &p->adj_N, cb, "adjoint spectrum");
   if (!rc) rc = dalloc(p, &p->adj_a, rb, "adjoint field");
   if (!rc) rc = dalloc(p, &p->adj_b, rb, "adjoint field");
-  if (!rc) rc = dalloc(p, &p->adj_part, (kAdjBlocks + 1) * sizeof(double), "adjoint partials");
+  if (!rc) rc = dalloc(p, &p->adj_part, kPartials * sizeof(double), "reduction partials");
   if (rc) return rc;
   const long n = (long)p->inbox();
   PfcOp o = p->op;
@@ -640,7 +644,7 @@ int adjoint_impl(pfc_hip_plan_s *p, const void *psi, void *lambda, double *dr, h
   if (dr) {
     if ((rc = r2c_impl<T>(p, psi, p->adj_P, s))) return rc;
     hipLaunchKernelGGL(k_cube<T>, dim3(ic_blocks(n)), dim3(256), 0, s, (const T *)psi,
-                       (T *)p->adj_a, n);
+                       (T *)p->adj_a, n, nonlin<T>(p));
     PFC_HIP_TRY(hipGetLastError(), "adjoint cube");
     if ((rc = r2c_impl<T>(p, p->adj_a, p->adj_N, s))) return rc;
   }
@@ -663,8 +667,39 @@ int adjoint_impl(pfc_hip_plan_s *p, const void *psi, void *lambda, double *dr, h
   if ((rc = c2r_impl<T>(p, p->adj_P, p->adj_a, s))) return rc;
   if ((rc = c2r_impl<T>(p, p->adj_N, p->adj_b, s))) return rc;
   hipLaunchKernelGGL(k_adj_combine<T>, dim3(ic_blocks(n)), dim3(256), 0, s, (const T *)psi,
-                     (const T *)p->adj_a, (const T *)p->adj_b, (T *)lambda, n);
+                     (const T *)p->adj_a, (const T *)p->adj_b, (T *)lambda, n,
+                     (T)(p->mean_split ? p->mean : 0.0));
   PFC_HIP_TRY(hipGetLastError(), "adjoint combine");
+  return 0;
+}
+
+// Free energy of the plain PFC model (energy_kernels.hpp), summed in double.
+template <typename T>
+int free_energy_impl(pfc_hip_plan_s *p, const void *psi, double *F, hipStream_t s) {
+  int rc = dalloc(p, &p->adj_P, p->outbox() * 2 * p->esize, "free-energy spectrum");
+  if (!rc) rc = dalloc(p, &p->adj_part, kPartials * sizeof(double), "reduction partials");
+  if (rc) return rc;
+  if ((rc = r2c_impl<T>(p, psi, p->adj_P, s))) return rc;
+  const double npts = (double)p->nx * (double)p->ny * (double)p->nz;
+  const double m = p->mean_split ? p->mean : 0.0;
+  double *part = (double *)p->adj_part;
+  hipLaunchKernelGGL(k_fe_spectral<T>, dim3(kAdjBlocks), dim3(256), 0, s, p->op, p->W,
+                     (const cplx<T> *)p->adj_P, npts * m, part);
+  PFC_HIP_TRY(hipGetLastError(), "free-energy spectral sum");
+  hipLaunchKernelGGL(k_fe_quartic<T>, dim3(kAdjBlocks), dim3(256), 0, s, (const T *)psi,
+                     (long)p->inbox(), m, part + kAdjBlocks);
+  PFC_HIP_TRY(hipGetLastError(), "free-energy quartic sum");
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, (const double *)part,
+                     (int)kAdjBlocks, part + 2 * kAdjBlocks);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, (const double *)part + kAdjBlocks,
+                     (int)kAdjBlocks, part + 2 * kAdjBlocks + 1);
+  PFC_HIP_TRY(hipGetLastError(), "free-energy reduction");
+  double h[2] = {0, 0};
+  PFC_HIP_TRY(hipMemcpyAsync(h, part + 2 * kAdjBlocks, sizeof(h), hipMemcpyDeviceToHost, s),
+              "free-energy copy");
+  PFC_HIP_TRY(hipStreamSynchronize(s), "free-energy sync");
+  const double dV = p->spacing[0] * p->spacing[1] * p->spacing[2];
+  *F = dV * (h[0] / (2.0 * npts) + 0.25 * h[1]);
   return 0;
 }
 
@@ -880,6 +915,9 @@ int pfc_hip_pfc_set_operators(pfc_hip_plan_t plan, int model_kind,
   o.ny = plan->ny;
   o.nz = plan->nz;
   plan->op = o;
+  plan->spacing[0] = dx;
+  plan->spacing[1] = dy;
+  plan->spacing[2] = dz;
   if (model_kind == PFC_HIP_MODEL_TUNGSTEN) {
     plan->top = tungsten_scalars(params + 3);
     for (int i = 0; i < 4; ++i) plan->tq[i] = params[13 + i]; // p3, p4, q3, q4
@@ -982,13 +1020,21 @@ int pfc_hip_pfc_adjoint_step(pfc_hip_plan_t plan, const void *psi_n, void *lambd
     return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_adjoint_step: lambda must not alias psi_n");
   if (!plan->ops_ready || plan->model_kind != PFC_HIP_MODEL_PFC)
     return fail(PFC_HIP_ESTATE, "pfc_hip_pfc_adjoint_step: PFC operators not set");
-  if (plan->mean_split)
-    return fail(PFC_HIP_EUNSUPPORTED,
-                "pfc_hip_pfc_adjoint_step: the mean-split mode has no adjoint here");
   if (int rc = set_device(plan)) return rc;
   hipStream_t s = (hipStream_t)stream;
   return plan->prec == PFC_HIP_F64 ? adjoint_impl<double>(plan, psi_n, lambda, dr, s)
                                    : adjoint_impl<float>(plan, psi_n, lambda, dr, s);
+}
+
+int pfc_hip_pfc_free_energy(pfc_hip_plan_t plan, const void *psi, double *F, void *stream) {
+  if (int rc = check_single(plan, "pfc_hip_pfc_free_energy")) return rc;
+  if (!psi || !F) return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_free_energy: null argument");
+  if (!plan->ops_ready || plan->model_kind != PFC_HIP_MODEL_PFC)
+    return fail(PFC_HIP_ESTATE, "pfc_hip_pfc_free_energy: PFC operators not set");
+  if (int rc = set_device(plan)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  return plan->prec == PFC_HIP_F64 ? free_energy_impl<double>(plan, psi, F, s)
+                                   : free_energy_impl<float>(plan, psi, F, s);
 }
 
 int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,

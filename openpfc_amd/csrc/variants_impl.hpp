@@ -31,7 +31,8 @@ template <> struct Plans<256> {
 template <> struct Plans<512> {
   using L = PlanList<FftPlan<512, 8, RadixList<8, 8, 8>>,
                      FftPlan<512, 16, RadixList<16, 16, 2>>,
-                     FftPlan<512, 4, RadixList<4, 4, 4, 4, 2>>>;
+                     FftPlan<512, 4, RadixList<4, 4, 4, 4, 2>>,
+                     FftPlan<512, 32, RadixList<16, 16, 2>>>;
 };
 // Mixed-radix lengths of the uneven-size ensembles (radix 12 = 4 x 3 and
 // 6 = 2 x 3 in registers, radix 3 as its own pass).
@@ -54,7 +55,8 @@ template <> struct Plans<192> {
 template <> struct Plans<1024> {
   using L = PlanList<FftPlan<1024, 16, RadixList<16, 16, 4>>,
                      FftPlan<1024, 8, RadixList<8, 8, 4, 4>>,
-                     FftPlan<1024, 4, RadixList<4, 4, 4, 4, 4>>>;
+                     FftPlan<1024, 4, RadixList<4, 4, 4, 4, 4>>,
+                     FftPlan<1024, 32, RadixList<16, 16, 4>>>;
 };
 
 template <typename List> struct RadixOf;
@@ -211,10 +213,24 @@ void rows_for_plan(std::vector<V> &v, int pid, Make make) {
   (void)pid;
 }
 
+// Plans with 32 values per thread serve the single-precision transposed
+// kernels only (in double they would hold 64 registers of data per thread).
+template <typename T, typename P, bool TRANSPOSED> constexpr bool use_plan() {
+  return P::E < 32 || (TRANSPOSED && sizeof(T) == 4);
+}
+template <typename T, bool TRANSPOSED, typename P, typename F>
+void call_plan(F &f, int pid) {
+  if constexpr (use_plan<T, P, TRANSPOSED>()) f(P{}, pid);
+}
 template <typename T, int N, typename F, typename... Ps>
 void for_plans(PlanList<Ps...>, F &&f) {
   int pid = 0;
-  ((f(Ps{}, pid++)), ...);
+  ((call_plan<T, false, Ps>(f, pid), ++pid), ...);
+}
+template <typename T, int N, typename F, typename... Ps>
+void for_t_plans(PlanList<Ps...>, F &&f) {
+  int pid = 0;
+  ((call_plan<T, true, Ps>(f, pid), ++pid), ...);
 }
 
 // Lengths with several plans and tile shapes get the full sweep; the rest a
@@ -514,11 +530,11 @@ void add_t_rows_wide(std::vector<TV<T>> &v, int pid) {
 template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
   std::vector<TV<T>> v;
   if constexpr (N >= 128 && (N & (N - 1)) == 0) {
-    for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+    for_t_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
       t_for_plan<T, decltype(p), KIND>(v, pid);
     });
     if constexpr (KIND == TK_X_FWD) {
-      for_plans<T, N / 2>(typename Plans<N / 2>::L{}, [&](auto p, int pid) {
+      for_t_plans<T, N / 2>(typename Plans<N / 2>::L{}, [&](auto p, int pid) {
         using P = decltype(p);
         constexpr int G0 = 128 / (2 * (int)sizeof(T));
         add_t_fwd2<T, P, G0, true>(v, pid);
@@ -528,7 +544,7 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
       });
     }
     if constexpr (KIND == TK_X_FWD) {
-      for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+      for_t_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
         using P = decltype(p);
         constexpr int G0 = 128 / (2 * (int)sizeof(T));
         add_t_fwd_w<T, P, G0 / 2, true>(v, pid);
@@ -539,7 +555,7 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
       });
     }
     if constexpr (KIND == TK_Y_FWD || KIND == TK_Y_INV) {
-      for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+      for_t_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
         using P = decltype(p);
         constexpr int G0 = 128 / (2 * (int)sizeof(T));
         add_t_y_w<T, P, G0, true, 2, KIND>(v, pid);
@@ -553,7 +569,7 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
     if constexpr (KIND == TK_Z) {
       // one or two lines per block: waves of different blocks drift apart, so
       // loads of one overlap the transforms of another
-      for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+      for_t_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
         using P = decltype(p);
         add_t<T, P, 1, true, true, KIND>(v, pid);
         add_t<T, P, 1, false, true, KIND>(v, pid);
@@ -565,7 +581,7 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
       });
     }
     if constexpr (KIND == TK_X_R2C || KIND == TK_X_TUNG) {
-      for_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+      for_t_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
         using P = decltype(p);
         constexpr int G0 = 128 / (2 * (int)sizeof(T));
         add_t_rows_wide<T, P, G0 / 2, KIND>(v, pid);

@@ -63,6 +63,8 @@ def lib():
                                             ctypes.c_double, ctypes.c_double]
         L.orc_ic_single_seed_masked.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 9
                                                 + [ctypes.c_int, _dp])
+        L.orc_pfc_free_energy.restype = ctypes.c_double
+        L.orc_pfc_free_energy.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [_dp]
         L.orc_sum.restype = ctypes.c_double
         L.orc_sum.argtypes = [_dp, ctypes.c_size_t]
         L.orc_sum_sq.restype = ctypes.c_double
@@ -213,6 +215,13 @@ def pfc_adjoint_step(psi, lam, opL, opN, dopL=None, dopN=None):
     lib().orc_pfc_adjoint_step(nx, ny, nz, _p(opL), _p(opN), _p(dl), _p(dn), _p(p), _p(l),
                                ctypes.cast(ctypes.pointer(dr), _dp) if want else None)
     return l, (dr.value if want else None)
+
+
+def pfc_free_energy(psi: np.ndarray, spacing, eps: float) -> float:
+    """F = dV [sum_k |Psi_k|^2 (eps + (1-k^2)^2) / (2N) + sum psi^4 / 4]."""
+    nz, ny, nx = psi.shape
+    p = np.ascontiguousarray(psi, dtype=np.float64)
+    return float(lib().orc_pfc_free_energy(nx, ny, nz, *map(float, spacing), float(eps), _p(p)))
 
 
 def bc_fixed(a: np.ndarray, x0: float, dx: float, rho_low: float, rho_high: float,

@@ -26,6 +26,8 @@
  *       include/openpfc/kernel/field/operations.hpp:81-105 (x-fastest loop)
  *       include/openpfc/kernel/data/domain.hpp:185-189 (x = origin + i*dx)
  *   - adjoint of the plain PFC step (its transpose; no reference code)
+ *   - free energy of the plain PFC model (the functional of that step's
+ *     PDE; no reference code)
  *   - FixedBC boundary condition:
  *       include/openpfc/kernel/simulation/boundary_conditions/fixed_bc.hpp
  *   - Tungsten mean-field PFC operators and step:
@@ -590,6 +592,42 @@ int orc_pfc_adjoint_step(int nx, int ny, int nz, const double *opL, const double
   for (size_t i = 0; i < nr; ++i) lambda[i] = a[i] + 3.0 * psi[i] * psi[i] * b[i];
   free(L); free(X); free(Y); free(a); free(b);
   return 0;
+}
+
+/* Free energy of the plain PFC model (no reference counterpart: the
+ * functional whose conserved gradient flow is the step of
+ * 10_complete_pfc_model.cpp:121-144):
+ *   F = dV [ 1/(2N) sum_k |Psi_k|^2 (eps + (1-k^2)^2) + 1/4 sum_x psi^4 ],
+ * the k sum over the half spectrum with weight 2 off the kx = 0 and
+ * kx = nx/2 planes, accumulated in long double. */
+double orc_pfc_free_energy(int nx, int ny, int nz, double dx, double dy, double dz, double eps,
+                           const double *psi) {
+  const size_t nr = (size_t)nx * ny * nz;
+  const int w = nx / 2 + 1;
+  cdbl *X = (cdbl *)malloc(sizeof(cdbl) * (size_t)w * ny * nz);
+  if (!X) return NAN;
+  orc_r2c3d(nx, ny, nz, psi, (double *)X);
+  const double two_pi = 6.283185307179586;
+  const double fx = two_pi / (dx * nx), fy = two_pi / (dy * ny), fz = two_pi / (dz * nz);
+  long double spec = 0, quart = 0;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < w; ++i) {
+        const double kx = orc_k_component(i, nx, fx), ky = orc_k_component(j, ny, fy);
+        const double kz = orc_k_component(k, nz, fz);
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        const double om = 1.0 - k2;
+        const cdbl v = X[((size_t)k * ny + j) * w + i];
+        const double wt = (i == 0 || 2 * i == nx) ? 1.0 : 2.0;
+        spec += (long double)wt * ((long double)creal(v) * creal(v) +
+                                   (long double)cimag(v) * cimag(v)) * (eps + om * om);
+      }
+  for (size_t i = 0; i < nr; ++i) {
+    const long double u2 = (long double)psi[i] * psi[i];
+    quart += u2 * u2;
+  }
+  free(X);
+  return (double)((long double)dx * dy * dz * (spec / (2.0L * (long double)nr) + 0.25L * quart));
 }
 
 /* ------------------------------------------------------------------------ */

@@ -7,7 +7,10 @@ the libtorch autograd::Function over trajectories (openpfc_amd_torch):
   directions) and eps against FP64 central finite differences of the
   device forward, within 1e-6 relative;
 * the gradients do not depend on the checkpoint interval (the segments are
-  recomputed bit for bit).
+  recomputed bit for bit);
+* the FP32-mixed mode (FP32 deviation from the exact FP64 mean): one adjoint
+  step against the oracle's FP64 adjoint at psi = mean + d within 1e-5, and
+  trajectory gradients within 1e-4 of FP64 central differences.
 """
 import ctypes
 import math
@@ -107,3 +110,61 @@ def test_gradients_independent_of_checkpoint_interval(torch_gpu):
         out.append((psi0.grad.clone(), float(eps.grad.item())))
     for g, ge in out[1:]:
         assert torch.equal(g, out[0][0]) and ge == out[0][1]
+
+
+def test_mixed_adjoint_step_matches_fp64_oracle(torch_gpu):
+    import openpfc_amd as P
+    torch = torch_gpu
+    n = (32, 16, 16)
+    eps = -0.3
+    psi, lam = _fields(torch, n, 15)
+    m = float(psi.mean())
+    f = P.HipFft(*n, precision=P.F32)
+    st = P.PfcStepper(f, (DX,) * 3, eps)
+    st.initialize(DT)
+    st.set_mean(m)
+    dpsi = torch.from_numpy(psi - m).float().cuda()
+    dlam = torch.from_numpy(lam).float().cuda()
+    dr = ctypes.c_double(0.0)
+    P.check(f._lib.pfc_hip_pfc_adjoint_step(f.handle, ctypes.c_void_p(dpsi.data_ptr()),
+                                            ctypes.c_void_p(dlam.data_ptr()),
+                                            ctypes.byref(dr), None))
+    opL, opN = O.pfc_operators(n, (DX,) * 3, eps, DT)
+    dL, dN = O.pfc_operator_derivs(n, (DX,) * 3, eps, DT)
+    ref, ref_dr = O.pfc_adjoint_step(psi, lam, opL, opN, dL, dN)
+    got = dlam.double().cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"mixed adjoint step: rel max {err:.2e}, dr rel {abs(dr.value - ref_dr) / abs(ref_dr):.2e}")
+    assert err <= 1e-5
+    assert abs(dr.value - ref_dr) <= 1e-5 * abs(ref_dr), (dr.value, ref_dr)
+
+
+def test_mixed_trajectory_gradients_close_to_fp64_central_differences(torch_gpu):
+    from openpfc_amd.autograd import pfc_trajectory
+    torch = torch_gpu
+    n, nsteps = (32, 32, 32), 10
+    psi0_np, target_np = _fields(torch, n, 17)
+    target = torch.from_numpy(target_np).cuda()
+    psi0 = torch.from_numpy(psi0_np).cuda().requires_grad_(True)
+    eps = torch.tensor(-0.3, dtype=torch.float64, requires_grad=True)
+    psiT = pfc_trajectory(psi0, eps, nsteps, DT, (DX,) * 3, every=4, mixed=True)
+    ref_T = pfc_trajectory(psi0.detach(), eps.detach(), nsteps, DT, (DX,) * 3, every=4)
+    assert psiT.dtype == torch.float64
+    fwd = float(((psiT.detach() - ref_T).norm() / ref_T.norm()).item())
+    assert fwd <= 1e-5, fwd
+    J = ((psiT - target) ** 2).sum()
+    J.backward()
+    g, ge = psi0.grad.detach(), float(eps.grad.item())
+    rng = np.random.default_rng(18)
+    base = psi0.detach()
+    v = torch.from_numpy(rng.standard_normal(psi0_np.shape)).cuda()
+    h = 1e-6
+    fd = (_loss(torch, pfc_trajectory, base + h * v, -0.3, target, nsteps) -
+          _loss(torch, pfc_trajectory, base - h * v, -0.3, target, nsteps)) / (2 * h)
+    ad = float((g * v).sum().item())
+    fdr = (_loss(torch, pfc_trajectory, base, -0.3 + h, target, nsteps) -
+           _loss(torch, pfc_trajectory, base, -0.3 - h, target, nsteps)) / (2 * h)
+    print(f"mixed gradients: psi0 direction rel {abs(ad - fd) / abs(fd):.2e}, "
+          f"r rel {abs(ge - fdr) / abs(fdr):.2e}, forward rel L2 {fwd:.2e}")
+    assert abs(fd - ad) <= 1e-4 * abs(fd), (fd, ad)
+    assert abs(fdr - ge) <= 1e-4 * abs(fdr), (fdr, ge)
