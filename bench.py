@@ -316,7 +316,7 @@ def tungsten_main(args, P, torch):
         with open(tpath) as fh:
             tk = json.load(fh).get("kernels", {})
         if kkey in tk:
-            traffic = tk[kkey]["traffic_bytes"]
+            traffic = (tk[kkey][0] + tk[kkey][1]) * 1e9
     out = {
         "metric": TUNG_METRIC, "value": 1000.0 / ms, "unit": "steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
@@ -650,7 +650,7 @@ def main(argv=None):
         with open(tpath) as fh:
             tk = json.load(fh).get("kernels", {})
         if kkey in tk:
-            traffic = tk[kkey]["traffic_bytes"]
+            traffic = (tk[kkey][0] + tk[kkey][1]) * 1e9
 
     out = {
         "metric": METRIC,

@@ -14,8 +14,8 @@ counter reads for the plain device-to-device copy of `bench.py
 the calibration value is the median over the trailing __amd_rocclr_copyBuffer
 dispatches with the grid size of the last one (other copies, such as the
 autotuner's backup of the field, come earlier and differ in size).  The scaled bytes per dispatch of every kernel go to the
-markdown table and, with --traffic-json, to a JSON file bench.py uses for
-roofline.traffic.
+markdown table and, for the FP64 kernels, with --traffic-json to a JSON file
+({name: [GB read, GB written]}) bench.py uses for roofline.traffic.
 """
 from __future__ import annotations
 
@@ -139,9 +139,13 @@ def main(argv=None):
                 lines.append(f"| `{k}` | {rb / 1e9:.3f} | {wb / 1e9:.3f} | "
                              f"{(rb + wb) / 1e9:.3f} |")
         if a.traffic_json:
+            # FP64 step kernels only (the table bench.py reads), GB per dispatch
+            table = {k: [round(v["read_bytes"] / 1e9, 4), round(v["write_bytes"] / 1e9, 4)]
+                     for k, v in sorted(traffic.items()) if "<double" in k}
             with open(a.traffic_json, "w") as fh:
                 json.dump({"command": a.cmd, "scale": {"fetch": sf, "write": sw},
-                           "kernels": traffic}, fh, indent=1)
+                           "units": "GB read, GB written per dispatch",
+                           "kernels": table}, fh, indent=0)
     with open(a.out, "w") as fh:
         fh.write("\n".join(lines) + "\n")
     print("\n".join(lines))
