@@ -33,6 +33,30 @@
 
 namespace pfc_amd {
 
+// XCD-aware block order.  A block's G lines are adjacent in the strided
+// dimension, so when G values of cplx<T> fill less than a 128-byte line, the
+// P = 128 / (G * sizeof(cplx<T>)) blocks of consecutive line groups share
+// every line they touch there.  Workgroups go to the 8 XCDs round-robin
+// (physical block b on XCD b mod 8) and each XCD has its own L2, so those P
+// blocks are made to run on one XCD, back to back: physical block
+// 8*(P*q + r) + x executes logical block P*(8*q + x) + r.  The last partial
+// group of 8*P blocks keeps its order; the map is a bijection on [0, n).
+template <typename T, int G> constexpr unsigned line_partners() {
+  constexpr int bytes = G * 2 * (int)sizeof(T);
+  return bytes >= 128 ? 1u : 128u / bytes;
+}
+template <typename T, int G>
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned n) {
+  constexpr unsigned P = line_partners<T, G>(), X = 8, S = X * P;
+  if constexpr (P <= 1) {
+    return b;
+  } else {
+    if (b >= n / S * S) return b;
+    const unsigned x = b % X, r = b / X;
+    return ((r / P) * X + x) * P + (r % P);
+  }
+}
+
 // LDS plane size (elements) of G lines of an N-point plan, row layout.
 template <typename Plan, int G> constexpr int t_plane() {
   return G * row_padded(Plan::N);
@@ -98,7 +122,8 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd(TStepArgs<T> a) {
   __shared__ T lds[(SPLIT ? 1 : 2) * PL];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngy = (a.ny + G - 1) / G;
-  const int z = blockIdx.x / ngy, y0 = (blockIdx.x % ngy) * G;
+  const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
+  const int z = (int)bid / ngy, y0 = ((int)bid % ngy) * G;
   const int y = y0 + g;
   const bool active = y < a.ny;
   const T *__restrict__ row = a.psi_in + ((long)z * a.ny + (active ? y : y0)) * N;
@@ -173,7 +198,8 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd(TStepArgs<T> a) {
   __shared__ T lds[(SPLIT ? 1 : 2) * PL];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngz = (a.nz + G - 1) / G;
-  const int kx = blockIdx.x / ngz, z0 = (blockIdx.x % ngz) * G;
+  const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
+  const int kx = (int)bid / ngz, z0 = ((int)bid % ngz) * G;
   const int z = z0 + g;
   const bool active = z < a.nz;
   const cplx<T> *__restrict__ in = blockIdx.y ? a.B1 : a.A1;
@@ -313,7 +339,8 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv(TStepArgs<T> a) {
   __shared__ T lds[(SPLIT ? 1 : 2) * PL];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngz = (a.nz + G - 1) / G;
-  const int kx = blockIdx.x / ngz, z0 = (blockIdx.x % ngz) * G;
+  const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
+  const int kx = (int)bid / ngz, z0 = ((int)bid % ngz) * G;
   const int z = z0 + g;
 
   cplx<T> v[E];
@@ -363,7 +390,8 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
   __shared__ T lds[tx_inv_lds<T, Plan, G, TILE>()];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngy = (a.ny + G - 1) / G;
-  const int z = blockIdx.x / ngy, y0 = (blockIdx.x % ngy) * G;
+  const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
+  const int z = (int)bid / ngy, y0 = ((int)bid % ngy) * G;
   const int y = y0 + g;
 
   cplx<T> v[E];
@@ -509,7 +537,8 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_r2c(TStepArgs<T> a) {
   __shared__ T lds[WIDE ? tx_inv_lds<T, Plan, G, true>() : 2 * t_plane<Plan, G>()];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngy = (a.ny + G - 1) / G;
-  const int z = blockIdx.x / ngy, y0 = (blockIdx.x % ngy) * G;
+  const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
+  const int z = (int)bid / ngy, y0 = ((int)bid % ngy) * G;
   const int y = y0 + g < a.ny ? y0 + g : y0;
   const cplx<T> *__restrict__ row =
       reinterpret_cast<const cplx<T> *>(a.psi_in + ((long)z * a.ny + y) * (2L * M));
@@ -539,7 +568,8 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_tung(TStepArgs<T> a) {
   __shared__ T lds[tx_inv_lds<T, Plan, G, TILE>()];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngy = (a.ny + G - 1) / G;
-  const int z = blockIdx.x / ngy, y0 = (blockIdx.x % ngy) * G;
+  const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
+  const int z = (int)bid / ngy, y0 = ((int)bid % ngy) * G;
   const int y = y0 + g;
   const int yy = y < a.ny ? y : y0;
   cplx<T> *__restrict__ spec = a.B1;
@@ -606,7 +636,8 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd2(TStepArgs<T> a) {
   __shared__ T lds[2 * t_plane<Plan, G>()];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngy = (a.ny + G - 1) / G;
-  const int z = blockIdx.x / ngy, y0 = (blockIdx.x % ngy) * G;
+  const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
+  const int z = (int)bid / ngy, y0 = ((int)bid % ngy) * G;
   const int y = y0 + g < a.ny ? y0 + g : y0;
   const cplx<T> *__restrict__ row =
       reinterpret_cast<const cplx<T> *>(a.psi_in + ((long)z * a.ny + y) * (2L * M));
@@ -645,7 +676,8 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w(TStepArgs<T> a) {
   __shared__ T lds[tx_fwd_w_lds<T, Plan, G, SPLIT>()];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngy = (a.ny + G - 1) / G;
-  const int z = blockIdx.x / ngy, y0 = (blockIdx.x % ngy) * G;
+  const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
+  const int z = (int)bid / ngy, y0 = ((int)bid % ngy) * G;
   const int y = y0 + g;
   const bool active = y < a.ny;
   const T *__restrict__ row = a.psi_in + ((long)z * a.ny + (active ? y : y0)) * N;
@@ -717,7 +749,8 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd_w(TStepArgs<T> a) {
   __shared__ T lds[ty_w_lds<T, Plan, G, SPLIT, NR>()];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngz = (a.nz + G - 1) / G;
-  const int kx = blockIdx.x / ngz, z0 = (blockIdx.x % ngz) * G;
+  const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
+  const int kx = (int)bid / ngz, z0 = ((int)bid % ngz) * G;
   const int z = z0 + g;
   const bool active = z < a.nz;
   const cplx<T> *__restrict__ in = blockIdx.y ? a.B1 : a.A1;
@@ -759,7 +792,8 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv_w(TStepArgs<T> a) {
   __shared__ T lds[ty_w_lds<T, Plan, G, SPLIT, NR>()];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngz = (a.nz + G - 1) / G;
-  const int kx = blockIdx.x / ngz, z0 = (blockIdx.x % ngz) * G;
+  const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
+  const int kx = (int)bid / ngz, z0 = ((int)bid % ngz) * G;
   const int z = z0 + g;
 
   cplx<T> v[E];
