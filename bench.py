@@ -13,7 +13,11 @@ psi_hat = opL*F(psi) + opN*F(psi^3), psi = F^-1(psi_hat)
     python bench.py --model tungsten     # Tungsten mean-field step (see below)
 
 Prints one JSON line.  `value` is FP64 steps/s with the field resident in
-HBM; the same workload in FP32-mixed mode (FP32 deviation field and
+HBM, timed over one K-step call (which fuses step boundaries when the
+autotuner found that faster); `step_hbm_frac` is the per-step model bytes
+2R + 12C (R a real field, C a half spectrum) over the step time, and
+`run_hbm_frac` the bytes the timed run itself needs per step (12C + 2R/K when
+step boundaries are fused).  The same workload in FP32-mixed mode (FP32 deviation field and
 transforms, FP64 conserved mean) is reported under "fp32_mixed" with its
 error against the FP64 run after the same steps.  `roofline` is the
 dominant kernel's algorithmic bytes per launch over its mean launch duration
@@ -54,7 +58,9 @@ PASS_KERNELS = {
     "natural": [("k_pfc_x", "pfc_x"), ("k_col", "col_y"), ("k_pfc_z", "pfc_z"),
                 ("k_col", "col_y"), ("k_x_c2r", "x_c2r")]}
 STEP_ROLES = ["step_layout", "pfc_x", "col_y", "pfc_z", "x_c2r", "t_x_fwd", "t_y_fwd",
-              "t_z", "t_y_inv", "t_x_inv"]
+              "t_z", "t_y_inv", "t_x_inv", "t_x_fuse", "step_fuse"]
+# launches of a fused multi-step run (pfc_hip_pfc_step_timed with 6 entries)
+FUSED_NAMES = ["x_fwd_first", "y_fwd", "z_fused", "y_inv", "x_inv_last", "x_boundary_fused"]
 TUNG_METRIC = "Tungsten mean-field step, steps/s at 1024^3 FP64 (% of HBM peak)"
 TUNG_PASS_NAMES = ["x_fwd_r2c", "y_fwd", "z_meanfield", "y_inv_mf", "x_fused_nl",
                    "y_fwd_n", "z_combine", "y_inv", "x_inv_c2r"]
@@ -73,6 +79,13 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
     """rocprof short name (openpfc_amd.profile_summary.short_name) of a variant;
     `mode` is the z-pass kind (0 PFC, 1 Tungsten mean field, 2 Tungsten combine)."""
     import re
+    t = "double" if prec == "f64" else "float"
+    if kernel == "k_tx_fuse":  # "rA/EA+rB/EB/G8/split": nx/2- and nx-point plans
+        half, full = variant.split("+")
+        ra, ea = half.split("/")
+        pb = full.split("/")
+        return (f"k_tx_fuse<{t},N{n // 2}/E{ea[1:]},r{ra[1:]}+N{n}/E{pb[1][1:]},r{pb[0][1:]},"
+                f"{pb[2][1:]},{'true' if pb[3] == 'split' else 'false'}>")
     parts = variant.split("/")
     if kernel == "k_tx_fwd" and "r2c2" in parts:
         kernel, n = "k_tx_fwd2", n // 2
@@ -115,6 +128,13 @@ def pass_bytes(n: int, esize: int):
     C = 2 * esize * (n // 2 + 1) * n * n   # one half spectrum
     return {"x_fwd_pfc": R + 2 * C, "y_fwd": 4 * C, "z_fused": 3 * C,
             "y_inv": 2 * C, "x_inv_c2r": C + R}
+
+
+def fused_pass_bytes(n: int, esize: int):
+    """Algorithmic bytes per launch of the steady-state passes of a fused
+    multi-step run: y forward 4C, z 3C, y inverse 2C, fused boundary C + 2C."""
+    C = 2 * esize * (n // 2 + 1) * n * n
+    return {"y_fwd": 4 * C, "z_fused": 3 * C, "y_inv": 2 * C, "x_boundary_fused": 3 * C}
 
 
 def tung_pass_bytes(n: int, esize: int):
@@ -630,6 +650,7 @@ def main(argv=None):
             json.dump(chosen64, fh, indent=1)
     ms64, wall64 = time_steps(torch, st, psi, args.steps, args.warmup)
     kms = st.step_timed(psi, 3)
+    kf = st.step_timed_fused(psi, max(args.steps, 2))
     finite = bool(torch.isfinite(psi).all().item())
     mean64 = psi.mean().item()
     psi64 = psi
@@ -637,13 +658,30 @@ def main(argv=None):
     torch.cuda.empty_cache()
 
     pb = pass_bytes(n, 8)
-    dom = max(range(5), key=lambda i: kms[i])
-    dname = PASS_NAMES[dom]
-    achieved = pb[dname] / (kms[dom] * 1e-3)
     step_bytes = sum(pb.values())
     layout = chosen64.get("step_layout", "natural")
-    kern, role = PASS_KERNELS[layout][dom]
-    kkey = variant_key(kern, "f64", n, chosen64[role], direction=(-1 if dom == 1 else 1))
+    fused = kf[5] > 0
+    if fused:
+        # the timed K-step call fuses step boundaries: its steady-state
+        # passes are y forward, z, y inverse and the fused boundary
+        fb = fused_pass_bytes(n, 8)
+        fk = dict(zip(FUSED_NAMES, kf))
+        dname = max(fb, key=lambda k: fk[k])
+        achieved = fb[dname] / (fk[dname] * 1e-3)
+        kern, role = {"y_fwd": ("k_ty_fwd", "t_y_fwd"), "z_fused": ("k_tz", "t_z"),
+                      "y_inv": ("k_ty_inv", "t_y_inv"),
+                      "x_boundary_fused": ("k_tx_fuse", "t_x_fuse")}[dname]
+        alg_launch = fb[dname]
+        run_bytes = sum(fb.values()) + 2 * 8 * n ** 3 / args.steps
+    else:
+        dom = max(range(5), key=lambda i: kms[i])
+        dname = PASS_NAMES[dom]
+        achieved = pb[dname] / (kms[dom] * 1e-3)
+        kern, role = PASS_KERNELS[layout][dom]
+        alg_launch = pb[dname]
+        run_bytes = step_bytes
+    kkey = variant_key(kern, "f64", n, chosen64[role],
+                       direction=(-1 if dname == "y_fwd" else 1))
     traffic = None
     tpath = args.traffic or os.path.join(ROOT, "profiles", f"traffic_pfc_step_{n}_f64.json")
     if os.path.exists(tpath):
@@ -671,12 +709,15 @@ def main(argv=None):
         "ns_per_dof_step": ms64 * 1e6 / n ** 3,
         "step_hbm_frac": step_bytes / (ms64 * 1e-3) / HBM_PEAK,
         "step_alg_bytes": step_bytes,
+        "run_bytes_per_step": run_bytes,
+        "run_hbm_frac": run_bytes / (ms64 * 1e-3) / HBM_PEAK,
         "wall_s_timed": wall64,
         "kernels_ms": dict(zip(PASS_NAMES, kms)),
+        "fused_run_ms": dict(zip(FUSED_NAMES, kf)) if fused else None,
         "roofline": {"bound": "hbm", "kernel": kkey, "pass": dname,
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "alg_bytes_per_launch": pb[dname]},
+                     "alg_bytes_per_launch": alg_launch},
         "variants": chosen64,
         "checks": {"finite": finite, "mean_after": mean64},
     }
@@ -695,6 +736,7 @@ def main(argv=None):
         chosen32 = choose_variants(P, fft, st, d, args, "fp32")
         ms32, _ = time_steps(torch, st, d, args.steps, args.warmup)
         k32 = st.step_timed(d, 3)
+        k32f = st.step_timed_fused(d, max(args.steps, 2))
         got = d.double() + mean0
         diff = got - psi64
         l2 = (torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(psi64)).item()
@@ -708,8 +750,14 @@ def main(argv=None):
             "ns_per_dof_step": ms32 * 1e6 / n ** 3,
             "speedup_vs_fp64": ms64 / ms32,
             "step_hbm_frac": sum(pass_bytes(n, 4).values()) / (ms32 * 1e-3) / HBM_PEAK,
-            "kernels_ms": dict(zip(PASS_NAMES, k32)), "variants": chosen32,
-            "error_vs_fp64_after": {"steps": args.warmup + args.steps + 3,
+            "run_hbm_frac": ((sum(fused_pass_bytes(n, 4).values()) + 2 * 4 * n ** 3 / args.steps)
+                             if k32f[5] > 0 else sum(pass_bytes(n, 4).values()))
+                            / (ms32 * 1e-3) / HBM_PEAK,
+            "kernels_ms": dict(zip(PASS_NAMES, k32)),
+            "fused_run_ms": dict(zip(FUSED_NAMES, k32f)) if k32f[5] > 0 else None,
+            "variants": chosen32,
+            "error_vs_fp64_after": {"steps": args.warmup + 2 * args.steps + 3 if args.steps >= 2
+                                    else args.warmup + args.steps + 5,
                                     "rel_l2": l2, "rel_max": mx, "mean_drift": drift}}
     del psi64
     torch.cuda.empty_cache()

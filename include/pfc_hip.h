@@ -143,7 +143,13 @@ int pfc_hip_pfc_set_operators(pfc_hip_plan_t plan, int model_kind,
  * Advance psi (device, inbox real values, updated in place) by nsteps ETD1
  * steps: psi_hat <- opL*F(psi) + opN*F(psi^3); psi <- F^-1(psi_hat).
  * Replaces PFCModel::step (10_complete_pfc_model.cpp:121-144) called
- * nsteps times.  Requires pfc_hip_pfc_set_operators.
+ * nsteps times.  Requires pfc_hip_pfc_set_operators.  With nsteps >= 2 in
+ * the transposed layout and role 16 set to 1, the inverse x pass of each
+ * step and the forward x pass of the next run as one fused pass that keeps
+ * psi on chip (role 15), so psi is written once at the end: 2R fewer bytes
+ * per step boundary.  The fused pass may pair different transform plans
+ * than the two passes it replaces, so such a call agrees with repeated
+ * single steps to rounding, not bit for bit.
  */
 int pfc_hip_pfc_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *stream);
 
@@ -226,7 +232,11 @@ int pfc_hip_pfc_set_members(pfc_hip_plan_t plan, const double *eps, const double
  * Tungsten (nkernels >= 9): 0 forward-x (psi), 1 forward-y, 2 z (forward,
  * mean-field filter, inverse), 3 inverse-y, 4 x (inverse, N, forward),
  * 5 forward-y, 6 z (forward, combine, inverse), 7 inverse-y, 8 inverse-x.
- * No reference counterpart (profiling aid).
+ * PFC with nkernels >= 6 and nsteps >= 2 where pfc_hip_pfc_step fuses step
+ * boundaries: the launches of that fused run, kernel_ms[0] the forward x of
+ * the first step, 1..3 the mean y, z and inverse y passes, 4 the inverse x of
+ * the last step and 5 the mean fused boundary pass (otherwise kernel_ms[5]
+ * = 0).  No reference counterpart (profiling aid).
  */
 int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,
                            void *stream, double *kernel_ms, int nkernels);
@@ -298,6 +308,11 @@ int pfc_hip_tungsten_operators(pfc_hip_plan_t plan, void *filterMF, void *expL,
  *   12 natural Tungsten step: fused x pass (inverse, N, forward)
  *   13 transposed Tungsten step: forward x of psi
  *   14 transposed Tungsten step: fused x pass
+ *   15 transposed PFC multi-step runs: fused step boundary (inverse x of a
+ *      step with the forward x of the next)
+ *   16 multi-step runs: 0 as single steps (default), 1 with fused step
+ *      boundaries (transposed layout; the autotuner picks 1 when the fused
+ *      pass takes under 95% of the time of the two it replaces)
  * The Tungsten step also uses roles 0, 1, 2, 5 (natural) and 8..11
  * (transposed).  Index 0 is the default of each role.  No reference counterpart: the
  * reference delegates shapes to rocFFT; its GPU autotuner only times 1-D
@@ -319,7 +334,9 @@ enum {
   PFC_HIP_ROLE_X_TUNG = 12,
   PFC_HIP_ROLE_T_X_R2C = 13,
   PFC_HIP_ROLE_T_X_TUNG = 14,
-  PFC_HIP_ROLE_COUNT = 15
+  PFC_HIP_ROLE_T_X_FUSE = 15,
+  PFC_HIP_ROLE_STEP_FUSE = 16,
+  PFC_HIP_ROLE_COUNT = 17
 };
 int pfc_hip_variant_count(pfc_hip_plan_t plan, int role, int *count);
 /* Name of a variant ("r16x16x4/E16/L8/split"), NULL if out of range. */
@@ -335,8 +352,9 @@ int pfc_hip_variant_set(pfc_hip_plan_t plan, int role, int index);
  * layout.  psi is restored before return (the stream is synchronised).
  * times_ms (may be NULL) receives the fastest time per step of the passes
  * each variant runs, for every variant tried, roles in the order
- * 4, 2, 5, 1, 7, 8, 9, 10, 11 (PFC) or 0, 2, 5, 12, 1, 13, 8, 9, 10, 14, 11
- * (Tungsten); roles of an unsupported layout contribute no entries.  Then
+ * 4, 2, 5, 1, 7, 8, 9, 10, 11, 15 (PFC) or 0, 2, 5, 12, 1, 13, 8, 9, 10, 14, 11
+ * (Tungsten); roles of an unsupported layout contribute no entries (role 15,
+ * the fused step boundary, is timed on its own launches).  Then
  * follow the fastest step times of the transposed and of the natural layout (0 if
  * not run); up to max_times entries are written.
  */

@@ -97,7 +97,8 @@ class HipFft:
     # -- launch variants ------------------------------------------------------
     ROLES = {"x_r2c": 0, "x_c2r": 1, "col_y": 2, "col_z": 3, "pfc_x": 4, "pfc_z": 5,
              "step_layout": 6, "t_x_fwd": 7, "t_y_fwd": 8, "t_z": 9, "t_y_inv": 10,
-             "t_x_inv": 11, "x_tung": 12, "t_x_r2c": 13, "t_x_tung": 14}
+             "t_x_inv": 11, "x_tung": 12, "t_x_r2c": 13, "t_x_tung": 14, "t_x_fuse": 15,
+             "step_fuse": 16}
 
     def variants(self, role) -> list:
         r = self.ROLES.get(role, role)
@@ -247,7 +248,7 @@ class PfcStepper(_Stepper):
 
     NPASSES = 5
     TUNE_ROLES = ["pfc_x", "col_y", "pfc_z", "x_c2r", "t_x_fwd", "t_y_fwd", "t_z",
-                  "t_y_inv", "t_x_inv"]
+                  "t_y_inv", "t_x_inv", "t_x_fuse"]
     NATURAL_ROLES = 4
 
     def __init__(self, fft: HipFft, spacing, epsilon: float):
@@ -255,6 +256,17 @@ class PfcStepper(_Stepper):
         self.spacing = tuple(float(s) for s in spacing)
         self.epsilon = float(epsilon)
         self.dt = None
+
+    def step_timed_fused(self, psi, nsteps: int, stream=None):
+        """The launches of a fused multi-step run (nsteps >= 2): returns
+        [forward x (first step), y forward, z, y inverse (means), inverse x
+        (last step), fused boundary (mean)] in ms; the last entry is 0 when
+        the plan does not fuse step boundaries."""
+        self._check_psi(psi, "step_timed_fused")
+        out = (ctypes.c_double * 6)()
+        check(self.fft._lib.pfc_hip_pfc_step_timed(self.fft.handle, _ptr(psi), int(nsteps),
+                                                   _stream_ptr(stream), out, 6))
+        return list(out)
 
     def initialize(self, dt: float) -> None:
         params = (ctypes.c_double * 4)(*self.spacing, self.epsilon)

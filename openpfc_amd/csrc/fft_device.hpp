@@ -289,9 +289,30 @@ __device__ __forceinline__ int pad_pos(int pos) {
     return pos;
 }
 
+// Ordering of the LDS exchanges of one line.  WAVE: the line's threads lie
+// in one wavefront (row layouts with TL <= 64 dividing 64), whose LDS
+// instructions execute in program order, so a wavefront barrier (which also
+// keeps the compiler from moving LDS accesses across it) is enough and the
+// other lines of the block need not wait.  Otherwise a block barrier.
+template <bool WAVE> __device__ __forceinline__ void lds_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// Row layouts (a line's TL threads consecutive in the block) whose lines fit
+// in one wavefront.
+template <typename Plan> constexpr bool row_wave() {
+  return Plan::TL <= 64 && 64 % Plan::TL == 0;
+}
+
 // One Stockham pass (radix R, current span NS) followed, when more passes
 // remain, by an LDS exchange.
-template <typename T, typename Plan, int DIR, bool SPLIT, bool ROWPAD, int NS,
+template <typename T, typename Plan, int DIR, bool SPLIT, bool ROWPAD, bool WAVE, int NS,
           int TWOFF, int R, int... Rest>
 struct Pass {
   static constexpr int N = Plan::N, E = Plan::E, TL = Plan::TL;
@@ -329,7 +350,7 @@ struct Pass {
     }
     if constexpr (!LAST) {
       exchange(v, t, lds);
-      Pass<T, Plan, DIR, SPLIT, ROWPAD, NS * R,
+      Pass<T, Plan, DIR, SPLIT, ROWPAD, WAVE, NS * R,
            TWOFF + (NS > 1 ? NS * (R - 1) : 0), Rest...>::run(v, t, lds, tw);
     }
   }
@@ -349,21 +370,21 @@ struct Pass {
 #pragma unroll
         for (int q = 0; q < R; ++q)
           lds.base[lds.idx(pad_pos<ROWPAD>(dst_pos(t, b, q)))] = v[b + NB * q].x;
-      __syncthreads();
+      lds_sync<WAVE>();
 #pragma unroll
       for (int e = 0; e < E; ++e)
         v[e].x = lds.base[lds.idx(pad_pos<ROWPAD>(t + TL * e))];
-      __syncthreads();
+      lds_sync<WAVE>();
 #pragma unroll
       for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int q = 0; q < R; ++q)
           lds.base[lds.idx(pad_pos<ROWPAD>(dst_pos(t, b, q)))] = v[b + NB * q].y;
-      __syncthreads();
+      lds_sync<WAVE>();
 #pragma unroll
       for (int e = 0; e < E; ++e)
         v[e].y = lds.base[lds.idx(pad_pos<ROWPAD>(t + TL * e))];
-      __syncthreads();
+      lds_sync<WAVE>();
     } else {
 #pragma unroll
       for (int b = 0; b < NB; ++b)
@@ -373,27 +394,30 @@ struct Pass {
           lds.base[i] = v[b + NB * q].x;
           lds.base[i + lds.plane] = v[b + NB * q].y;
         }
-      __syncthreads();
+      lds_sync<WAVE>();
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int i = lds.idx(pad_pos<ROWPAD>(t + TL * e));
         v[e].x = lds.base[i];
         v[e].y = lds.base[i + lds.plane];
       }
-      __syncthreads();
+      lds_sync<WAVE>();
     }
   }
 };
 
-template <typename T, typename Plan, int DIR, bool SPLIT, bool ROWPAD,
+// One line transform.  WAVE (see lds_sync): the exchanges wait for the
+// line's own wavefront only; the caller then separates any use of the LDS
+// by other lines, before or after, with a block barrier.
+template <typename T, typename Plan, int DIR, bool SPLIT, bool ROWPAD, bool WAVE = false,
           typename List = typename Plan::List>
 struct LineFft;
-template <typename T, typename Plan, int DIR, bool SPLIT, bool ROWPAD, int... Rs>
-struct LineFft<T, Plan, DIR, SPLIT, ROWPAD, RadixList<Rs...>> {
+template <typename T, typename Plan, int DIR, bool SPLIT, bool ROWPAD, bool WAVE, int... Rs>
+struct LineFft<T, Plan, DIR, SPLIT, ROWPAD, WAVE, RadixList<Rs...>> {
   __device__ __forceinline__ static void run(cplx<T> (&v)[Plan::E], int t,
                                              const LdsLine<T> &lds,
                                              const cplx<T> *__restrict__ tw) {
-    Pass<T, Plan, DIR, SPLIT, ROWPAD, 1, 0, Rs...>::run(v, t, lds, tw);
+    Pass<T, Plan, DIR, SPLIT, ROWPAD, WAVE, 1, 0, Rs...>::run(v, t, lds, tw);
   }
 };
 

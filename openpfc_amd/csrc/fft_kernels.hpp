@@ -25,6 +25,30 @@
 
 namespace pfc_amd {
 
+// XCD-aware block order.  A block's G lines (or L columns) are adjacent in
+// the strided dimension, so when G values of cplx<T> fill less than a 128-byte line, the
+// P = 128 / (G * sizeof(cplx<T>)) blocks of consecutive line groups share
+// every line they touch there.  Workgroups go to the 8 XCDs round-robin
+// (physical block b on XCD b mod 8) and each XCD has its own L2, so those P
+// blocks are made to run on one XCD, back to back: physical block
+// 8*(P*q + r) + x executes logical block P*(8*q + x) + r.  The last partial
+// group of 8*P blocks keeps its order; the map is a bijection on [0, n).
+template <typename T, int G> constexpr unsigned line_partners() {
+  constexpr int bytes = G * 2 * (int)sizeof(T);
+  return bytes >= 128 ? 1u : 128u / bytes;
+}
+template <typename T, int G>
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned n) {
+  constexpr unsigned P = line_partners<T, G>(), X = 8, S = X * P;
+  if constexpr (P <= 1) {
+    return b;
+  } else {
+    if (b >= n / S * S) return b;
+    const unsigned x = b % X, r = b / X;
+    return ((r / P) * X + x) * P + (r % P);
+  }
+}
+
 // ----------------------------------------------------------------------------
 // Column transform along y or z.
 //   column (x, outer) starts at outer*s_outer + x; position p at + p*s_pos.
@@ -52,8 +76,9 @@ __global__ void __launch_bounds__(L *Plan::TL)
   const cplx<T> *__restrict__ src = a.src[f];
   cplx<T> *__restrict__ dst = a.dst[f];
   const int c = threadIdx.x % L, t = threadIdx.x / L;
-  const int tx = blockIdx.x % a.ntx;
-  const long outer = blockIdx.x / a.ntx;
+  const unsigned bid = xcd_block<T, L>(blockIdx.x, gridDim.x);
+  const int tx = (int)(bid % (unsigned)a.ntx);
+  const long outer = bid / (unsigned)a.ntx;
   const int x = tx * L + c;
   const bool active = x < a.W;
   const long base = outer * a.s_outer + x;
@@ -74,7 +99,7 @@ __global__ void __launch_bounds__(L *Plan::TL)
 // Row helpers: fetch the value held at line position `pp(e)` for every owned
 // element, through LDS (row layout with padding).
 // ----------------------------------------------------------------------------
-template <typename T, int E, int TL, typename PartnerPos>
+template <typename T, int E, int TL, bool WAVE = false, typename PartnerPos>
 __device__ __forceinline__ void row_fetch_partner(const cplx<T> (&v)[E],
                                                   cplx<T> (&p)[E], int t,
                                                   T *lds_re, T *lds_im,
@@ -85,13 +110,13 @@ __device__ __forceinline__ void row_fetch_partner(const cplx<T> (&v)[E],
     lds_re[i] = v[e].x;
     lds_im[i] = v[e].y;
   }
-  __syncthreads();
+  lds_sync<WAVE>();
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int i = pad_pos<true>(pp(t + TL * e));
     p[e] = cplx<T>{lds_re[i], lds_im[i]};
   }
-  __syncthreads();
+  lds_sync<WAVE>();
 }
 
 // ----------------------------------------------------------------------------
@@ -160,10 +185,10 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_x_r2c(XArgs<T> a) {
   T *re = lds + c * RP;
   T *im = lds + LPB * RP + c * RP;
   const LdsLine<T> ll{lds, c * RP, 1, LPB * RP};
-  LineFft<T, Plan, -1, false, true>::run(v, t, ll, a.tw);
+  LineFft<T, Plan, -1, false, true, row_wave<Plan>()>::run(v, t, ll, a.tw);
 
   cplx<T> pz[E];
-  row_fetch_partner<T, E, TL>(v, pz, t, re, im,
+  row_fetch_partner<T, E, TL, row_wave<Plan>()>(v, pz, t, re, im,
                               [](int k) { return k == 0 ? 0 : M - k; });
   if (!active) return;
 #pragma unroll
@@ -201,7 +226,7 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_x_c2r(XArgs<T> a) {
     v[e] = c2r_merge(in[k], in[M - k], a.wn[k], k == 0);
   }
   const LdsLine<T> ll{lds, c * RP, 1, LPB * RP};
-  LineFft<T, Plan, +1, false, true>::run(v, t, ll, a.tw);
+  LineFft<T, Plan, +1, false, true, row_wave<Plan>()>::run(v, t, ll, a.tw);
   if (!active) return;
 #pragma unroll
   for (int e = 0; e < E; ++e) out[t + TL * e] = c_scale(v[e], a.scale);
@@ -265,10 +290,10 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_pfc_x(PfcXArgs<T> a) {
   T *re = lds + c * RP;
   T *im = lds + LPB * RP + c * RP;
   const LdsLine<T> ll{lds, c * RP, 1, LPB * RP};
-  LineFft<T, Plan, -1, false, true>::run(v, t, ll, a.tw);
+  LineFft<T, Plan, -1, false, true, row_wave<Plan>()>::run(v, t, ll, a.tw);
 
   cplx<T> pz[E];
-  row_fetch_partner<T, E, TL>(v, pz, t, re, im,
+  row_fetch_partner<T, E, TL, row_wave<Plan>()>(v, pz, t, re, im,
                               [](int k) { return k == 0 ? 0 : N - k; });
   if (!active) return;
   cplx<T> *__restrict__ oa = a.outA + r * a.pitch;
@@ -502,9 +527,10 @@ __global__ void __launch_bounds__(L *Plan::TL) k_pfc_z(PfcZArgs<T> a) {
   constexpr int NP = row_padded(N);
   __shared__ T lds[(SPLIT ? 1 : 2) * L * NP];
   const int c = threadIdx.x % L, t = threadIdx.x / L;
-  const int tx = blockIdx.x % a.ntx;
+  const unsigned bid = xcd_block<T, L>(blockIdx.x, gridDim.x);
+  const int tx = (int)(bid % (unsigned)a.ntx);
   // blocks run over (field, y, x tile); one field unless the plan is batched
-  const long yb = blockIdx.x / a.ntx;
+  const long yb = bid / (unsigned)a.ntx;
   const int y = (int)(yb % a.op.ny);
   const long b = yb / a.op.ny;
   const int x = tx * L + c;
@@ -618,7 +644,7 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_x_tung(TungXArgs<T> a) {
     const int k = t + TL * e;
     v[e] = c2r_merge(sp[k], sp[M - k], a.wn[k], k == 0);
   }
-  LineFft<T, Plan, +1, false, true>::run(v, t, ll, a.tw);
+  LineFft<T, Plan, +1, false, true, row_wave<Plan>()>::run(v, t, ll, a.tw);
   // v[e] = (psi_MF[2k], psi_MF[2k+1]), k = t + TL*e
   const cplx<T> *__restrict__ pr =
       reinterpret_cast<const cplx<T> *>(a.psi + r * (2L * M));
@@ -632,9 +658,9 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_x_tung(TungXArgs<T> a) {
     const cplx<T> u = pr[t + TL * e];
     v[e] = cplx<T>{a.nl(u.x, v[e].x), a.nl(u.y, v[e].y)};
   }
-  LineFft<T, Plan, -1, false, true>::run(v, t, ll, a.tw);
+  LineFft<T, Plan, -1, false, true, row_wave<Plan>()>::run(v, t, ll, a.tw);
   cplx<T> pz[E];
-  row_fetch_partner<T, E, TL>(v, pz, t, lds + c * RP, lds + LPB * RP + c * RP,
+  row_fetch_partner<T, E, TL, row_wave<Plan>()>(v, pz, t, lds + c * RP, lds + LPB * RP + c * RP,
                               [](int k) { return k == 0 ? 0 : M - k; });
   if (!active) return;
 #pragma unroll

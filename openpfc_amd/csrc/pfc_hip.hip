@@ -144,6 +144,7 @@ struct pfc_hip_plan_s {
     case ROLE_T_Z: return nz;
     case ROLE_T_X_INV: return nx / 2;
     case ROLE_X_TUNG: case ROLE_T_X_R2C: case ROLE_T_X_TUNG: return nx / 2;
+    case ROLE_T_X_FUSE: return nx / 2;
     default: return 0;
     }
   }
@@ -213,6 +214,7 @@ int t_kind_of_role(int role) {
   if (role >= ROLE_T_X_FWD && role <= ROLE_T_X_INV) return role - ROLE_T_X_FWD;
   if (role == ROLE_T_X_R2C) return TK_X_R2C;
   if (role == ROLE_T_X_TUNG) return TK_X_TUNG;
+  if (role == ROLE_T_X_FUSE) return TK_X_FUSE;
   return -1;
 }
 
@@ -242,6 +244,8 @@ void free_all(pfc_hip_plan_s *p) {
 template <typename T> int variant_count(pfc_hip_plan_s *p, int role) {
   const int n = p->role_len(role);
   if (role == ROLE_STEP_LAYOUT) return p->t_ok ? 2 : 1;
+  if (role == ROLE_STEP_FUSE)
+    return p->t_ok && !t_variants<T>(TK_X_FUSE, p->role_len(ROLE_T_X_FUSE)).empty() ? 2 : 1;
   if (t_kind_of_role(role) >= 0) return (int)t_variants<T>(t_kind_of_role(role), n).size();
   switch (role) {
   case ROLE_X_R2C: return (int)xr2c_variants<T>(n).size();
@@ -256,9 +260,11 @@ template <typename T> int variant_count(pfc_hip_plan_s *p, int role) {
 
 template <typename T> const std::string *variant_name(pfc_hip_plan_s *p, int role, int v) {
   static const std::string layouts[2] = {"transposed", "natural"};
+  static const std::string fuse[2] = {"per_step", "fused"};
   const int n = p->role_len(role);
   if (v < 0 || v >= variant_count<T>(p, role)) return nullptr;
   if (role == ROLE_STEP_LAYOUT) return &layouts[p->t_ok ? v : 1];
+  if (role == ROLE_STEP_FUSE) return &fuse[v];
   if (t_kind_of_role(role) >= 0) return &t_variants<T>(t_kind_of_role(role), n)[v].name;
   switch (role) {
   case ROLE_X_R2C: return &xr2c_variants<T>(n)[v].name;
@@ -440,11 +446,7 @@ int step_buffers(pfc_hip_plan_s *p) {
   return rc;
 }
 
-template <typename T>
-int step_transposed(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
-  auto mark = [&](int i) {
-    if (ev) (void)hipEventRecord(ev[i], s);
-  };
+template <typename T> TStepArgs<T> pfc_t_args(pfc_hip_plan_s *p, void *psi) {
   TStepArgs<T> a{};
   a.psi_in = (const T *)psi;
   a.psi_out = (T *)psi;
@@ -462,18 +464,70 @@ int step_transposed(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev)
   a.ny = p->ny;
   a.nz = p->nz;
   a.W = p->W;
-  const int roles[5] = {ROLE_T_X_FWD, ROLE_T_Y_FWD, ROLE_T_Z, ROLE_T_Y_INV, ROLE_T_X_INV};
-  const char *what[5] = {"step x forward", "step y forward", "step z pass",
-                         "step y inverse", "step x inverse"};
+  return a;
+}
+
+// Pass k of the transposed PFC step: 0 forward x, 1 forward y, 2 z,
+// 3 inverse y, 4 inverse x; 5 the fused step boundary (inverse x of one step
+// with the forward x of the next).
+constexpr int kTPassRoles[6] = {ROLE_T_X_FWD, ROLE_T_Y_FWD, ROLE_T_Z,
+                                ROLE_T_Y_INV, ROLE_T_X_INV, ROLE_T_X_FUSE};
+template <typename T>
+int t_pass(pfc_hip_plan_s *p, TStepArgs<T> a, int k, hipStream_t s) {
+  static const char *what[6] = {"step x forward", "step y forward", "step z pass",
+                                "step y inverse", "step x inverse", "step boundary x"};
+  const int role = kTPassRoles[k];
+  const int n = p->role_len(role);
+  const auto &v = t_variants<T>(t_kind_of_role(role), n)[p->var[role]];
+  const cplx<T> *tw = twp<T>(p, v.half ? n / 2 : n, v.plan_id);
+  a.tw_x = a.tw_xm = a.tw_y = a.tw_z = tw;
+  if (v.plan_id2 >= 0) a.tw_x = twp<T>(p, 2 * n, v.plan_id2);
+  PFC_HIP_TRY(v.fn(a, s), what[k]);
+  return 0;
+}
+
+template <typename T>
+int step_transposed(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], s);
+  };
+  const TStepArgs<T> a = pfc_t_args<T>(p, psi);
   mark(0);
   for (int k = 0; k < 5; ++k) {
-    const int n = p->role_len(roles[k]);
-    const auto &v = t_variants<T>(k, n)[p->var[roles[k]]];
-    const cplx<T> *tw = twp<T>(p, v.half ? n / 2 : n, v.plan_id);
-    a.tw_x = a.tw_xm = a.tw_y = a.tw_z = tw;
-    PFC_HIP_TRY(v.fn(a, s), what[k]);
+    if (int rc = t_pass<T>(p, a, k, s)) return rc;
     mark(k + 1);
   }
+  return 0;
+}
+
+// Whether multi-step PFC runs fuse their step boundaries (role 16).
+template <typename T> bool fused_steps(pfc_hip_plan_s *p) {
+  return p->transposed() && !p->tungsten() && p->var[ROLE_STEP_FUSE] == 1 &&
+         !t_variants<T>(TK_X_FUSE, p->role_len(ROLE_T_X_FUSE)).empty();
+}
+
+// nsteps >= 1 PFC steps with fused boundaries: forward x once, then per
+// step the y, z and inverse y passes, with the fused boundary pass between
+// steps, and the inverse x once at the end (4*nsteps + 1 launches).  `ev`
+// (optional) receives 4*nsteps + 2 events around the launches, in order.
+template <typename T>
+int steps_fused(pfc_hip_plan_s *p, void *psi, int nsteps, hipStream_t s, hipEvent_t *ev) {
+  int e = 0;
+  auto mark = [&]() {
+    if (ev) (void)hipEventRecord(ev[e++], s);
+  };
+  const TStepArgs<T> a = pfc_t_args<T>(p, psi);
+  mark();
+  for (int i = 0; i < nsteps; ++i) {
+    if (int rc = t_pass<T>(p, a, i == 0 ? 0 : 5, s)) return rc;
+    mark();
+    for (int k = 1; k <= 3; ++k) {
+      if (int rc = t_pass<T>(p, a, k, s)) return rc;
+      mark();
+    }
+  }
+  if (int rc = t_pass<T>(p, a, 4, s)) return rc;
+  mark();
   return 0;
 }
 
@@ -987,6 +1041,14 @@ static int step_checks(pfc_hip_plan_t plan, void *psi, int nsteps, const char *f
 int pfc_hip_pfc_step(pfc_hip_plan_t plan, void *psi, int nsteps, void *stream) {
   if (int rc = step_checks(plan, psi, nsteps, "pfc_hip_pfc_step")) return rc;
   hipStream_t s = (hipStream_t)stream;
+  if (nsteps >= 2) {
+    const bool f64 = plan->prec == PFC_HIP_F64;
+    if (f64 ? fused_steps<double>(plan) : fused_steps<float>(plan)) {
+      if (int rc = step_buffers(plan)) return rc;
+      return f64 ? steps_fused<double>(plan, psi, nsteps, s, nullptr)
+                 : steps_fused<float>(plan, psi, nsteps, s, nullptr);
+    }
+  }
   for (int i = 0; i < nsteps; ++i) {
     int rc = plan->prec == PFC_HIP_F64 ? step_impl<double>(plan, psi, s, nullptr)
                                        : step_impl<float>(plan, psi, s, nullptr);
@@ -1046,8 +1108,12 @@ int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,
                                     std::to_string(np) + " entries");
   if (nsteps < 1) return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_step_timed: nsteps < 1");
   hipStream_t s = (hipStream_t)stream;
-  const int ne = np + 1;
-  std::vector<hipEvent_t> ev((size_t)ne * nsteps, nullptr);
+  const bool f64 = plan->prec == PFC_HIP_F64;
+  // with a sixth entry and a fused run: the launches of pfc_hip_pfc_step
+  const bool fused = nkernels >= 6 && nsteps >= 2 &&
+                     (f64 ? fused_steps<double>(plan) : fused_steps<float>(plan));
+  const int ne = fused ? 4 * nsteps + 2 : (np + 1) * nsteps;
+  std::vector<hipEvent_t> ev((size_t)ne, nullptr);
   int rc = 0;
   for (auto &e : ev) {
     hipError_t he = hipEventCreate(&e);
@@ -1057,21 +1123,41 @@ int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,
       break;
     }
   }
-  for (int i = 0; i < nsteps && !rc; ++i)
-    rc = plan->prec == PFC_HIP_F64 ? step_impl<double>(plan, psi, s, &ev[(size_t)ne * i])
-                                   : step_impl<float>(plan, psi, s, &ev[(size_t)ne * i]);
+  if (!rc && fused) {
+    rc = step_buffers(plan);
+    if (!rc)
+      rc = f64 ? steps_fused<double>(plan, psi, nsteps, s, ev.data())
+               : steps_fused<float>(plan, psi, nsteps, s, ev.data());
+  } else if (!rc) {
+    const int pe = np + 1;
+    for (int i = 0; i < nsteps && !rc; ++i)
+      rc = f64 ? step_impl<double>(plan, psi, s, &ev[(size_t)pe * i])
+               : step_impl<float>(plan, psi, s, &ev[(size_t)pe * i]);
+  }
   if (!rc) {
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) rc = fail(PFC_HIP_EDEVICE, hipGetErrorString(e));
   }
-  if (!rc) {
+  auto span = [&](size_t i) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+    return (double)ms;
+  };
+  if (!rc && fused) {
+    // launch j of step i is event interval 4*i + j; the exit is 4*nsteps
+    for (int k = 0; k < 6; ++k) kernel_ms[k] = 0;
+    kernel_ms[0] = span(0);
+    for (int i = 0; i < nsteps; ++i) {
+      if (i > 0) kernel_ms[5] += span((size_t)4 * i) / (nsteps - 1);
+      for (int k = 1; k <= 3; ++k) kernel_ms[k] += span((size_t)4 * i + k) / nsteps;
+    }
+    kernel_ms[4] = span((size_t)4 * nsteps);
+  } else if (!rc) {
+    const int pe = np + 1;
     for (int k = 0; k < np; ++k) kernel_ms[k] = 0;
     for (int i = 0; i < nsteps; ++i)
-      for (int k = 0; k < np; ++k) {
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, ev[(size_t)ne * i + k], ev[(size_t)ne * i + k + 1]);
-        kernel_ms[k] += ms / nsteps;
-      }
+      for (int k = 0; k < np; ++k) kernel_ms[k] += span((size_t)pe * i + k) / nsteps;
+    if (nkernels > np) kernel_ms[np] = 0;
   }
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
@@ -1299,6 +1385,8 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
   };
   int slot = 0;
   double layout_ms[2] = {0, 0};
+  double role_best[ROLE_COUNT];
+  for (double &b : role_best) b = 1e300;
   const int layouts = plan->t_ok ? 2 : 1;
   for (int li = 0; li < layouts && !rc; ++li) {
     const int layout = plan->t_ok ? 1 - li : 1; // natural first (slot order)
@@ -1340,7 +1428,47 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
         }
       }
       plan->var[sp.role] = best_v;
+      role_best[sp.role] = best;
       slot += count;
+    }
+    if (!rc && layout == 0 && !plan->tungsten()) {
+      // the fused step boundary of multi-step runs, timed on its own: each
+      // variant starts from the T1 spectrum a full step leaves (the pass
+      // rewrites T1 in place), one warm-up launch, then `reps` timed ones
+      int count = 0;
+      pfc_hip_variant_count(plan, ROLE_T_X_FUSE, &count);
+      double best = 1e300;
+      int best_v = plan->var[ROLE_T_X_FUSE];
+      for (int v = 0; v < count && !rc; ++v) {
+        plan->var[ROLE_T_X_FUSE] = v;
+        double pm[9], fastest = 1e300;
+        rc = run_step(pm);
+        for (int r = 0; r <= reps && !rc; ++r) {
+          PFC_HIP_TRY(hipEventRecord(ev[0], s), "autotune event");
+          rc = plan->prec == PFC_HIP_F64
+                   ? t_pass<double>(plan, pfc_t_args<double>(plan, psi), 5, s)
+                   : t_pass<float>(plan, pfc_t_args<float>(plan, psi), 5, s);
+          if (rc) break;
+          PFC_HIP_TRY(hipEventRecord(ev[1], s), "autotune event");
+          PFC_HIP_TRY(hipEventSynchronize(ev[1]), "autotune sync");
+          float ms = 0;
+          (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+          if (r > 0) fastest = std::min(fastest, (double)ms);
+        }
+        if (times_ms && slot + v < max_times) times_ms[slot + v] = fastest;
+        if (fastest < best) {
+          best = fastest;
+          best_v = v;
+        }
+      }
+      plan->var[ROLE_T_X_FUSE] = best_v;
+      slot += count;
+      // fuse step boundaries when one fused pass beats the two it replaces
+      // by a clear margin (the two timings come from different launches)
+      constexpr double kFuseMargin = 0.95;
+      if (count > 0)
+        plan->var[ROLE_STEP_FUSE] =
+            best < kFuseMargin * (role_best[ROLE_T_X_FWD] + role_best[ROLE_T_X_INV]) ? 1 : 0;
     }
     double fastest = 1e300, pm[9];
     for (int r = 0; r <= reps && !rc; ++r) {

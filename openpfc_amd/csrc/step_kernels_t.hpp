@@ -33,30 +33,6 @@
 
 namespace pfc_amd {
 
-// XCD-aware block order.  A block's G lines are adjacent in the strided
-// dimension, so when G values of cplx<T> fill less than a 128-byte line, the
-// P = 128 / (G * sizeof(cplx<T>)) blocks of consecutive line groups share
-// every line they touch there.  Workgroups go to the 8 XCDs round-robin
-// (physical block b on XCD b mod 8) and each XCD has its own L2, so those P
-// blocks are made to run on one XCD, back to back: physical block
-// 8*(P*q + r) + x executes logical block P*(8*q + x) + r.  The last partial
-// group of 8*P blocks keeps its order; the map is a bijection on [0, n).
-template <typename T, int G> constexpr unsigned line_partners() {
-  constexpr int bytes = G * 2 * (int)sizeof(T);
-  return bytes >= 128 ? 1u : 128u / bytes;
-}
-template <typename T, int G>
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned n) {
-  constexpr unsigned P = line_partners<T, G>(), X = 8, S = X * P;
-  if constexpr (P <= 1) {
-    return b;
-  } else {
-    if (b >= n / S * S) return b;
-    const unsigned x = b % X, r = b / X;
-    return ((r / P) * X + x) * P + (r % P);
-  }
-}
-
 // LDS plane size (elements) of G lines of an N-point plan, row layout.
 template <typename Plan, int G> constexpr int t_plane() {
   return G * row_padded(Plan::N);
@@ -70,25 +46,26 @@ template <int G> __device__ __forceinline__ int tile_idx(int pos, int g) {
 
 // Fetch, for every owned element, the value at line position pp(k) of the
 // same line through LDS; one plane at a time when SPLIT (half the LDS).
-template <typename T, int E, int TL, bool SPLIT, typename PartnerPos>
+template <typename T, int E, int TL, bool SPLIT, bool WAVE = false,
+          typename PartnerPos>
 __device__ __forceinline__ void fetch_partner(const cplx<T> (&v)[E], cplx<T> (&p)[E],
                                               int t, T *re, T *im,
                                               PartnerPos pp) {
   if constexpr (SPLIT) {
 #pragma unroll
     for (int e = 0; e < E; ++e) re[pad_pos<true>(t + TL * e)] = v[e].x;
-    __syncthreads();
+    lds_sync<WAVE>();
 #pragma unroll
     for (int e = 0; e < E; ++e) p[e].x = re[pad_pos<true>(pp(t + TL * e))];
-    __syncthreads();
+    lds_sync<WAVE>();
 #pragma unroll
     for (int e = 0; e < E; ++e) re[pad_pos<true>(t + TL * e)] = v[e].y;
-    __syncthreads();
+    lds_sync<WAVE>();
 #pragma unroll
     for (int e = 0; e < E; ++e) p[e].y = re[pad_pos<true>(pp(t + TL * e))];
-    __syncthreads();
+    lds_sync<WAVE>();
   } else {
-    row_fetch_partner<T, E, TL>(v, p, t, re, im, pp);
+    row_fetch_partner<T, E, TL, WAVE>(v, p, t, re, im, pp);
   }
 }
 
@@ -137,9 +114,9 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd(TStepArgs<T> a) {
   T *re = lds + g * RP;
   T *im = lds + (SPLIT ? 0 : PL) + g * RP;
   const LdsLine<T> ll{lds, g * RP, 1, PL};
-  LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_x);
+  LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_x);
   cplx<T> pz[E];
-  fetch_partner<T, E, TL, SPLIT>(v, pz, t, re, im,
+  fetch_partner<T, E, TL, SPLIT, row_wave<Plan>()>(v, pz, t, re, im,
                                  [](int k) { return k == 0 ? 0 : N - k; });
   // split Z into Psi (v) and N (pz) for k = t + TL*e, e < E/2
 #pragma unroll
@@ -168,6 +145,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd(TStepArgs<T> a) {
     return;
   }
   // transposed store of kx = TL*r + j, j < TL, r < E/2, through a TL x G tile
+  __syncthreads(); // every line's transform is done with the LDS
   T *tr = lds, *ti = lds + TL * (G + 1);
 #pragma unroll
   for (int f = 0; f < 2; ++f) {
@@ -210,7 +188,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd(TStepArgs<T> a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = line[t + TL * e];
   const LdsLine<T> ll{lds, g * RP, 1, PL};
-  LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_y);
+  LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_y);
   if constexpr (!TILE) {
     if (active) {
 #pragma unroll
@@ -219,6 +197,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd(TStepArgs<T> a) {
     return;
   }
   // transposed store: T2[(kx*ny + ky)*nz + z], ky = TL*r + j
+  __syncthreads(); // every line's transform is done with the LDS
   T *tr = lds, *ti = lds + TL * (G + 1);
 #pragma unroll
   for (int r = 0; r < E; ++r) {
@@ -259,7 +238,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
     // A = Fz(A) (kept for the combine); B = Fz^-1(fMF * A)
 #pragma unroll
     for (int e = 0; e < E; ++e) acc[e] = la[t + TL * e];
-    LineFft<T, Plan, -1, SPLIT, true>::run(acc, t, ll, a.tw_z);
+    LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(acc, t, ll, a.tw_z);
     if (active) {
 #pragma unroll
       for (int e = 0; e < E; ++e) la[t + TL * e] = acc[e];
@@ -304,16 +283,16 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
     }
     if constexpr (MODE == ZM_PFC) {
       // A = Fz^-1(opL*Fz(A) + opN*Fz(B))
-      LineFft<T, Plan, -1, SPLIT, true>::run(acc, t, ll, a.tw_z);
+      LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(acc, t, ll, a.tw_z);
 #pragma unroll
       for (int e = 0; e < E; ++e) acc[e] = c_scale(acc[e], opl[e]);
-      LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_z);
+      LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_z);
 #pragma unroll
       for (int e = 0; e < E; ++e) acc[e] = c_add(acc[e], c_scale(v[e], opn[e]));
       if (a.zero_k0 && l == 0 && t == 0) acc[0] = cplx<T>{0, 0};
     } else {
       // A = Fz^-1(expL*A + nW*Fz(B)), A already a full spectrum
-      LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_z);
+      LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_z);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const cplx<T> h = PRE ? acc[e] : la[t + TL * e];
@@ -321,7 +300,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
       }
     }
   }
-  LineFft<T, Plan, +1, SPLIT, true>::run(acc, t, ll, a.tw_z);
+  LineFft<T, Plan, +1, SPLIT, true, row_wave<Plan>()>::run(acc, t, ll, a.tw_z);
   if (active) {
     cplx<T> *__restrict__ out = MODE == ZM_TMF ? lb : la;
 #pragma unroll
@@ -362,7 +341,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv(TStepArgs<T> a) {
     __syncthreads();
   }
   const LdsLine<T> ll{lds, g * RP, 1, PL};
-  LineFft<T, Plan, +1, SPLIT, true>::run(v, t, ll, a.tw_y);
+  LineFft<T, Plan, +1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_y);
   if (z < a.nz) {
     cplx<T> *__restrict__ line = a.A1 + ((long)z * a.W + kx) * N;
 #pragma unroll
@@ -439,7 +418,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
   __syncthreads(); // the staging area becomes the FFT exchange area
   }
   const LdsLine<T> ll{lds, g * RP, 1, PL};
-  LineFft<T, Plan, +1, false, true>::run(v, t, ll, a.tw_xm);
+  LineFft<T, Plan, +1, false, true, row_wave<Plan>()>::run(v, t, ll, a.tw_xm);
   if (y < a.ny) {
     cplx<T> *__restrict__ out =
         reinterpret_cast<cplx<T> *>(a.psi_out + ((long)z * a.ny + y) * (2L * M));
@@ -467,6 +446,7 @@ __device__ __forceinline__ void t1_store_rows(const cplx<T> (&v)[E], cplx<T> xm,
       for (int r = 0; r < E; ++r) out[((long)z * W + TL * r + t) * ny + y] = v[r];
     }
   } else {
+    __syncthreads(); // every line's transform is done with the LDS
     T *tr = lds, *ti = lds + TL * (G + 1);
 #pragma unroll
     for (int r = 0; r < E; ++r) {
@@ -518,9 +498,9 @@ __device__ __forceinline__ void row_r2c(cplx<T> (&v)[Plan::E], cplx<T> &xm, T *l
   constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N, RP = row_padded(M);
   constexpr int PL = t_plane<Plan, G>();
   const LdsLine<T> ll{lds, g * RP, 1, PL};
-  LineFft<T, Plan, -1, false, true>::run(v, t, ll, tw);
+  LineFft<T, Plan, -1, false, true, row_wave<Plan>()>::run(v, t, ll, tw);
   cplx<T> pz[E];
-  row_fetch_partner<T, E, TL>(v, pz, t, lds + g * RP, lds + PL + g * RP,
+  row_fetch_partner<T, E, TL, row_wave<Plan>()>(v, pz, t, lds + g * RP, lds + PL + g * RP,
                               [](int k) { return k == 0 ? 0 : M - k; });
   xm = cplx<T>{v[0].x - v[0].y, (T)0};
 #pragma unroll
@@ -602,7 +582,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_tung(TStepArgs<T> a) {
     __syncthreads(); // the staging area becomes the FFT exchange area
   }
   const LdsLine<T> ll{lds, g * RP, 1, PL};
-  LineFft<T, Plan, +1, false, true>::run(v, t, ll, a.tw_xm);
+  LineFft<T, Plan, +1, false, true, row_wave<Plan>()>::run(v, t, ll, a.tw_xm);
   // v[e] = (psi_MF[2k], psi_MF[2k+1]), k = t + TL*e
   const long ro = ((long)z * a.ny + yy) * (2L * M);
   if (a.mf_out && y < a.ny) {
@@ -656,44 +636,21 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd2(TStepArgs<T> a) {
   t1_store_rows<T, E, TL, G, TILE>(v, xm, a.B1, lds, t, g, z, y0, a.ny, a.W);
 }
 
-// ----------------------------------------------------------------------------
-// k_tx_fwd with a wide transposed store: per field, all W half-spectrum
-// values of the G rows go to one W x G LDS tile, then the whole block stores
-// it (G contiguous y values per kx).  Two block-wide syncs per field instead
-// of two per E/2 rounds.
-// ----------------------------------------------------------------------------
-template <typename T, typename Plan, int G, bool SPLIT> constexpr int tx_fwd_w_lds() {
-  constexpr int stage = 2 * (Plan::N / 2 + 1) * (G + 1);
-  constexpr int fft = (SPLIT ? 1 : 2) * t_plane<Plan, G>();
-  return stage > fft ? stage : fft;
-}
-
+// The forward x of G rows held as v[e] = psi + i*N(psi) at positions
+// t + TL*e: N-point transform, split into the two half spectra, and the wide
+// transposed store of each into T1 (A1: Psi, B1: N) through one W x G tile.
 template <typename T, typename Plan, int G, bool SPLIT>
-__global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w(TStepArgs<T> a) {
+__device__ __forceinline__ void fwd_split_store_wide(cplx<T> (&v)[Plan::E], T *lds, int t, int g,
+                                                     int z, int y0, const TStepArgs<T> &a) {
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
   constexpr int PL = t_plane<Plan, G>();
   constexpr int W = N / 2 + 1;
-  __shared__ T lds[tx_fwd_w_lds<T, Plan, G, SPLIT>()];
-  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
-  const int ngy = (a.ny + G - 1) / G;
-  const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
-  const int z = (int)bid / ngy, y0 = ((int)bid % ngy) * G;
-  const int y = y0 + g;
-  const bool active = y < a.ny;
-  const T *__restrict__ row = a.psi_in + ((long)z * a.ny + (active ? y : y0)) * N;
-
-  cplx<T> v[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const T u = row[t + TL * e];
-    v[e] = cplx<T>{u, a.nl(u)};
-  }
   T *re = lds + g * RP;
   T *im = lds + (SPLIT ? 0 : PL) + g * RP;
   const LdsLine<T> ll{lds, g * RP, 1, PL};
-  LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_x);
+  LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_x);
   cplx<T> pz[E];
-  fetch_partner<T, E, TL, SPLIT>(v, pz, t, re, im,
+  fetch_partner<T, E, TL, SPLIT, row_wave<Plan>()>(v, pz, t, re, im,
                                  [](int k) { return k == 0 ? 0 : N - k; });
 #pragma unroll
   for (int e = 0; e <= E / 2 && e < E; ++e) {
@@ -729,6 +686,39 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w(TStepArgs<T> a) {
 }
 
 // ----------------------------------------------------------------------------
+// k_tx_fwd with a wide transposed store: per field, all W half-spectrum
+// values of the G rows go to one W x G LDS tile, then the whole block stores
+// it (G contiguous y values per kx).  Two block-wide syncs per field instead
+// of two per E/2 rounds.
+// ----------------------------------------------------------------------------
+template <typename T, typename Plan, int G, bool SPLIT> constexpr int tx_fwd_w_lds() {
+  constexpr int stage = 2 * (Plan::N / 2 + 1) * (G + 1);
+  constexpr int fft = (SPLIT ? 1 : 2) * t_plane<Plan, G>();
+  return stage > fft ? stage : fft;
+}
+
+template <typename T, typename Plan, int G, bool SPLIT>
+__global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w(TStepArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N;
+  __shared__ T lds[tx_fwd_w_lds<T, Plan, G, SPLIT>()];
+  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
+  const int ngy = (a.ny + G - 1) / G;
+  const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
+  const int z = (int)bid / ngy, y0 = ((int)bid % ngy) * G;
+  const int y = y0 + g;
+  const bool active = y < a.ny;
+  const T *__restrict__ row = a.psi_in + ((long)z * a.ny + (active ? y : y0)) * N;
+
+  cplx<T> v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const T u = row[t + TL * e];
+    v[e] = cplx<T>{u, a.nl(u)};
+  }
+  fwd_split_store_wide<T, Plan, G, SPLIT>(v, lds, t, g, z, y0, a);
+}
+
+// ----------------------------------------------------------------------------
 // y passes with few-round transpositions: the N positions of the G lines go
 // through the LDS in NR rounds of N/NR rows ([position][line] tile), one
 // block-wide store (forward) or load (inverse) per round instead of one per
@@ -761,7 +751,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd_w(TStepArgs<T> a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = line[t + TL * e];
   const LdsLine<T> ll{lds, g * RP, 1, PL};
-  LineFft<T, Plan, -1, SPLIT, true>::run(v, t, ll, a.tw_y);
+  LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_y);
   // ky = TL*e + t; round h holds e in [h*EPR, (h+1)*EPR), i.e. ky in [h*ROWS, (h+1)*ROWS)
   T *tr = lds, *ti = lds + ROWS * (G + 1);
 #pragma unroll
@@ -816,12 +806,87 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv_w(TStepArgs<T> a) {
     __syncthreads(); // the tile is refilled (or becomes the FFT exchange area)
   }
   const LdsLine<T> ll{lds, g * RP, 1, PL};
-  LineFft<T, Plan, +1, SPLIT, true>::run(v, t, ll, a.tw_y);
+  LineFft<T, Plan, +1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_y);
   if (z < a.nz) {
     cplx<T> *__restrict__ line = a.A1 + ((long)z * a.W + kx) * N;
 #pragma unroll
     for (int e = 0; e < E; ++e) line[t + TL * e] = v[e];
   }
+}
+
+// ----------------------------------------------------------------------------
+// Step boundary of a multi-step PFC run, fused: the inverse x pass of step n
+// (T1 rows of Psi -> psi, nx/2-point plan PH) and the forward x pass of step
+// n+1 (psi -> A1: Psi, B1: N, nx-point plan PF) for G consecutive rows of
+// plane z, psi never leaving the block.  The C2R leaves psi as the pairs
+// (psi[2m], psi[2m+1]) of m = t + TL*e; one LDS exchange hands each thread
+// the positions t + TL*e of the nx-point ownership, then the forward x runs
+// as in k_tx_fwd_w.  PH and PF use the same threads per line.  Reads C and
+// writes 2C: 2R less than the two passes it replaces.  A block reads and then
+// rewrites only its own (z, y-group) tile of A1.
+// ----------------------------------------------------------------------------
+template <typename T, typename PH, typename PF, int G, bool SPLIT> constexpr int tx_fuse_lds() {
+  constexpr int inv = tx_inv_lds<T, PH, G, true>();
+  constexpr int xch = G * row_padded(PF::N);
+  constexpr int fwd = tx_fwd_w_lds<T, PF, G, SPLIT>();
+  return inv > xch ? (inv > fwd ? inv : fwd) : (xch > fwd ? xch : fwd);
+}
+
+template <typename T, typename PH, typename PF, int G, bool SPLIT>
+__global__ void __launch_bounds__(G *PF::TL) k_tx_fuse(TStepArgs<T> a) {
+  static_assert(PH::TL == PF::TL && PF::N == 2 * PH::N, "paired plans");
+  constexpr int TL = PH::TL, EH = PH::E, EF = PF::E, M = PH::N;
+  constexpr int RPH = row_padded(M), RPF = row_padded(PF::N);
+  constexpr int PLH = t_plane<PH, G>();
+  constexpr int W = M + 1;
+  __shared__ T lds[tx_fuse_lds<T, PH, PF, G, SPLIT>()];
+  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
+  const int ngy = (a.ny + G - 1) / G;
+  const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
+  const int z = (int)bid / ngy, y0 = ((int)bid % ngy) * G;
+
+  // inverse x: transposed load of the G rows, C2R merge, inverse transform
+  cplx<T> v[EH];
+  {
+    T *sr = lds, *si = lds + W * (G + 1);
+    for (int i = threadIdx.x; i < W * G; i += G * TL) {
+      const int kx = i / G, gg = i % G;
+      cplx<T> w{0, 0};
+      if (y0 + gg < a.ny) w = a.A1[((long)z * a.W + kx) * a.ny + y0 + gg];
+      sr[tile_idx<G>(kx, gg)] = w.x;
+      si[tile_idx<G>(kx, gg)] = w.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < EH; ++e) {
+      const int k = t + TL * e;
+      v[e] = c2r_merge(cplx<T>{sr[tile_idx<G>(k, g)], si[tile_idx<G>(k, g)]},
+                       cplx<T>{sr[tile_idx<G>(M - k, g)], si[tile_idx<G>(M - k, g)]},
+                       a.wn[k], k == 0);
+    }
+    __syncthreads(); // the staging area becomes the FFT exchange area
+  }
+  const LdsLine<T> llh{lds, g * RPH, 1, PLH};
+  LineFft<T, PH, +1, false, true, row_wave<PH>()>::run(v, t, llh, a.tw_xm);
+
+  // psi to the nx-point ownership, then psi + i*N(psi)
+  __syncthreads();
+  T *row = lds + g * RPF;
+#pragma unroll
+  for (int e = 0; e < EH; ++e) {
+    const int m = t + TL * e;
+    row[pad_pos<true>(2 * m)] = v[e].x;
+    row[pad_pos<true>(2 * m + 1)] = v[e].y;
+  }
+  __syncthreads();
+  cplx<T> w[EF];
+#pragma unroll
+  for (int e = 0; e < EF; ++e) {
+    const T u = row[pad_pos<true>(t + TL * e)];
+    w[e] = cplx<T>{u, a.nl(u)};
+  }
+  __syncthreads(); // the LDS becomes the forward transform's exchange area
+  fwd_split_store_wide<T, PF, G, SPLIT>(w, lds, t, g, z, y0, a);
 }
 
 } // namespace pfc_amd

@@ -31,13 +31,15 @@ enum Role {
   ROLE_X_TUNG = 12,     // Tungsten natural step: fused inverse x, N, forward x
   ROLE_T_X_R2C = 13,    // Tungsten transposed step: psi -> T1 (psi only)
   ROLE_T_X_TUNG = 14,   // Tungsten transposed step: fused x pass on T1
-  ROLE_COUNT = 15
+  ROLE_T_X_FUSE = 15,   // PFC multi-step run: inverse x of one step fused with the next forward x
+  ROLE_STEP_FUSE = 16,  // 0: multi-step runs as single steps, 1: fused step boundaries
+  ROLE_COUNT = 17
 };
 
 // Kinds of transposed-step kernels (one variant list per kind and length).
 enum TKind {
   TK_X_FWD = 0, TK_Y_FWD = 1, TK_Z = 2, TK_Y_INV = 3, TK_X_INV = 4,
-  TK_X_R2C = 5, TK_X_TUNG = 6, TK_COUNT = 7
+  TK_X_R2C = 5, TK_X_TUNG = 6, TK_X_FUSE = 7, TK_COUNT = 8
 };
 
 template <typename T> struct ColV {
@@ -73,6 +75,7 @@ template <typename T> struct TV {
   int plan_id;
   hipError_t (*fn)(TStepArgs<T> a, hipStream_t s);
   int half = 0; // 1: the kernel runs the n/2-point plan `plan_id` of a length-n role
+  int plan_id2 = -1; // >= 0: also the 2n-point plan plan_id2 (its twiddles in tw_x)
 };
 
 // Variant lists for a line length n (empty when n is unsupported).

@@ -406,6 +406,13 @@ hipError_t tx_tung_wide_launch(TStepArgs<T> a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename T, typename PH, typename PF, int G, bool S>
+hipError_t tx_fuse_launch(TStepArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * PF::TL);
+  hipLaunchKernelGGL((k_tx_fuse<T, PH, PF, G, S>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
 template <typename T, typename Plan, int G, bool S, int NR>
 hipError_t ty_fwd_w_launch(TStepArgs<T> a, hipStream_t s) {
   if (a.nfields != 1 && a.nfields != 2) return hipErrorInvalidValue;
@@ -515,6 +522,19 @@ void add_t_z_pre(std::vector<TV<T>> &v, int pid) {
                      "/pre",
                  pid, &tz_pre_launch<T, Plan, G, S>});
 }
+// Fused step boundary (k_tx_fuse): an nx/2-point plan PH paired with an
+// nx-point plan PF of the same threads per line
+template <typename T, typename PH, typename PF, int G, bool S>
+void add_t_fuse(std::vector<TV<T>> &v, int pid, int pid2) {
+  if constexpr (PH::TL == PF::TL && G * PF::TL <= kMaxThreads && G * PF::TL >= 64 &&
+                tx_fuse_lds<T, PH, PF, G, S>() * (int)sizeof(T) <= kMaxLds) {
+    TV<T> e{plan_name<PH>() + "+" + plan_name<PF>() + "/G" + std::to_string(G) +
+                (S ? "/split" : "/full"),
+            pid, &tx_fuse_launch<T, PH, PF, G, S>};
+    e.plan_id2 = pid2;
+    v.push_back(e);
+  }
+}
 // Tungsten row kernels with the wide transposed store
 template <typename T, typename Plan, int G, int KIND>
 void add_t_rows_wide(std::vector<TV<T>> &v, int pid) {
@@ -579,6 +599,22 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
         add_t_z_pre<T, P, 1, false>(v, pid);
         add_t_z_pre<T, P, 2, true>(v, pid);
       });
+    }
+    if constexpr (KIND == TK_X_FUSE) {
+      // N is nx/2 here; the forward half of the pass runs the 2N-point plans
+      if constexpr (2 * N <= 1024) {
+        for_t_plans<T, N>(typename Plans<N>::L{}, [&](auto ph, int pid) {
+          using PH = decltype(ph);
+          for_t_plans<T, 2 * N>(typename Plans<2 * N>::L{}, [&](auto pf, int pid2) {
+            using PF = decltype(pf);
+            constexpr int G0 = 128 / (2 * (int)sizeof(T));
+            add_t_fuse<T, PH, PF, G0 / 2, true>(v, pid, pid2);
+            add_t_fuse<T, PH, PF, G0 / 2, false>(v, pid, pid2);
+            add_t_fuse<T, PH, PF, G0, true>(v, pid, pid2);
+            add_t_fuse<T, PH, PF, G0, false>(v, pid, pid2);
+          });
+        });
+      }
     }
     if constexpr (KIND == TK_X_R2C || KIND == TK_X_TUNG) {
       for_t_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
@@ -655,7 +691,7 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
         make_t_kind<T, N_, 0>(), make_t_kind<T, N_, 1>(),                      \
         make_t_kind<T, N_, 2>(), make_t_kind<T, N_, 3>(),                      \
         make_t_kind<T, N_, 4>(), make_t_kind<T, N_, 5>(),                      \
-        make_t_kind<T, N_, 6>()};                                              \
+        make_t_kind<T, N_, 6>(), make_t_kind<T, N_, 7>()};                     \
                           return v[kind];)                                     \
     return empty;                                                              \
   }

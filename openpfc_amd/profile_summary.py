@@ -45,14 +45,14 @@ def short_name(name: str) -> str:
         return name.split("(")[0][:80]
     args = m.group(2)
     t = "double" if args.startswith("double") else ("float" if args.startswith("float") else "")
-    radices = re.search(r"RadixList<([\d, ]+)>", args)
-    npl = re.search(r"FftPlan<(\d+), (\d+)", args)
+    # one or two (paired) line plans: "N512/E8,r8x8x8[+N1024/E16,r16x16x4]"
+    plans = [f"N{n}/E{e},r" + r.replace(", ", "x")
+             for n, e, r in re.findall(r"FftPlan<(\d+), (\d+), pfc_amd::RadixList<([\d, ]+)> >",
+                                       args)]
     rest = re.sub(r".*RadixList<[\d, ]+> >,?\s*", "", args)
     desc = []
-    if npl:
-        desc.append(f"N{npl.group(1)}/E{npl.group(2)}")
-    if radices:
-        desc.append("r" + radices.group(1).replace(", ", "x"))
+    if plans:
+        desc.append("+".join(plans))
     if rest:
         desc.append(rest.replace(" ", ""))
     return f"{m.group(1)}<{t}{',' if desc else ''}{','.join(desc)}>"

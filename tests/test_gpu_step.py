@@ -150,3 +150,35 @@ def test_fp64_mean_split_matches_plain(torch_gpu):
     st.step(d, 20)
     got = mean + d.cpu().numpy()
     assert _rel_max(got, ref) <= 1e-10
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6)])
+def test_fused_multi_step_matches_single_steps(torch_gpu, prec, tol):
+    """A K-step call fuses every step boundary (inverse x of one step with the
+    forward x of the next, psi kept in registers); K single-step calls do
+    not.  The two agree to rounding, and the fused run's launch timing
+    reports the boundary pass."""
+    import openpfc_amd as P
+    torch = torch_gpu
+    n = (256, 128, 128)
+    psi0 = _psi0(n)
+    f = P.HipFft(*n, precision=P.F64 if prec == "f64" else P.F32)
+    assert f.variants("step_layout")[f.get_variant("step_layout")] == "transposed"
+    assert len(f.variants("t_x_fuse")) > 0
+    f.set_variant("step_fuse", 1)
+    st = P.PfcStepper(f, (DX, DX, DX), EPS)
+    st.initialize(DT)
+    rdt = torch.float64 if prec == "f64" else torch.float32
+    a = torch.from_numpy(psi0).to(rdt).cuda()
+    b = a.clone()
+    st.step(a, 12)
+    for _ in range(12):
+        st.step(b, 1)
+    err = float(((a - b).abs().max() / b.abs().max()).item())
+    print(f"fused vs single steps ({prec}): rel max {err:.2e}")
+    assert err <= tol
+    opL, opN = O.pfc_operators(n, (DX, DX, DX), EPS, DT)
+    ref = O.pfc_step(psi0, opL, opN, 12)
+    assert _rel_max(a.double().cpu().numpy(), ref) <= (1e-10 if prec == "f64" else 1e-5)
+    t = st.step_timed_fused(a, 4)
+    assert len(t) == 6 and all(v > 0 for v in t), t

@@ -83,18 +83,24 @@ def test_all_transposed_step_variants(torch_gpu, n, prec):
     st.initialize(DT)
     rdt = torch.float64 if prec == "f64" else torch.float32
     tol = 1e-12 if prec == "f64" else 2e-5
-    cases = [("step_layout", 1)] + [(r, i) for r in ["t_x_fwd", "t_y_fwd", "t_z", "t_y_inv",
-                                                     "t_x_inv"]
-                                    for i in range(len(f.variants(r)))]
-    for role, i in cases:
+    # per-step runs, then fused step boundaries (st.step(psi, 2) runs the
+    # fused pass, role t_x_fuse, between the two steps) with every variant
+    assert f.variants("step_fuse") == ["per_step", "fused"]
+    cases = [("step_layout", 1, 0)] + [(r, i, fu) for fu in (0, 1)
+                                       for r in ["t_x_fwd", "t_y_fwd", "t_z", "t_y_inv",
+                                                 "t_x_inv"] + (["t_x_fuse"] if fu else [])
+                                       for i in range(len(f.variants(r)))]
+    for role, i, fu in cases:
         name = f.variants(role)[i]
-        print(f"{n} {prec} {role}[{i}] {name}", flush=True)
+        print(f"{n} {prec} fuse={fu} {role}[{i}] {name}", flush=True)
+        f.set_variant("step_fuse", fu)
         f.set_variant(role, i)
         psi = torch.from_numpy(psi0).to(rdt).cuda()
         st.step(psi, 2)
         err = np.abs(psi.double().cpu().numpy() - ref).max() / np.abs(ref).max()
-        assert err <= tol, f"{role} {name}: {err:.2e}"
+        assert err <= tol, f"{role} {name} fuse={fu}: {err:.2e}"
         f.set_variant(role, 0)
+    f.set_variant("step_fuse", 0)
 
 
 def test_operators_match_oracle(torch_gpu):
