@@ -394,6 +394,8 @@ __global__ void k_pfc_operators(PfcOp o, int W, T *opL, T *opN) {
 //   opCk = ((stabP + p2_bar) - Bx*exp(-T/T0)*gf) + q2_bar*fMF
 //   L = kLap*opCk, expL = exp(L*dt),
 //   nW = kLap * (|L| < 1e-12 ? dt + 0.5*L*dt*dt : expm1(L*dt)/L)
+// (for L*dt < -1, where most modes are, expm1(L*dt) is taken as expL - 1,
+// within 2 ulp of expm1 there, so one exponential serves both multipliers)
 // The mode-independent scalars rTol = -alpha2*log(farTol) - 1,
 // c_peak = Bx*exp(-T/T0) and sp2 = stabP + p2_bar are formed on the host in
 // the same order.  Step multipliers are scaled by PfcOp::scale.
@@ -433,6 +435,10 @@ __device__ __forceinline__ void tung_op_mode(const PfcOp &o, const TungOp &p,
   double phi1;
   if (fabs(L) < 1e-12)
     phi1 = o.dt + (((0.5 * L) * o.dt) * o.dt);
+  else if (arg < -1.0)
+    // expm1(arg) from the exp above: |exp(arg) - 1| > 0.63 here, so the
+    // subtraction adds at most one rounding (within 2 ulp of expm1)
+    phi1 = (expL - 1.0) / L;
   else
     phi1 = expm1(arg) / L;
   nW = kLap * phi1;
