@@ -70,6 +70,17 @@ public:
   }
   void device_field_modified(const std::string &) override { m_host_stale = true; }
 
+  // Free energy F = dV sum [psi (eps + (1 + lap)^2) psi / 2 + psi^4 / 4] of
+  // the current field, summed in double on the device (the reference model
+  // has no such method; the functional is the one its step descends).
+  double free_energy() {
+    double F = 0;
+    pfc::detail::check_abi(pfc_hip_pfc_free_energy(m_hfft.plan(), m_psi.data(), &F,
+                                                   m_hfft.stream()),
+                           "PFCModelHIP::free_energy");
+    return F;
+  }
+
   double get_epsilon() const { return m_epsilon; }
   core::DataBuffer<backend::HipTag, double> &device_density() { return m_psi; }
 

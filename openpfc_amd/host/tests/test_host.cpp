@@ -5,7 +5,8 @@
 //   test_host gpu-fft                  HipFft overloads, known answers, errors
 //   test_host gpu-sim N STEPS OUT.bin  PFCModelHIP driven by Simulator from
 //                                      Constant(-0.2) + UniformNoise(1234, +-0.01);
-//                                      writes the final field (raw f64)
+//                                      writes the final field (raw f64) and
+//                                      prints its free energy
 //   test_host gpu-tung STEPS OUT MFOUT TungstenHIP<double> on the reference's
 //                                      CPU-vs-HIP recipe (32^3, spacing 1,
 //                                      T = 0.5, dt = 0.01, -0.4 + 0.1 sin(2 pi i/n));
@@ -307,6 +308,7 @@ static int run_gpu_sim(int n, int steps, const char *out) {
   std::ofstream o(out, std::ios::binary);
   o.write(reinterpret_cast<const char *>(psi.data()), (std::streamsize)(psi.size() * sizeof(double)));
   std::printf("steps %d increment %d\n", steps, time.get_increment());
+  std::printf("free_energy %.17g\n", model.free_energy());
   CHECK(time.get_increment() == steps);
   return g_fail;
 }

@@ -52,6 +52,9 @@ def test_host_pfc_model_simulator_matches_oracle(torch_gpu, tmp_path, n, steps):
     ref = O.pfc_step(psi0, opL, opN, steps)
     err = np.abs(got - ref).max() / np.abs(ref).max()
     assert err <= 1e-10, err
+    F = float(next(ln for ln in r.stdout.splitlines() if ln.startswith("free_energy")).split()[1])
+    Fref = O.pfc_free_energy(ref, (dx, dx, dx), -0.25)
+    assert abs(F - Fref) <= 1e-10 * abs(Fref), (F, Fref)
 
 
 @pytest.mark.gpu
