@@ -87,6 +87,10 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
         return (f"k_tx_fuse<{t},N{n // 2}/E{ea[1:]},r{ra[1:]}+N{n}/E{pb[1][1:]},r{pb[0][1:]},"
                 f"{pb[2][1:]},{'true' if pb[3] == 'split' else 'false'}>")
     parts = variant.split("/")
+    if "nt" in parts:  # non-temporal form: the same kernel name with "_nt"
+        key = variant_key(kernel, prec, n, "/".join(p for p in parts if p != "nt"),
+                          direction, mode)
+        return key.replace("<", "_nt<", 1)
     if kernel == "k_tx_fwd" and "r2c2" in parts:
         kernel, n = "k_tx_fwd2", n // 2
     if kernel == "k_tx_fwd" and "wide" in parts:

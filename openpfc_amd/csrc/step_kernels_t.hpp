@@ -215,9 +215,35 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd(TStepArgs<T> a) {
 
 // ----------------------------------------------------------------------------
 // T2 z pass, in place: G consecutive z lines (kx*ny + ky) per block.
+//
+// The *_nt kernels here mark their loads and stores of lines non-temporal
+// (ld_line / st_line with NT): every such value is read or written once per
+// pass, and the hint keeps the stream from displacing the twiddles and the
+// lines other blocks still share.
 // ----------------------------------------------------------------------------
-template <typename T, typename Plan, int G, bool SPLIT, int MODE, bool PRE = false>
-__global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
+template <bool NT, typename T> __device__ __forceinline__ cplx<T> ld_line(const cplx<T> *p) {
+  if constexpr (NT)
+    return cplx<T>{__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y)};
+  else
+    return *p;
+}
+template <bool NT, typename T> __device__ __forceinline__ T ld_val(const T *p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+template <bool NT, typename T> __device__ __forceinline__ void st_line(cplx<T> *p, cplx<T> v) {
+  if constexpr (NT) {
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+  } else {
+    *p = v;
+  }
+}
+
+template <typename T, typename Plan, int G, bool SPLIT, int MODE, bool PRE, bool NT>
+__device__ __forceinline__ void tz_body(TStepArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
   constexpr int PL = t_plane<Plan, G>();
   __shared__ T lds[(SPLIT ? 1 : 2) * PL];
@@ -237,11 +263,11 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
   if constexpr (MODE == ZM_TMF) {
     // A = Fz(A) (kept for the combine); B = Fz^-1(fMF * A)
 #pragma unroll
-    for (int e = 0; e < E; ++e) acc[e] = la[t + TL * e];
+    for (int e = 0; e < E; ++e) acc[e] = ld_line<NT>(la + t + TL * e);
     LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(acc, t, ll, a.tw_z);
     if (active) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) la[t + TL * e] = acc[e];
+      for (int e = 0; e < E; ++e) st_line<NT>(la + t + TL * e, acc[e]);
     }
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -269,16 +295,16 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
     if constexpr (PRE) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        acc[e] = la[t + TL * e];
-        v[e] = lb[t + TL * e];
+        acc[e] = ld_line<NT>(la + t + TL * e);
+        v[e] = ld_line<NT>(lb + t + TL * e);
       }
       multipliers();
     } else {
       multipliers();
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        if constexpr (MODE == ZM_PFC) acc[e] = la[t + TL * e];
-        v[e] = lb[t + TL * e];
+        if constexpr (MODE == ZM_PFC) acc[e] = ld_line<NT>(la + t + TL * e);
+        v[e] = ld_line<NT>(lb + t + TL * e);
       }
     }
     if constexpr (MODE == ZM_PFC) {
@@ -295,7 +321,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
       LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_z);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const cplx<T> h = PRE ? acc[e] : la[t + TL * e];
+        const cplx<T> h = PRE ? acc[e] : ld_line<NT>(la + t + TL * e);
         acc[e] = c_add(c_scale(h, opl[e]), c_scale(v[e], opn[e]));
       }
     }
@@ -304,8 +330,17 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
   if (active) {
     cplx<T> *__restrict__ out = MODE == ZM_TMF ? lb : la;
 #pragma unroll
-    for (int e = 0; e < E; ++e) out[t + TL * e] = acc[e];
+    for (int e = 0; e < E; ++e) st_line<NT>(out + t + TL * e, acc[e]);
   }
+}
+
+template <typename T, typename Plan, int G, bool SPLIT, int MODE, bool PRE = false>
+__global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
+  tz_body<T, Plan, G, SPLIT, MODE, PRE, false>(a);
+}
+template <typename T, typename Plan, int G, bool SPLIT, int MODE, bool PRE>
+__global__ void __launch_bounds__(G *Plan::TL) k_tz_nt(TStepArgs<T> a) {
+  tz_body<T, Plan, G, SPLIT, MODE, PRE, true>(a);
 }
 
 // ----------------------------------------------------------------------------
@@ -361,8 +396,8 @@ template <typename T, typename Plan, int G, bool TILE> constexpr int tx_inv_lds(
   return stage > fft ? stage : fft;
 }
 
-template <typename T, typename Plan, int G, bool TILE>
-__global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
+template <typename T, typename Plan, int G, bool TILE, bool NT>
+__device__ __forceinline__ void tx_inv_body(TStepArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N, RP = row_padded(M);
   constexpr int PL = t_plane<Plan, G>();
   constexpr int W = M + 1;
@@ -397,7 +432,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
   for (int i = threadIdx.x; i < W * G; i += G * TL) {
     const int kx = i / G, gg = i % G;
     cplx<T> w{0, 0};
-    if (y0 + gg < a.ny) w = a.A1[((long)z * a.W + kx) * a.ny + y0 + gg];
+    if (y0 + gg < a.ny) w = ld_line<NT>(a.A1 + ((long)z * a.W + kx) * a.ny + y0 + gg);
     sr[tile_idx<G>(kx, gg)] = w.x;
     si[tile_idx<G>(kx, gg)] = w.y;
   }
@@ -423,8 +458,17 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
     cplx<T> *__restrict__ out =
         reinterpret_cast<cplx<T> *>(a.psi_out + ((long)z * a.ny + y) * (2L * M));
 #pragma unroll
-    for (int e = 0; e < E; ++e) out[t + TL * e] = v[e];
+    for (int e = 0; e < E; ++e) st_line<NT>(out + t + TL * e, v[e]);
   }
+}
+
+template <typename T, typename Plan, int G, bool TILE>
+__global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
+  tx_inv_body<T, Plan, G, TILE, false>(a);
+}
+template <typename T, typename Plan, int G, bool TILE>
+__global__ void __launch_bounds__(G *Plan::TL) k_tx_inv_nt(TStepArgs<T> a) {
+  tx_inv_body<T, Plan, G, TILE, true>(a);
 }
 
 // ----------------------------------------------------------------------------
@@ -639,7 +683,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd2(TStepArgs<T> a) {
 // The forward x of G rows held as v[e] = psi + i*N(psi) at positions
 // t + TL*e: N-point transform, split into the two half spectra, and the wide
 // transposed store of each into T1 (A1: Psi, B1: N) through one W x G tile.
-template <typename T, typename Plan, int G, bool SPLIT>
+template <typename T, typename Plan, int G, bool SPLIT, bool NT = false>
 __device__ __forceinline__ void fwd_split_store_wide(cplx<T> (&v)[Plan::E], T *lds, int t, int g,
                                                      int z, int y0, const TStepArgs<T> &a) {
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
@@ -679,8 +723,8 @@ __device__ __forceinline__ void fwd_split_store_wide(cplx<T> (&v)[Plan::E], T *l
     for (int i = threadIdx.x; i < W * G; i += G * TL) {
       const int kx = i / G, gg = i % G;
       if (y0 + gg < a.ny)
-        out[((long)z * a.W + kx) * a.ny + y0 + gg] =
-            cplx<T>{sr[tile_idx<G>(kx, gg)], si[tile_idx<G>(kx, gg)]};
+        st_line<NT>(out + ((long)z * a.W + kx) * a.ny + y0 + gg,
+                    cplx<T>{sr[tile_idx<G>(kx, gg)], si[tile_idx<G>(kx, gg)]});
     }
   }
 }
@@ -697,8 +741,8 @@ template <typename T, typename Plan, int G, bool SPLIT> constexpr int tx_fwd_w_l
   return stage > fft ? stage : fft;
 }
 
-template <typename T, typename Plan, int G, bool SPLIT>
-__global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w(TStepArgs<T> a) {
+template <typename T, typename Plan, int G, bool SPLIT, bool NT>
+__device__ __forceinline__ void tx_fwd_w_body(TStepArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N;
   __shared__ T lds[tx_fwd_w_lds<T, Plan, G, SPLIT>()];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
@@ -712,10 +756,19 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w(TStepArgs<T> a) {
   cplx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const T u = row[t + TL * e];
+    const T u = ld_val<NT>(row + t + TL * e);
     v[e] = cplx<T>{u, a.nl(u)};
   }
-  fwd_split_store_wide<T, Plan, G, SPLIT>(v, lds, t, g, z, y0, a);
+  fwd_split_store_wide<T, Plan, G, SPLIT, NT>(v, lds, t, g, z, y0, a);
+}
+
+template <typename T, typename Plan, int G, bool SPLIT>
+__global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w(TStepArgs<T> a) {
+  tx_fwd_w_body<T, Plan, G, SPLIT, false>(a);
+}
+template <typename T, typename Plan, int G, bool SPLIT>
+__global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w_nt(TStepArgs<T> a) {
+  tx_fwd_w_body<T, Plan, G, SPLIT, true>(a);
 }
 
 // ----------------------------------------------------------------------------
@@ -730,8 +783,8 @@ template <typename T, typename Plan, int G, bool SPLIT, int NR> constexpr int ty
   return stage > fft ? stage : fft;
 }
 
-template <typename T, typename Plan, int G, bool SPLIT, int NR>
-__global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd_w(TStepArgs<T> a) {
+template <typename T, typename Plan, int G, bool SPLIT, int NR, bool NT>
+__device__ __forceinline__ void ty_fwd_w_body(TStepArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
   constexpr int PL = t_plane<Plan, G>();
   constexpr int ROWS = N / NR, EPR = E / NR; // tile rows, values per thread per round
@@ -749,7 +802,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd_w(TStepArgs<T> a) {
 
   cplx<T> v[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = line[t + TL * e];
+  for (int e = 0; e < E; ++e) v[e] = ld_line<NT>(line + t + TL * e);
   const LdsLine<T> ll{lds, g * RP, 1, PL};
   LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_y);
   // ky = TL*e + t; round h holds e in [h*EPR, (h+1)*EPR), i.e. ky in [h*ROWS, (h+1)*ROWS)
@@ -767,14 +820,23 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd_w(TStepArgs<T> a) {
     for (int i = threadIdx.x; i < ROWS * G; i += G * TL) {
       const int row = i / G, gg = i % G;
       if (z0 + gg < a.nz)
-        out[((long)kx * N + h * ROWS + row) * a.nz + z0 + gg] =
-            cplx<T>{tr[tile_idx<G>(row, gg)], ti[tile_idx<G>(row, gg)]};
+        st_line<NT>(out + ((long)kx * N + h * ROWS + row) * a.nz + z0 + gg,
+                    cplx<T>{tr[tile_idx<G>(row, gg)], ti[tile_idx<G>(row, gg)]});
     }
   }
 }
 
 template <typename T, typename Plan, int G, bool SPLIT, int NR>
-__global__ void __launch_bounds__(G *Plan::TL) k_ty_inv_w(TStepArgs<T> a) {
+__global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd_w(TStepArgs<T> a) {
+  ty_fwd_w_body<T, Plan, G, SPLIT, NR, false>(a);
+}
+template <typename T, typename Plan, int G, bool SPLIT, int NR>
+__global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd_w_nt(TStepArgs<T> a) {
+  ty_fwd_w_body<T, Plan, G, SPLIT, NR, true>(a);
+}
+
+template <typename T, typename Plan, int G, bool SPLIT, int NR, bool NT>
+__device__ __forceinline__ void ty_inv_w_body(TStepArgs<T> a) {
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
   constexpr int PL = t_plane<Plan, G>();
   constexpr int ROWS = N / NR, EPR = E / NR;
@@ -793,7 +855,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv_w(TStepArgs<T> a) {
     for (int i = threadIdx.x; i < ROWS * G; i += G * TL) {
       const int row = i / G, gg = i % G;
       cplx<T> w{0, 0};
-      if (z0 + gg < a.nz) w = a.A2[((long)kx * N + h * ROWS + row) * a.nz + z0 + gg];
+      if (z0 + gg < a.nz) w = ld_line<NT>(a.A2 + ((long)kx * N + h * ROWS + row) * a.nz + z0 + gg);
       tr[tile_idx<G>(row, gg)] = w.x;
       ti[tile_idx<G>(row, gg)] = w.y;
     }
@@ -810,8 +872,17 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv_w(TStepArgs<T> a) {
   if (z < a.nz) {
     cplx<T> *__restrict__ line = a.A1 + ((long)z * a.W + kx) * N;
 #pragma unroll
-    for (int e = 0; e < E; ++e) line[t + TL * e] = v[e];
+    for (int e = 0; e < E; ++e) st_line<NT>(line + t + TL * e, v[e]);
   }
+}
+
+template <typename T, typename Plan, int G, bool SPLIT, int NR>
+__global__ void __launch_bounds__(G *Plan::TL) k_ty_inv_w(TStepArgs<T> a) {
+  ty_inv_w_body<T, Plan, G, SPLIT, NR, false>(a);
+}
+template <typename T, typename Plan, int G, bool SPLIT, int NR>
+__global__ void __launch_bounds__(G *Plan::TL) k_ty_inv_w_nt(TStepArgs<T> a) {
+  ty_inv_w_body<T, Plan, G, SPLIT, NR, true>(a);
 }
 
 // ----------------------------------------------------------------------------

@@ -340,10 +340,13 @@ hipError_t ty_inv_launch(TStepArgs<T> a, hipStream_t s) {
   hipLaunchKernelGGL((k_ty_inv<T, Plan, G, S, TILE>), grid, block, 0, s, a);
   return hipGetLastError();
 }
-template <typename T, typename Plan, int G, bool S, bool TILE>
+template <typename T, typename Plan, int G, bool S, bool TILE, bool NT = false>
 hipError_t tx_inv_launch(TStepArgs<T> a, hipStream_t s) {
   dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
-  hipLaunchKernelGGL((k_tx_inv<T, Plan, G, TILE>), grid, block, 0, s, a);
+  if constexpr (NT)
+    hipLaunchKernelGGL((k_tx_inv_nt<T, Plan, G, TILE>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_tx_inv<T, Plan, G, TILE>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 
@@ -367,10 +370,13 @@ hipError_t tx_fwd2_launch(TStepArgs<T> a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename T, typename Plan, int G, bool S>
+template <typename T, typename Plan, int G, bool S, bool NT = false>
 hipError_t tx_fwd_w_launch(TStepArgs<T> a, hipStream_t s) {
   dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
-  hipLaunchKernelGGL((k_tx_fwd_w<T, Plan, G, S>), grid, block, 0, s, a);
+  if constexpr (NT)
+    hipLaunchKernelGGL((k_tx_fwd_w_nt<T, Plan, G, S>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_tx_fwd_w<T, Plan, G, S>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 
@@ -387,6 +393,26 @@ hipError_t tz_pre_launch(TStepArgs<T> a, hipStream_t s) {
     break;
   case ZM_TCOMB:
     hipLaunchKernelGGL((k_tz<T, Plan, G, S, ZM_TCOMB, true>), grid, block, 0, s, a);
+    break;
+  default:
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+// z pass with non-temporal line loads and stores (k_tz_nt)
+template <typename T, typename Plan, int G, bool S, bool PRE>
+hipError_t tz_nt_launch(TStepArgs<T> a, hipStream_t s) {
+  const long lines = (long)a.W * a.ny;
+  dim3 grid((unsigned)((lines + G - 1) / G)), block(G * Plan::TL);
+  switch (a.mode) {
+  case ZM_PFC:
+    hipLaunchKernelGGL((k_tz_nt<T, Plan, G, S, ZM_PFC, PRE>), grid, block, 0, s, a);
+    break;
+  case ZM_TMF:
+    hipLaunchKernelGGL((k_tz_nt<T, Plan, G, S, ZM_TMF, PRE>), grid, block, 0, s, a);
+    break;
+  case ZM_TCOMB:
+    hipLaunchKernelGGL((k_tz_nt<T, Plan, G, S, ZM_TCOMB, PRE>), grid, block, 0, s, a);
     break;
   default:
     return hipErrorInvalidValue;
@@ -413,17 +439,23 @@ hipError_t tx_fuse_launch(TStepArgs<T> a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename T, typename Plan, int G, bool S, int NR>
+template <typename T, typename Plan, int G, bool S, int NR, bool NT = false>
 hipError_t ty_fwd_w_launch(TStepArgs<T> a, hipStream_t s) {
   if (a.nfields != 1 && a.nfields != 2) return hipErrorInvalidValue;
   dim3 grid((unsigned)(a.W * ((a.nz + G - 1) / G)), (unsigned)a.nfields), block(G * Plan::TL);
-  hipLaunchKernelGGL((k_ty_fwd_w<T, Plan, G, S, NR>), grid, block, 0, s, a);
+  if constexpr (NT)
+    hipLaunchKernelGGL((k_ty_fwd_w_nt<T, Plan, G, S, NR>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_ty_fwd_w<T, Plan, G, S, NR>), grid, block, 0, s, a);
   return hipGetLastError();
 }
-template <typename T, typename Plan, int G, bool S, int NR>
+template <typename T, typename Plan, int G, bool S, int NR, bool NT = false>
 hipError_t ty_inv_w_launch(TStepArgs<T> a, hipStream_t s) {
   dim3 grid((unsigned)(a.W * ((a.nz + G - 1) / G))), block(G * Plan::TL);
-  hipLaunchKernelGGL((k_ty_inv_w<T, Plan, G, S, NR>), grid, block, 0, s, a);
+  if constexpr (NT)
+    hipLaunchKernelGGL((k_ty_inv_w_nt<T, Plan, G, S, NR>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_ty_inv_w<T, Plan, G, S, NR>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 
@@ -493,25 +525,36 @@ void add_t_fwd2(std::vector<TV<T>> &v, int pid) {
 }
 
 // PFC forward x with the wide transposed store (k_tx_fwd_w).
-template <typename T, typename Plan, int G, bool S>
+// NT: non-temporal loads and stores of the lines (k_*_nt kernels).
+template <typename T, typename Plan, int G, bool S, bool NT = false>
 void add_t_fwd_w(std::vector<TV<T>> &v, int pid) {
   if constexpr (G >= 1 && G * Plan::TL <= kMaxThreads && G * Plan::TL >= 64 &&
                 tx_fwd_w_lds<T, Plan, G, S>() * (int)sizeof(T) <= kMaxLds)
     v.push_back({plan_name<Plan>() + "/G" + std::to_string(G) + (S ? "/split" : "/full") +
-                     "/wide",
-                 pid, &tx_fwd_w_launch<T, Plan, G, S>});
+                     "/wide" + (NT ? "/nt" : ""),
+                 pid, &tx_fwd_w_launch<T, Plan, G, S, NT>});
 }
 
 // y passes with NR-round transpositions (k_ty_fwd_w / k_ty_inv_w).
-template <typename T, typename Plan, int G, bool S, int NR, int KIND>
+template <typename T, typename Plan, int G, bool S, int NR, int KIND, bool NT = false>
 void add_t_y_w(std::vector<TV<T>> &v, int pid) {
   if constexpr (Plan::E % NR == 0 && G * Plan::TL <= kMaxThreads && G * Plan::TL >= 64 &&
                 ty_w_lds<T, Plan, G, S, NR>() * (int)sizeof(T) <= kMaxLds) {
     const std::string nm = plan_name<Plan>() + "/G" + std::to_string(G) +
-                           (S ? "/split" : "/full") + "/wide" + std::to_string(NR);
-    if constexpr (KIND == TK_Y_FWD) v.push_back({nm, pid, &ty_fwd_w_launch<T, Plan, G, S, NR>});
-    if constexpr (KIND == TK_Y_INV) v.push_back({nm, pid, &ty_inv_w_launch<T, Plan, G, S, NR>});
+                           (S ? "/split" : "/full") + "/wide" + std::to_string(NR) +
+                           (NT ? "/nt" : "");
+    if constexpr (KIND == TK_Y_FWD)
+      v.push_back({nm, pid, &ty_fwd_w_launch<T, Plan, G, S, NR, NT>});
+    if constexpr (KIND == TK_Y_INV)
+      v.push_back({nm, pid, &ty_inv_w_launch<T, Plan, G, S, NR, NT>});
   }
+}
+// inverse x with the LDS-tiled transposed load and non-temporal lines
+template <typename T, typename Plan, int G>
+void add_t_x_inv_nt(std::vector<TV<T>> &v, int pid) {
+  if constexpr (t_fits<T, Plan, G, false, true, TK_X_INV>())
+    v.push_back({plan_name<Plan>() + "/G" + std::to_string(G) + "/nt", pid,
+                 &tx_inv_launch<T, Plan, G, false, true, true>});
 }
 
 // z pass with the loads issued before the multipliers (k_tz<..., PRE>)
@@ -521,6 +564,13 @@ void add_t_z_pre(std::vector<TV<T>> &v, int pid) {
     v.push_back({plan_name<Plan>() + "/G" + std::to_string(G) + (S ? "/split" : "/full") +
                      "/pre",
                  pid, &tz_pre_launch<T, Plan, G, S>});
+}
+template <typename T, typename Plan, int G, bool S, bool PRE>
+void add_t_z_nt(std::vector<TV<T>> &v, int pid) {
+  if constexpr (t_fits<T, Plan, G, S, true, TK_Z>())
+    v.push_back({plan_name<Plan>() + "/G" + std::to_string(G) + (S ? "/split" : "/full") +
+                     (PRE ? "/pre" : "") + "/nt",
+                 pid, &tz_nt_launch<T, Plan, G, S, PRE>});
 }
 // Fused step boundary (k_tx_fuse): an nx/2-point plan PH paired with an
 // nx-point plan PF of the same threads per line
@@ -572,6 +622,8 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
         add_t_fwd_w<T, P, G0, true>(v, pid);
         add_t_fwd_w<T, P, G0, false>(v, pid);
         add_t_fwd_w<T, P, G0 / 4, true>(v, pid);
+        add_t_fwd_w<T, P, G0, true, true>(v, pid);
+        add_t_fwd_w<T, P, G0, false, true>(v, pid);
       });
     }
     if constexpr (KIND == TK_Y_FWD || KIND == TK_Y_INV) {
@@ -584,6 +636,14 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
         add_t_y_w<T, P, G0 / 2, true, 1, KIND>(v, pid);
         add_t_y_w<T, P, G0 / 2, true, 2, KIND>(v, pid);
         add_t_y_w<T, P, 2 * G0, true, 4, KIND>(v, pid);
+        add_t_y_w<T, P, G0, true, 1, KIND, true>(v, pid);
+        add_t_y_w<T, P, G0, true, 2, KIND, true>(v, pid);
+        add_t_y_w<T, P, G0, true, 4, KIND, true>(v, pid);
+      });
+    }
+    if constexpr (KIND == TK_X_INV) {
+      for_t_plans<T, N>(typename Plans<N>::L{}, [&](auto p, int pid) {
+        add_t_x_inv_nt<T, decltype(p), 128 / (2 * (int)sizeof(T))>(v, pid);
       });
     }
     if constexpr (KIND == TK_Z) {
@@ -598,6 +658,10 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
         add_t_z_pre<T, P, 1, true>(v, pid);
         add_t_z_pre<T, P, 1, false>(v, pid);
         add_t_z_pre<T, P, 2, true>(v, pid);
+        add_t_z_nt<T, P, 1, true, false>(v, pid);
+        add_t_z_nt<T, P, 1, false, false>(v, pid);
+        add_t_z_nt<T, P, 1, true, true>(v, pid);
+        add_t_z_nt<T, P, 1, false, true>(v, pid);
       });
     }
     if constexpr (KIND == TK_X_FUSE) {
