@@ -75,6 +75,10 @@ TUNG_STEP_ROLES = ["step_layout", "x_r2c", "col_y", "pfc_z", "x_tung", "x_c2r", 
                    "t_y_fwd", "t_z", "t_y_inv", "t_x_tung", "t_x_inv"]
 
 
+# kernels whose last template argument selects non-temporal line accesses
+NT_KERNELS = ("k_tz", "k_tx_inv", "k_tx_fwd_w", "k_ty_fwd_w", "k_ty_inv_w")
+
+
 def variant_key(kernel, prec, n, variant, direction=None, mode=0):
     """rocprof short name (openpfc_amd.profile_summary.short_name) of a variant;
     `mode` is the z-pass kind (0 PFC, 1 Tungsten mean field, 2 Tungsten combine)."""
@@ -87,10 +91,8 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
         return (f"k_tx_fuse<{t},N{n // 2}/E{ea[1:]},r{ra[1:]}+N{n}/E{pb[1][1:]},r{pb[0][1:]},"
                 f"{pb[2][1:]},{'true' if pb[3] == 'split' else 'false'}>")
     parts = variant.split("/")
-    if "nt" in parts:  # non-temporal form: the same kernel name with "_nt"
-        key = variant_key(kernel, prec, n, "/".join(p for p in parts if p != "nt"),
-                          direction, mode)
-        return key.replace("<", "_nt<", 1)
+    nt = "nt" in parts  # non-temporal lines: the kernel's last template argument
+    parts = [p for p in parts if p != "nt"]
     if kernel == "k_tx_fwd" and "r2c2" in parts:
         kernel, n = "k_tx_fwd2", n // 2
     if kernel == "k_tx_fwd" and "wide" in parts:
@@ -121,6 +123,8 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
         rest.append("false" if "direct" in parts else "true")
         if kernel in ("k_tx_r2c", "k_tx_tung"):
             rest.append("true" if "wide" in parts else "false")
+    if kernel in NT_KERNELS:
+        rest.append("true" if nt else "false")
     t = "double" if prec == "f64" else "float"
     return f"{kernel}<{t},N{nn}/E{e},r{radices}" + ("," + ",".join(rest) if rest else "") + ">"
 

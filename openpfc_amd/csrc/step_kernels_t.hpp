@@ -216,10 +216,10 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd(TStepArgs<T> a) {
 // ----------------------------------------------------------------------------
 // T2 z pass, in place: G consecutive z lines (kx*ny + ky) per block.
 //
-// The *_nt kernels here mark their loads and stores of lines non-temporal
-// (ld_line / st_line with NT): every such value is read or written once per
-// pass, and the hint keeps the stream from displacing the twiddles and the
-// lines other blocks still share.
+// NT = true (here and in k_tx_inv, k_tx_fwd_w, k_ty_fwd_w, k_ty_inv_w) marks
+// the loads and stores of lines non-temporal (ld_line / st_line): every such
+// value is read or written once per pass, and the hint keeps the stream from
+// displacing the twiddles and the lines other blocks still share.
 // ----------------------------------------------------------------------------
 template <bool NT, typename T> __device__ __forceinline__ cplx<T> ld_line(const cplx<T> *p) {
   if constexpr (NT)
@@ -242,11 +242,12 @@ template <bool NT, typename T> __device__ __forceinline__ void st_line(cplx<T> *
   }
 }
 
-template <typename T, typename Plan, int G, bool SPLIT, int MODE, bool PRE, bool NT>
-__device__ __forceinline__ void tz_body(TStepArgs<T> a) {
+template <typename T, typename Plan, int G, bool SPLIT, int MODE, bool PRE = false,
+          bool NT = false>
+__global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
+  __shared__ T lds[(SPLIT ? 1 : 2) * t_plane<Plan, G>()];
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
   constexpr int PL = t_plane<Plan, G>();
-  __shared__ T lds[(SPLIT ? 1 : 2) * PL];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const long nlines = (long)a.W * a.ny;
   const long l0 = (long)blockIdx.x * G;
@@ -263,11 +264,11 @@ __device__ __forceinline__ void tz_body(TStepArgs<T> a) {
   if constexpr (MODE == ZM_TMF) {
     // A = Fz(A) (kept for the combine); B = Fz^-1(fMF * A)
 #pragma unroll
-    for (int e = 0; e < E; ++e) acc[e] = ld_line<NT>(la + t + TL * e);
+    for (int e = 0; e < E; ++e) acc[e] = ld_line<NT>(la + (t + TL * e));
     LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(acc, t, ll, a.tw_z);
     if (active) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) st_line<NT>(la + t + TL * e, acc[e]);
+      for (int e = 0; e < E; ++e) st_line<NT>(la + (t + TL * e), acc[e]);
     }
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -295,16 +296,16 @@ __device__ __forceinline__ void tz_body(TStepArgs<T> a) {
     if constexpr (PRE) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        acc[e] = ld_line<NT>(la + t + TL * e);
-        v[e] = ld_line<NT>(lb + t + TL * e);
+        acc[e] = ld_line<NT>(la + (t + TL * e));
+        v[e] = ld_line<NT>(lb + (t + TL * e));
       }
       multipliers();
     } else {
       multipliers();
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        if constexpr (MODE == ZM_PFC) acc[e] = ld_line<NT>(la + t + TL * e);
-        v[e] = ld_line<NT>(lb + t + TL * e);
+        if constexpr (MODE == ZM_PFC) acc[e] = ld_line<NT>(la + (t + TL * e));
+        v[e] = ld_line<NT>(lb + (t + TL * e));
       }
     }
     if constexpr (MODE == ZM_PFC) {
@@ -321,7 +322,7 @@ __device__ __forceinline__ void tz_body(TStepArgs<T> a) {
       LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_z);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const cplx<T> h = PRE ? acc[e] : ld_line<NT>(la + t + TL * e);
+        const cplx<T> h = PRE ? acc[e] : ld_line<NT>(la + (t + TL * e));
         acc[e] = c_add(c_scale(h, opl[e]), c_scale(v[e], opn[e]));
       }
     }
@@ -330,18 +331,10 @@ __device__ __forceinline__ void tz_body(TStepArgs<T> a) {
   if (active) {
     cplx<T> *__restrict__ out = MODE == ZM_TMF ? lb : la;
 #pragma unroll
-    for (int e = 0; e < E; ++e) st_line<NT>(out + t + TL * e, acc[e]);
+    for (int e = 0; e < E; ++e) st_line<NT>(out + (t + TL * e), acc[e]);
   }
 }
 
-template <typename T, typename Plan, int G, bool SPLIT, int MODE, bool PRE = false>
-__global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
-  tz_body<T, Plan, G, SPLIT, MODE, PRE, false>(a);
-}
-template <typename T, typename Plan, int G, bool SPLIT, int MODE, bool PRE>
-__global__ void __launch_bounds__(G *Plan::TL) k_tz_nt(TStepArgs<T> a) {
-  tz_body<T, Plan, G, SPLIT, MODE, PRE, true>(a);
-}
 
 // ----------------------------------------------------------------------------
 // T2 -> T1 inverse y: block = (kx, z0..z0+G-1); transposed load.
@@ -396,12 +389,12 @@ template <typename T, typename Plan, int G, bool TILE> constexpr int tx_inv_lds(
   return stage > fft ? stage : fft;
 }
 
-template <typename T, typename Plan, int G, bool TILE, bool NT>
-__device__ __forceinline__ void tx_inv_body(TStepArgs<T> a) {
+template <typename T, typename Plan, int G, bool TILE, bool NT = false>
+__global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
+  __shared__ T lds[tx_inv_lds<T, Plan, G, TILE>()];
   constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N, RP = row_padded(M);
   constexpr int PL = t_plane<Plan, G>();
   constexpr int W = M + 1;
-  __shared__ T lds[tx_inv_lds<T, Plan, G, TILE>()];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngy = (a.ny + G - 1) / G;
   const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
@@ -432,7 +425,7 @@ __device__ __forceinline__ void tx_inv_body(TStepArgs<T> a) {
   for (int i = threadIdx.x; i < W * G; i += G * TL) {
     const int kx = i / G, gg = i % G;
     cplx<T> w{0, 0};
-    if (y0 + gg < a.ny) w = ld_line<NT>(a.A1 + ((long)z * a.W + kx) * a.ny + y0 + gg);
+    if (y0 + gg < a.ny) w = ld_line<NT>(a.A1 + (((long)z * a.W + kx) * a.ny + y0 + gg));
     sr[tile_idx<G>(kx, gg)] = w.x;
     si[tile_idx<G>(kx, gg)] = w.y;
   }
@@ -458,18 +451,10 @@ __device__ __forceinline__ void tx_inv_body(TStepArgs<T> a) {
     cplx<T> *__restrict__ out =
         reinterpret_cast<cplx<T> *>(a.psi_out + ((long)z * a.ny + y) * (2L * M));
 #pragma unroll
-    for (int e = 0; e < E; ++e) st_line<NT>(out + t + TL * e, v[e]);
+    for (int e = 0; e < E; ++e) st_line<NT>(out + (t + TL * e), v[e]);
   }
 }
 
-template <typename T, typename Plan, int G, bool TILE>
-__global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
-  tx_inv_body<T, Plan, G, TILE, false>(a);
-}
-template <typename T, typename Plan, int G, bool TILE>
-__global__ void __launch_bounds__(G *Plan::TL) k_tx_inv_nt(TStepArgs<T> a) {
-  tx_inv_body<T, Plan, G, TILE, true>(a);
-}
 
 // ----------------------------------------------------------------------------
 // Half-spectrum rows of G consecutive y rows of plane z, transposed store into
@@ -723,7 +708,7 @@ __device__ __forceinline__ void fwd_split_store_wide(cplx<T> (&v)[Plan::E], T *l
     for (int i = threadIdx.x; i < W * G; i += G * TL) {
       const int kx = i / G, gg = i % G;
       if (y0 + gg < a.ny)
-        st_line<NT>(out + ((long)z * a.W + kx) * a.ny + y0 + gg,
+        st_line<NT>(out + (((long)z * a.W + kx) * a.ny + y0 + gg),
                     cplx<T>{sr[tile_idx<G>(kx, gg)], si[tile_idx<G>(kx, gg)]});
     }
   }
@@ -741,10 +726,10 @@ template <typename T, typename Plan, int G, bool SPLIT> constexpr int tx_fwd_w_l
   return stage > fft ? stage : fft;
 }
 
-template <typename T, typename Plan, int G, bool SPLIT, bool NT>
-__device__ __forceinline__ void tx_fwd_w_body(TStepArgs<T> a) {
-  constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N;
+template <typename T, typename Plan, int G, bool SPLIT, bool NT = false>
+__global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w(TStepArgs<T> a) {
   __shared__ T lds[tx_fwd_w_lds<T, Plan, G, SPLIT>()];
+  constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N;
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngy = (a.ny + G - 1) / G;
   const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
@@ -756,20 +741,12 @@ __device__ __forceinline__ void tx_fwd_w_body(TStepArgs<T> a) {
   cplx<T> v[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const T u = ld_val<NT>(row + t + TL * e);
+    const T u = ld_val<NT>(row + (t + TL * e));
     v[e] = cplx<T>{u, a.nl(u)};
   }
   fwd_split_store_wide<T, Plan, G, SPLIT, NT>(v, lds, t, g, z, y0, a);
 }
 
-template <typename T, typename Plan, int G, bool SPLIT>
-__global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w(TStepArgs<T> a) {
-  tx_fwd_w_body<T, Plan, G, SPLIT, false>(a);
-}
-template <typename T, typename Plan, int G, bool SPLIT>
-__global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w_nt(TStepArgs<T> a) {
-  tx_fwd_w_body<T, Plan, G, SPLIT, true>(a);
-}
 
 // ----------------------------------------------------------------------------
 // y passes with few-round transpositions: the N positions of the G lines go
@@ -783,13 +760,13 @@ template <typename T, typename Plan, int G, bool SPLIT, int NR> constexpr int ty
   return stage > fft ? stage : fft;
 }
 
-template <typename T, typename Plan, int G, bool SPLIT, int NR, bool NT>
-__device__ __forceinline__ void ty_fwd_w_body(TStepArgs<T> a) {
+template <typename T, typename Plan, int G, bool SPLIT, int NR, bool NT = false>
+__global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd_w(TStepArgs<T> a) {
+  __shared__ T lds[ty_w_lds<T, Plan, G, SPLIT, NR>()];
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
   constexpr int PL = t_plane<Plan, G>();
   constexpr int ROWS = N / NR, EPR = E / NR; // tile rows, values per thread per round
   static_assert(E % NR == 0, "NR must divide E");
-  __shared__ T lds[ty_w_lds<T, Plan, G, SPLIT, NR>()];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngz = (a.nz + G - 1) / G;
   const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
@@ -802,7 +779,7 @@ __device__ __forceinline__ void ty_fwd_w_body(TStepArgs<T> a) {
 
   cplx<T> v[E];
 #pragma unroll
-  for (int e = 0; e < E; ++e) v[e] = ld_line<NT>(line + t + TL * e);
+  for (int e = 0; e < E; ++e) v[e] = ld_line<NT>(line + (t + TL * e));
   const LdsLine<T> ll{lds, g * RP, 1, PL};
   LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_y);
   // ky = TL*e + t; round h holds e in [h*EPR, (h+1)*EPR), i.e. ky in [h*ROWS, (h+1)*ROWS)
@@ -820,28 +797,21 @@ __device__ __forceinline__ void ty_fwd_w_body(TStepArgs<T> a) {
     for (int i = threadIdx.x; i < ROWS * G; i += G * TL) {
       const int row = i / G, gg = i % G;
       if (z0 + gg < a.nz)
-        st_line<NT>(out + ((long)kx * N + h * ROWS + row) * a.nz + z0 + gg,
+        st_line<NT>(out + (((long)kx * N + h * ROWS + row) * a.nz + z0 + gg),
                     cplx<T>{tr[tile_idx<G>(row, gg)], ti[tile_idx<G>(row, gg)]});
     }
   }
 }
 
-template <typename T, typename Plan, int G, bool SPLIT, int NR>
-__global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd_w(TStepArgs<T> a) {
-  ty_fwd_w_body<T, Plan, G, SPLIT, NR, false>(a);
-}
-template <typename T, typename Plan, int G, bool SPLIT, int NR>
-__global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd_w_nt(TStepArgs<T> a) {
-  ty_fwd_w_body<T, Plan, G, SPLIT, NR, true>(a);
-}
 
-template <typename T, typename Plan, int G, bool SPLIT, int NR, bool NT>
-__device__ __forceinline__ void ty_inv_w_body(TStepArgs<T> a) {
+
+template <typename T, typename Plan, int G, bool SPLIT, int NR, bool NT = false>
+__global__ void __launch_bounds__(G *Plan::TL) k_ty_inv_w(TStepArgs<T> a) {
+  __shared__ T lds[ty_w_lds<T, Plan, G, SPLIT, NR>()];
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
   constexpr int PL = t_plane<Plan, G>();
   constexpr int ROWS = N / NR, EPR = E / NR;
   static_assert(E % NR == 0, "NR must divide E");
-  __shared__ T lds[ty_w_lds<T, Plan, G, SPLIT, NR>()];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngz = (a.nz + G - 1) / G;
   const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
@@ -855,7 +825,7 @@ __device__ __forceinline__ void ty_inv_w_body(TStepArgs<T> a) {
     for (int i = threadIdx.x; i < ROWS * G; i += G * TL) {
       const int row = i / G, gg = i % G;
       cplx<T> w{0, 0};
-      if (z0 + gg < a.nz) w = ld_line<NT>(a.A2 + ((long)kx * N + h * ROWS + row) * a.nz + z0 + gg);
+      if (z0 + gg < a.nz) w = ld_line<NT>(a.A2 + (((long)kx * N + h * ROWS + row) * a.nz + z0 + gg));
       tr[tile_idx<G>(row, gg)] = w.x;
       ti[tile_idx<G>(row, gg)] = w.y;
     }
@@ -872,18 +842,10 @@ __device__ __forceinline__ void ty_inv_w_body(TStepArgs<T> a) {
   if (z < a.nz) {
     cplx<T> *__restrict__ line = a.A1 + ((long)z * a.W + kx) * N;
 #pragma unroll
-    for (int e = 0; e < E; ++e) st_line<NT>(line + t + TL * e, v[e]);
+    for (int e = 0; e < E; ++e) st_line<NT>(line + (t + TL * e), v[e]);
   }
 }
 
-template <typename T, typename Plan, int G, bool SPLIT, int NR>
-__global__ void __launch_bounds__(G *Plan::TL) k_ty_inv_w(TStepArgs<T> a) {
-  ty_inv_w_body<T, Plan, G, SPLIT, NR, false>(a);
-}
-template <typename T, typename Plan, int G, bool SPLIT, int NR>
-__global__ void __launch_bounds__(G *Plan::TL) k_ty_inv_w_nt(TStepArgs<T> a) {
-  ty_inv_w_body<T, Plan, G, SPLIT, NR, true>(a);
-}
 
 // ----------------------------------------------------------------------------
 // Step boundary of a multi-step PFC run, fused: the inverse x pass of step n

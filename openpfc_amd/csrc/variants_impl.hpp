@@ -344,7 +344,7 @@ template <typename T, typename Plan, int G, bool S, bool TILE, bool NT = false>
 hipError_t tx_inv_launch(TStepArgs<T> a, hipStream_t s) {
   dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
   if constexpr (NT)
-    hipLaunchKernelGGL((k_tx_inv_nt<T, Plan, G, TILE>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_tx_inv<T, Plan, G, TILE, true>), grid, block, 0, s, a);
   else
     hipLaunchKernelGGL((k_tx_inv<T, Plan, G, TILE>), grid, block, 0, s, a);
   return hipGetLastError();
@@ -374,7 +374,7 @@ template <typename T, typename Plan, int G, bool S, bool NT = false>
 hipError_t tx_fwd_w_launch(TStepArgs<T> a, hipStream_t s) {
   dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
   if constexpr (NT)
-    hipLaunchKernelGGL((k_tx_fwd_w_nt<T, Plan, G, S>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_tx_fwd_w<T, Plan, G, S, true>), grid, block, 0, s, a);
   else
     hipLaunchKernelGGL((k_tx_fwd_w<T, Plan, G, S>), grid, block, 0, s, a);
   return hipGetLastError();
@@ -399,20 +399,20 @@ hipError_t tz_pre_launch(TStepArgs<T> a, hipStream_t s) {
   }
   return hipGetLastError();
 }
-// z pass with non-temporal line loads and stores (k_tz_nt)
+// z pass with non-temporal line loads and stores (k_tz<..., NT = true>)
 template <typename T, typename Plan, int G, bool S, bool PRE>
 hipError_t tz_nt_launch(TStepArgs<T> a, hipStream_t s) {
   const long lines = (long)a.W * a.ny;
   dim3 grid((unsigned)((lines + G - 1) / G)), block(G * Plan::TL);
   switch (a.mode) {
   case ZM_PFC:
-    hipLaunchKernelGGL((k_tz_nt<T, Plan, G, S, ZM_PFC, PRE>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_tz<T, Plan, G, S, ZM_PFC, PRE, true>), grid, block, 0, s, a);
     break;
   case ZM_TMF:
-    hipLaunchKernelGGL((k_tz_nt<T, Plan, G, S, ZM_TMF, PRE>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_tz<T, Plan, G, S, ZM_TMF, PRE, true>), grid, block, 0, s, a);
     break;
   case ZM_TCOMB:
-    hipLaunchKernelGGL((k_tz_nt<T, Plan, G, S, ZM_TCOMB, PRE>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_tz<T, Plan, G, S, ZM_TCOMB, PRE, true>), grid, block, 0, s, a);
     break;
   default:
     return hipErrorInvalidValue;
@@ -444,7 +444,7 @@ hipError_t ty_fwd_w_launch(TStepArgs<T> a, hipStream_t s) {
   if (a.nfields != 1 && a.nfields != 2) return hipErrorInvalidValue;
   dim3 grid((unsigned)(a.W * ((a.nz + G - 1) / G)), (unsigned)a.nfields), block(G * Plan::TL);
   if constexpr (NT)
-    hipLaunchKernelGGL((k_ty_fwd_w_nt<T, Plan, G, S, NR>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_ty_fwd_w<T, Plan, G, S, NR, true>), grid, block, 0, s, a);
   else
     hipLaunchKernelGGL((k_ty_fwd_w<T, Plan, G, S, NR>), grid, block, 0, s, a);
   return hipGetLastError();
@@ -453,7 +453,7 @@ template <typename T, typename Plan, int G, bool S, int NR, bool NT = false>
 hipError_t ty_inv_w_launch(TStepArgs<T> a, hipStream_t s) {
   dim3 grid((unsigned)(a.W * ((a.nz + G - 1) / G))), block(G * Plan::TL);
   if constexpr (NT)
-    hipLaunchKernelGGL((k_ty_inv_w_nt<T, Plan, G, S, NR>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_ty_inv_w<T, Plan, G, S, NR, true>), grid, block, 0, s, a);
   else
     hipLaunchKernelGGL((k_ty_inv_w<T, Plan, G, S, NR>), grid, block, 0, s, a);
   return hipGetLastError();
@@ -525,7 +525,7 @@ void add_t_fwd2(std::vector<TV<T>> &v, int pid) {
 }
 
 // PFC forward x with the wide transposed store (k_tx_fwd_w).
-// NT: non-temporal loads and stores of the lines (k_*_nt kernels).
+// NT: non-temporal loads and stores of the lines (the kernels' NT parameter).
 template <typename T, typename Plan, int G, bool S, bool NT = false>
 void add_t_fwd_w(std::vector<TV<T>> &v, int pid) {
   if constexpr (G >= 1 && G * Plan::TL <= kMaxThreads && G * Plan::TL >= 64 &&
