@@ -1,0 +1,47 @@
+"""bench.py names a variant's kernel the way openpfc_amd.profile_summary
+shortens the demangled name rocprofv3 prints, so the roofline's traffic
+lookup finds the kernel the step actually ran.  The demangled names below are
+the form rocprofv3 reports for these launches (template arguments in
+declaration order, defaulted ones included)."""
+import os
+import sys
+
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+import bench  # noqa: E402
+from openpfc_amd.profile_summary import short_name  # noqa: E402
+
+PLAN8 = "pfc_amd::FftPlan<1024, 8, pfc_amd::RadixList<8, 8, 4, 4> >"
+PLAN16 = "pfc_amd::FftPlan<1024, 16, pfc_amd::RadixList<16, 16, 4> >"
+HALF4 = "pfc_amd::FftPlan<512, 4, pfc_amd::RadixList<4, 4, 4, 4, 2> >"
+HALF8 = "pfc_amd::FftPlan<512, 8, pfc_amd::RadixList<8, 8, 8> >"
+
+
+def demangled(kernel, t, plan, rest):
+    return f"void pfc_amd::{kernel}<{t}, {plan}, {rest}>(pfc_amd::TStepArgs<{t}>)"
+
+
+@pytest.mark.parametrize("kernel,variant,mode,name", [
+    ("k_ty_fwd", "r8x8x4x4/E8/G8/split/wide2/nt", 0,
+     demangled("k_ty_fwd_w", "double", PLAN8, "8, true, 2, true")),
+    ("k_ty_fwd", "r8x8x4x4/E8/G8/split/wide2", 0,
+     demangled("k_ty_fwd_w", "double", PLAN8, "8, true, 2, false")),
+    ("k_tz", "r8x8x4x4/E8/G1/full/pre/nt", 0,
+     demangled("k_tz", "double", PLAN8, "1, false, 0, true, true")),
+    ("k_tz", "r8x8x4x4/E8/G1/split", 2,
+     demangled("k_tz", "double", PLAN8, "1, true, 2, false, false")),
+    ("k_tx_fwd", "r16x16x4/E16/G8/split/wide", 0,
+     demangled("k_tx_fwd_w", "double", PLAN16, "8, true, false")),
+    ("k_ty_inv", "r16x16x4/E16/G8/split/wide2", 0,
+     demangled("k_ty_inv_w", "double", PLAN16, "8, true, 2, false")),
+    ("k_tx_inv", "r4x4x4x4x2/E4/G8/nt", 0,
+     demangled("k_tx_inv", "double", HALF4, "8, true, true")),
+    ("k_tx_inv", "r8x8x8/E8/G16", 0,
+     demangled("k_tx_inv", "float", HALF8, "16, true, false")),
+    ("k_ty_fwd", "r16x16x4/E16/G16/split", 0,
+     demangled("k_ty_fwd", "float", PLAN16, "16, true, true")),
+])
+def test_variant_key_matches_profile_name(kernel, variant, mode, name):
+    prec = "f32" if "<float" in name else "f64"
+    assert bench.variant_key(kernel, prec, 1024, variant, mode=mode) == short_name(name)
