@@ -244,7 +244,7 @@ def copy_calibration(torch, nbytes=4 << 30, reps=5):
 
 
 def accuracy_run(P, torch, n, dx, dt, r, steps, every=100):
-    """Configuration 3: FP64 and FP32-mixed advanced in lockstep from the
+    """Accuracy run: FP64 and FP32-mixed advanced in lockstep from the
     same field; relative L2 / max error of FP32-mixed against FP64 every
     `every` steps (the FP64 GPU step is held to 1e-10 of the CPU oracle)."""
     fft64, st64, psi = setup(P, torch, n, P.F64, dx, dt, r)
@@ -399,7 +399,7 @@ def tungsten_main(args, P, torch):
 
 
 def adjoint_run(P, torch, n=256, nsteps=200, every=20):
-    """Configuration 5: gradients of J = ||psi_T - psi_target||^2 with respect
+    """256^3 x 200-step adjoint run: gradients of J = ||psi_T - psi_target||^2 with respect
     to psi0 and r through the libtorch autograd::Function (adjoint steps,
     checkpoints every `every` steps), psi_target from the seeded run at
     r = -0.30; checked against FP64 central differences (one random
@@ -480,7 +480,7 @@ ENS_METRIC = "ensemble DOF-steps/s (64 uneven PFC runs, FP32-mixed, one graph la
 
 
 def ensemble_draw(nruns=64, seed=4):
-    """Configuration 4: sizes from ENS_SIZES, r in U(-0.35, -0.15) and the
+    """Ensemble draw: sizes from ENS_SIZES, r in U(-0.35, -0.15) and the
     mean in U(-0.30, -0.15), all from one seeded numpy generator."""
     import numpy as np
     rng = np.random.default_rng(seed)
@@ -496,7 +496,7 @@ def ensemble_bytes(sizes, esize):
 
 
 def ensemble_run(P, torch, steps, warmup, nruns=64, seed=4, acc_steps=100):
-    """Configuration 4: nruns independent plain-PFC runs of uneven cubic
+    """Ensemble of nruns independent plain-PFC runs of uneven cubic
     sizes (dx = pi/4, dt = 0.5), psi0 = mean + U(-0.01, 0.01) (torch's
     device generator, seed 1000*seed + run), FP32-mixed (each field holds
     psi - <psi> in FP32; the mean is kept in FP64).  Autotuned per size
@@ -615,10 +615,10 @@ def main(argv=None):
                     help="run FP64 and FP32-mixed in lockstep for this many steps and "
                          "report the FP32-mixed error every 100 steps (no timing)")
     ap.add_argument("--adjoint", action="store_true",
-                    help="configuration 5: adjoint gradients at 256^3, 200 steps, against "
+                    help="adjoint gradients at 256^3, 200 steps, against "
                          "FP64 central differences (no timing line)")
     ap.add_argument("--ensemble", action="store_true",
-                    help="configuration 4: 64 uneven-size PFC runs (FP32-mixed) advanced "
+                    help="64 uneven-size PFC runs (FP32-mixed) advanced "
                          "together, one hipGraph launch per step; --steps K are timed")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also time a device-to-device copy of 2 x 4 GiB (the plain "

@@ -13,7 +13,7 @@ Fixtures (numpy .npz, no pickles):
                   166-213), 16^3, dx = pi/4, dt = 0.5, eps = -0.25,
                   psi0 = -0.2 + U(-0.01, 0.01) from mt19937_64(1234); psi after
                   0, 10 and 20 steps (full fields).
-  pfc_64_200.npz  the same model at 64^3 (the CPU-runnable configuration), 200 steps;
+  pfc_64_200.npz  the same model at 64^3 (the size the CPU finishes in seconds), 200 steps;
                   per snapshot at steps 0/50/100/150/200: sum, sum of squares,
                   min, max and every 64th value in linear order.
 The uniform draw is lo + (hi-lo) * ((r >> 11) * 2^-53) for each 64-bit output r.

@@ -1,9 +1,9 @@
 """GPU parity of the fused plain-PFC ETD1 step against the CPU oracle.
 
 The oracle restates docs/api/examples/10_complete_pfc_model.cpp:121-144
-(step) and :166-213 (operators) with the reference's FFT contract.  The bar
-is the north-star one: max|psi_gpu - psi_ref| / max|psi_ref| <= 1e-10 in FP64
-for the same seed and step count.  Invariants (reference
+(step) and :166-213 (operators) with the reference's FFT contract.  The bar:
+max|psi_gpu - psi_ref| / max|psi_ref| <= 1e-10 in FP64 for the same seed and
+step count.  Invariants (reference
 tests/integration/scenarios/complete_simulation/test_mass_conservation.cpp:47):
 the mean is conserved to 1e-12, a constant field is a fixed point.
 """
@@ -108,7 +108,7 @@ def test_step_before_operators_is_state_error(torch_gpu):
 
 
 @pytest.mark.parametrize("n,steps", [((64, 64, 64), 1000), ((256, 128, 128), 200)])
-def test_fp32_mixed_meets_north_star_bar(torch_gpu, n, steps):
+def test_fp32_mixed_within_1e4_of_fp64(torch_gpu, n, steps):
     """FP32 field and transforms, FP64 conserved mean (mean-split step):
     relative L2 error <= 1e-4 against the FP64 oracle after the run, and the
     mean exactly conserved."""

@@ -113,7 +113,7 @@ def test_trajectory_16_matches_numpy_fixture():
 
 
 def test_trajectory_64_200_matches_numpy_fixture():
-    """The CPU-runnable configuration: 64^3, 200 steps (t = 0 -> 100)."""
+    """The 64^3 demo workload: 200 steps (t = 0 -> 100)."""
     d = np.load(os.path.join(GOLD, "pfc_64_200.npz"))
     n = (64, 64, 64)
     a = O.uniform_noise(n, 1234, -0.01, 0.01, base=-0.2)
