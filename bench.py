@@ -123,6 +123,10 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
         rest.append("false" if "direct" in parts else "true")
         if kernel in ("k_tx_r2c", "k_tx_tung"):
             rest.append("true" if "wide" in parts else "false")
+        wpe = [p[1:] for p in parts if p[:1] == "w" and p[1:].isdigit()]
+        if kernel == "k_tx_tung" and wpe:  # registers capped for wpe waves per SIMD
+            kernel = "k_tx_tung_o"
+            rest.append(wpe[0])
     if kernel in NT_KERNELS:
         rest.append("true" if nt else "false")
     t = "double" if prec == "f64" else "float"

@@ -569,12 +569,11 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_r2c(TStepArgs<T> a) {
 // inverse), combined with the rows of psi into N(psi, psi_MF), transformed
 // forward and stored transposed.  psi_MF goes to HBM only when mf_out is set.
 // ----------------------------------------------------------------------------
-template <typename T, typename Plan, int G, bool TILE, bool WIDE = false>
-__global__ void __launch_bounds__(G *Plan::TL) k_tx_tung(TStepArgs<T> a) {
+template <typename T, typename Plan, int G, bool TILE, bool WIDE>
+__device__ __forceinline__ void tx_tung_body(const TStepArgs<T> &a, T *lds) {
   constexpr int TL = Plan::TL, E = Plan::E, M = Plan::N, RP = row_padded(M);
   constexpr int PL = t_plane<Plan, G>();
   constexpr int W = M + 1;
-  __shared__ T lds[tx_inv_lds<T, Plan, G, TILE>()];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
   const int ngy = (a.ny + G - 1) / G;
   const unsigned bid = xcd_block<T, G>(blockIdx.x, gridDim.x);
@@ -631,6 +630,21 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_tung(TStepArgs<T> a) {
     t1_store_rows_wide<T, E, TL, G>(v, xm, spec, lds, t, g, z, y0, a.ny, a.W);
   else
     t1_store_rows<T, E, TL, G, TILE>(v, xm, spec, lds, t, g, z, y0, a.ny, a.W);
+}
+
+template <typename T, typename Plan, int G, bool TILE, bool WIDE = false>
+__global__ void __launch_bounds__(G *Plan::TL) k_tx_tung(TStepArgs<T> a) {
+  __shared__ T lds[tx_inv_lds<T, Plan, G, TILE>()];
+  tx_tung_body<T, Plan, G, TILE, WIDE>(a, lds);
+}
+// The same pass with registers capped so that WPE waves fit on a SIMD: with
+// the 2 x (nx/2 + 1) x (G + 1) staging tile, G = 8 lines of an E = 8 plan
+// fit two blocks on a CU only at <= 128 VGPRs, which the compiler reaches
+// without spilling when asked to.
+template <typename T, typename Plan, int G, bool TILE, bool WIDE, int WPE>
+__global__ void __launch_bounds__(G *Plan::TL, WPE) k_tx_tung_o(TStepArgs<T> a) {
+  __shared__ T lds[tx_inv_lds<T, Plan, G, TILE>()];
+  tx_tung_body<T, Plan, G, TILE, WIDE>(a, lds);
 }
 
 // ----------------------------------------------------------------------------

@@ -431,6 +431,12 @@ hipError_t tx_tung_wide_launch(TStepArgs<T> a, hipStream_t s) {
   hipLaunchKernelGGL((k_tx_tung<T, Plan, G, true, true>), grid, block, 0, s, a);
   return hipGetLastError();
 }
+template <typename T, typename Plan, int G, int WPE>
+hipError_t tx_tung_wide_o_launch(TStepArgs<T> a, hipStream_t s) {
+  dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * Plan::TL);
+  hipLaunchKernelGGL((k_tx_tung_o<T, Plan, G, true, true, WPE>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
 
 template <typename T, typename PH, typename PF, int G, bool S>
 hipError_t tx_fuse_launch(TStepArgs<T> a, hipStream_t s) {
@@ -585,6 +591,14 @@ void add_t_fuse(std::vector<TV<T>> &v, int pid, int pid2) {
     v.push_back(e);
   }
 }
+// Tungsten fused x pass, wide store, registers capped for WPE waves per SIMD
+template <typename T, typename Plan, int G, int WPE>
+void add_t_tung_wide_o(std::vector<TV<T>> &v, int pid) {
+  if constexpr (G * Plan::TL <= kMaxThreads && G * Plan::TL >= 64 &&
+                tx_inv_lds<T, Plan, G, true>() * (int)sizeof(T) <= kMaxLds)
+    v.push_back({plan_name<Plan>() + "/G" + std::to_string(G) + "/wide/w" + std::to_string(WPE),
+                 pid, &tx_tung_wide_o_launch<T, Plan, G, WPE>});
+}
 // Tungsten row kernels with the wide transposed store
 template <typename T, typename Plan, int G, int KIND>
 void add_t_rows_wide(std::vector<TV<T>> &v, int pid) {
@@ -686,6 +700,8 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
         constexpr int G0 = 128 / (2 * (int)sizeof(T));
         add_t_rows_wide<T, P, G0 / 2, KIND>(v, pid);
         add_t_rows_wide<T, P, G0, KIND>(v, pid);
+        if constexpr (KIND == TK_X_TUNG && P::E == 8 && sizeof(T) == 8)
+          add_t_tung_wide_o<T, P, 8, 4>(v, pid);
       });
     }
   }

@@ -41,6 +41,10 @@ def demangled(kernel, t, plan, rest):
      demangled("k_tx_inv", "float", HALF8, "16, true, false")),
     ("k_ty_fwd", "r16x16x4/E16/G16/split", 0,
      demangled("k_ty_fwd", "float", PLAN16, "16, true, true")),
+    ("k_tx_tung", "r8x8x8/E8/G4/wide", 0,
+     demangled("k_tx_tung", "double", HALF8, "4, true, true")),
+    ("k_tx_tung", "r8x8x8/E8/G8/wide/w4", 0,
+     demangled("k_tx_tung_o", "double", HALF8, "8, true, true, 4")),
 ])
 def test_variant_key_matches_profile_name(kernel, variant, mode, name):
     prec = "f32" if "<float" in name else "f64"
