@@ -630,7 +630,7 @@ def main(argv=None):
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU only (no sharding or collectives); "
+        print("bench.py: the step runs on one GPU (it is not sharded; no collectives); "
               "use --gpus 1", file=sys.stderr)
         return 2
 
