@@ -21,7 +21,10 @@ step boundaries are fused).  The same workload in FP32-mixed mode (FP32 deviatio
 transforms, FP64 conserved mean) is reported under "fp32_mixed" with its
 error against the FP64 run after the same steps.  `roofline` is the
 dominant kernel's algorithmic bytes per launch over its mean launch duration
-(HIP events on the stream the kernels run on).  `cpu_baseline` times the CPU
+(HIP events on the stream the kernels run on); `stream_copy` is the
+library's plain streaming copy on arrays of one half spectrum, the bandwidth a
+pass could reach on the same GPU, and `roofline.frac_of_stream_copy` /
+`step_frac_of_stream_copy` compare against it.  `cpu_baseline` times the CPU
 oracle (the repository's C restatement of the reference path, OpenMP) on one
 step of the same 1024^3 workload.
 
@@ -225,6 +228,40 @@ def cpu_baseline(n, dx, dt, r, nsteps):
     O.pfc_step(a, opL, opN, nsteps)
     dt_s = (time.perf_counter() - t0) / nsteps
     return 1.0 / dt_s, O.num_threads()
+
+
+def stream_copy(P, torch, nbytes, reps=5):
+    """GB/s of the library's plain streaming copy (pfc_hip_stream_copy: one
+    16-byte value per thread, nbytes read + nbytes written per launch) on
+    the current stream: the bandwidth a pass could reach on this GPU."""
+    import ctypes
+    from openpfc_amd._lib import check, load
+    lib = load()
+    nbytes = nbytes // 16 * 16
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    s = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(s.cuda_stream)
+
+    def launch():
+        check(lib.pfc_hip_stream_copy(ctypes.c_void_p(a.data_ptr()),
+                                      ctypes.c_void_p(b.data_ptr()), nbytes, sp))
+    launch()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        launch()
+        e1.record(s)
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    ok = bool(torch.equal(a, b))
+    del a, b
+    torch.cuda.empty_cache()
+    return {"GBps": 2 * nbytes / (best * 1e-3) / 1e9, "bytes_per_launch": 2 * nbytes,
+            "ms": best, "copy_correct": ok}
 
 
 def copy_calibration(torch, nbytes=4 << 30, reps=5):
@@ -773,6 +810,13 @@ def main(argv=None):
                                     "rel_l2": l2, "rel_max": mx, "mean_drift": drift}}
     del psi64
     torch.cuda.empty_cache()
+
+    # ---- streaming-copy ceiling on this GPU (arrays the size of one half
+    # spectrum) ---------------------------------------------------------------
+    sc = stream_copy(P, torch, 16 * (n // 2 + 1) * n * n)
+    out["stream_copy"] = sc
+    out["roofline"]["frac_of_stream_copy"] = out["roofline"]["achieved"] / sc["GBps"]
+    out["step_frac_of_stream_copy"] = step_bytes / (ms64 * 1e-3) / 1e9 / sc["GBps"]
 
     # ---- CPU baseline (oracle, rank 0, N=1) --------------------------------
     if not args.no_cpu_baseline:

@@ -437,6 +437,15 @@ int pfc_hip_ensemble_autotune(pfc_hip_ensemble_t ens, int reps, void *stream);
  */
 int pfc_hip_ensemble_step(pfc_hip_ensemble_t ens, int nsteps, int use_graph, void *stream);
 
+/* ---- measurement aid ------------------------------------------------------
+ * No reference counterpart.  A plain device-to-device streaming copy (one
+ * 16-byte value per thread, one read and one write per byte): the bandwidth
+ * the step's passes can be set against on the same GPU.  `bytes` must be a
+ * multiple of 16 and both pointers 16-byte aligned (PFC_HIP_EINVAL
+ * otherwise); src and dst must not overlap.  Asynchronous on `stream`.
+ */
+int pfc_hip_stream_copy(const void *src, void *dst, size_t bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,6 +99,7 @@ SIGNATURES = {
     "pfc_hip_ensemble_autotune": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
     "pfc_hip_ensemble_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
     "pfc_hip_pfc_free_energy": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_double), _vp]),
+    "pfc_hip_stream_copy": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
 }
 
 _lib = None

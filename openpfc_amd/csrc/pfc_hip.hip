@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -757,6 +758,14 @@ int free_energy_impl(pfc_hip_plan_s *p, const void *psi, double *F, hipStream_t 
   return 0;
 }
 
+// One 16-byte value per thread: the plain streaming copy of
+// pfc_hip_stream_copy.
+__global__ void __launch_bounds__(256) k_stream_copy(const uint4 *__restrict__ src,
+                                                     uint4 *__restrict__ dst, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 int check_plan(pfc_hip_plan_t p) {
   if (!p) return fail(PFC_HIP_EINVAL, "null plan");
   return 0;
@@ -1097,6 +1106,23 @@ int pfc_hip_pfc_free_energy(pfc_hip_plan_t plan, const void *psi, double *F, voi
   hipStream_t s = (hipStream_t)stream;
   return plan->prec == PFC_HIP_F64 ? free_energy_impl<double>(plan, psi, F, s)
                                    : free_energy_impl<float>(plan, psi, F, s);
+}
+
+int pfc_hip_stream_copy(const void *src, void *dst, size_t bytes, void *stream) {
+  if (!src || !dst) return fail(PFC_HIP_EINVAL, "pfc_hip_stream_copy: null pointer");
+  if (bytes % 16 || (uintptr_t)src % 16 || (uintptr_t)dst % 16)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_stream_copy: bytes and pointers must be 16-byte multiples");
+  const char *a = (const char *)src, *b = (const char *)dst;
+  if (a < b + bytes && b < a + bytes)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_stream_copy: src and dst overlap");
+  const size_t n = bytes / 16;
+  if (n == 0) return 0;
+  const size_t blocks = (n + 255) / 256;
+  if (blocks > 0x7fffffffu) return fail(PFC_HIP_EINVAL, "pfc_hip_stream_copy: too large");
+  hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     (const uint4 *)src, (uint4 *)dst, n);
+  PFC_HIP_TRY(hipGetLastError(), "pfc_hip_stream_copy");
+  return 0;
 }
 
 int pfc_hip_pfc_step_timed(pfc_hip_plan_t plan, void *psi, int nsteps,
