@@ -50,3 +50,19 @@ def test_abi_version_and_error_string_without_device():
     assert lib.pfc_hip_plan_create(8, 8, 8, 7, 0, ctypes.byref(ctypes.c_void_p())) == 1
     assert lib.pfc_hip_sizes(None, None, None, None) == 1
     assert lib.pfc_hip_plan_destroy(None) == 0
+
+
+@pytest.mark.parametrize("src,dst,nbytes,why", [
+    (0, 0x20000, 16, "null"),
+    (0x10000, 0x20000, 24, "16-byte"),
+    (0x10008, 0x20000, 16, "16-byte"),
+    (0x10000, 0x10010, 64, "overlap"),
+])
+def test_stream_copy_rejects_bad_arguments_before_any_device_call(src, dst, nbytes, why):
+    """pfc_hip_stream_copy checks its arguments on the host: these calls
+    return PFC_HIP_EINVAL without touching a device (none is needed here)."""
+    from openpfc_amd._lib import load
+    lib = load()
+    rc = lib.pfc_hip_stream_copy(ctypes.c_void_p(src), ctypes.c_void_p(dst), nbytes, None)
+    assert rc == 1
+    assert why in lib.pfc_hip_last_error().decode()
