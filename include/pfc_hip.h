@@ -215,10 +215,13 @@ int pfc_hip_pfc_set_mean(pfc_hip_plan_t plan, int enable, double mean);
  * with the PFC model, whose spacing and dt all fields share).  eps: `batch`
  * values, or NULL to keep the operators' eps for every field.  mean_split
  * != 0 selects the mean-split step of pfc_hip_pfc_set_mean for every field,
- * with mean[b] the conserved average of field b (then not NULL).  The values
- * live in device memory read by the step kernels, so a captured graph of
- * the step picks up later changes.  On a plan with batch 1 this sets the
- * plan's eps and mean mode.  No reference counterpart.
+ * with mean[b] the conserved average of field b (then not NULL).  On a plan
+ * with batch > 1 eps and the means live in device memory read by the step
+ * kernels; on a plan with batch 1 they are the plan's scalars.  An ensemble
+ * (pfc_hip_ensemble_step) captures its graph again when a group plan changed
+ * anything its launches take as an argument (scalars, the mean-split mode,
+ * variants), so later replays always use the values set here.  No reference
+ * counterpart.
  */
 int pfc_hip_pfc_set_members(pfc_hip_plan_t plan, const double *eps, const double *mean,
                             int mean_split);
@@ -431,8 +434,9 @@ int pfc_hip_ensemble_autotune(pfc_hip_ensemble_t ens, int reps, void *stream);
 /*
  * Advance every run by nsteps.  use_graph != 0: one step of all groups (on
  * parallel branches) is captured once into a hipGraph -- again after an
- * autotune -- and each step is one hipGraphLaunch on `stream`; 0: the same
- * launches are enqueued directly.  Per-run eps and means may be changed
+ * autotune, or when a group plan changed a launch argument since -- and each
+ * step is one hipGraphLaunch on `stream`; 0: the same launches are enqueued
+ * directly.  Per-run eps, means and the mean-split mode may be changed
  * between steps through pfc_hip_pfc_set_members on a group's plan.
  */
 int pfc_hip_ensemble_step(pfc_hip_ensemble_t ens, int nsteps, int use_graph, void *stream);

@@ -1,10 +1,16 @@
 """Differentiable PFC trajectories: the libtorch autograd::Function built from
 csrc/torch_pfc.cpp (openpfc_amd/openpfc_amd_torch.so).
 
-    psiT = pfc_trajectory(psi0, eps, nsteps, dt, spacing, every=20, mixed=False)
+    psiT = pfc_trajectory(psi0, eps, nsteps, dt, spacing, every=20, mixed=False, snaps=0)
     loss = ((psiT - target) ** 2).sum(); loss.backward()
-    # psi0.grad, eps.grad: adjoint gradients (pfc_hip_pfc_adjoint_step),
-    # checkpoints every `every` steps, segments recomputed in the backward
+    # psi0.grad, eps.grad: adjoint gradients (pfc_hip_pfc_adjoint_step)
+
+Checkpoints: snaps=0 keeps one every `every` steps and recomputes one
+segment at a time in the backward; snaps >= 1 uses the binomial (revolve)
+schedule with at most `snaps` fields held besides psi0, recomputing the
+fewest steps that budget allows.  Both give the same gradients bit for bit;
+schedule_cost(nsteps, every, snaps) gives (recomputed steps, fields held).
+All work runs on torch's current stream.
 
 psi0: contiguous float64 device tensor (nz, ny, nx); eps: float64 scalar
 tensor.  mixed=True steps and differentiates in the FP32-mixed mode (FP32
@@ -39,7 +45,20 @@ def load_extension():
 
 
 def pfc_trajectory(psi0, eps, nsteps: int, dt: float, spacing, every: int = 20,
-                   mixed: bool = False):
+                   mixed: bool = False, snaps: int = 0):
     return load_extension().pfc_trajectory(psi0, eps, int(nsteps), float(dt),
                                            [float(s) for s in spacing], int(every),
-                                           bool(mixed))
+                                           bool(mixed), int(snaps))
+
+
+def schedule_cost(nsteps: int, every: int = 20, snaps: int = 0):
+    """(steps recomputed by the backward, fields held besides psi0) of the
+    uniform schedule (snaps=0, a checkpoint every `every` steps) or of the
+    binomial schedule with `snaps` slots."""
+    return tuple(load_extension().schedule_cost(int(nsteps), int(every), int(snaps)))
+
+
+def binomial_costs(nsteps: int, snaps: int) -> list:
+    """Forward steps of the optimal reversal of m = 1..nsteps steps with the
+    start state held and `snaps` free slots (the schedule's recurrence)."""
+    return [int(c) for c in load_extension().binomial_costs(int(nsteps), int(snaps))]

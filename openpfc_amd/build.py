@@ -48,8 +48,20 @@ SOURCES = ["pfc_hip.hip", "variants_common.hip", "variants_f64.hip", "variants_f
 OBJDIR = os.path.join(HERE, "_build")
 
 
-def build_library(verbose: bool = False) -> str:
-    """Compile every HIP source (in parallel) and link openpfc_amd/libopenpfc_hip.so."""
+def _stale(obj: str, src: str) -> bool:
+    """Whether obj is older than its source or any header it may include."""
+    if not os.path.exists(obj):
+        return True
+    deps = [src] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
+    deps.append(os.path.join(ROOT, "include", "pfc_hip.h"))
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build_library(verbose: bool = False, incremental: bool = False) -> str:
+    """Compile every HIP source (in parallel) and link openpfc_amd/libopenpfc_hip.so.
+    incremental=True recompiles only objects older than their sources or
+    headers (a development aid; the default compiles everything)."""
     from concurrent.futures import ThreadPoolExecutor
     os.makedirs(OBJDIR, exist_ok=True)
     objs = []
@@ -57,8 +69,10 @@ def build_library(verbose: bool = False) -> str:
     for src in SOURCES:
         obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
+        if incremental and not _stale(obj, os.path.join(CSRC, src)):
+            continue
         jobs.append([hipcc(), *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)])
-    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 4)) as ex:
+    with ThreadPoolExecutor(max_workers=max(1, min(len(jobs), os.cpu_count() or 4))) as ex:
         list(ex.map(lambda c: _run(c, quiet=not verbose), jobs))
     tmp = LIB + ".tmp"
     _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs],
@@ -82,13 +96,17 @@ def build_torch_extension(verbose: bool = False) -> str:
     abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
     cmd = [cxx, "-O2", "-std=c++17", "-shared", "-fPIC", "-w",
            f"-D_GLIBCXX_USE_CXX11_ABI={abi}", "-DTORCH_EXTENSION_NAME=openpfc_amd_torch",
-           "-DTORCH_API_INCLUDE_EXTENSION_H"]
-    for d in ce.include_paths() + [sysconfig.get_paths()["include"]]:
+           "-DTORCH_API_INCLUDE_EXTENSION_H",
+           # torch's current HIP stream (c10/hip/HIPStream.h) needs the HIP headers
+           "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1"]
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    for d in ce.include_paths() + [sysconfig.get_paths()["include"], os.path.join(rocm, "include")]:
         cmd += ["-isystem", d]
     cmd += [os.path.join(CSRC, "torch_pfc.cpp")]
     for d in ce.library_paths():
         cmd += [f"-L{d}", f"-Wl,-rpath,{d}"]
-    cmd += ["-lc10", "-ltorch", "-ltorch_cpu", "-ltorch_python", f"-L{HERE}", "-lopenpfc_hip",
+    cmd += ["-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_python", f"-L{HERE}",
+            "-lopenpfc_hip",
             "-Wl,-rpath,$ORIGIN", "-o", TORCH_EXT + ".tmp"]
     _run(cmd, quiet=not verbose)
     os.replace(TORCH_EXT + ".tmp", TORCH_EXT)
@@ -96,5 +114,5 @@ def build_torch_extension(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build_library(verbose="-v" in sys.argv))
+    print(build_library(verbose="-v" in sys.argv, incremental="-i" in sys.argv))
     print(build_torch_extension(verbose="-v" in sys.argv))

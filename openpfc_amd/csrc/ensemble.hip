@@ -23,6 +23,7 @@
 
 namespace pfc_amd {
 extern thread_local std::string g_last_error; // pfc_hip_last_error()
+unsigned plan_generation(const pfc_hip_plan_s *p); // pfc_hip.hip
 }
 
 namespace {
@@ -111,12 +112,16 @@ struct pfc_hip_ensemble_s {
   hipEvent_t fork = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<unsigned> cap_gen; // group plans' generations at the capture
 };
 
 namespace {
 
 void drop_graph(pfc_hip_ensemble_s *e) {
-  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->exec) {
+    (void)hipDeviceSynchronize(); // replays still in flight finish first
+    (void)hipGraphExecDestroy(e->exec);
+  }
   if (e->graph) (void)hipGraphDestroy(e->graph);
   e->exec = nullptr;
   e->graph = nullptr;
@@ -177,7 +182,19 @@ int capture(pfc_hip_ensemble_s *e) {
     drop_graph(e);
     return fail(PFC_HIP_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(he));
   }
+  e->cap_gen.clear();
+  for (const Group &gr : e->groups) e->cap_gen.push_back(pfc_amd::plan_generation(gr.plan));
   return 0;
+}
+
+// Whether a group plan changed a launch argument of its step since the
+// capture (an eps or mean held as a scalar, the mean-split mode, a variant):
+// the graph would replay the old values.
+bool graph_stale(const pfc_hip_ensemble_s *e) {
+  if (e->cap_gen.size() != e->groups.size()) return true;
+  for (size_t g = 0; g < e->groups.size(); ++g)
+    if (pfc_amd::plan_generation(e->groups[g].plan) != e->cap_gen[g]) return true;
+  return false;
 }
 
 int check_ens(pfc_hip_ensemble_t e) {
@@ -309,7 +326,7 @@ int pfc_hip_ensemble_step(pfc_hip_ensemble_t ens, int nsteps, int use_graph, voi
       if (int rc = enqueue_step(ens, s)) return rc;
     return 0;
   }
-  if (nsteps > 0 && !ens->exec)
+  if (nsteps > 0 && (!ens->exec || graph_stale(ens)))
     if (int rc = capture(ens)) return rc;
   for (int i = 0; i < nsteps; ++i) ENS_TRY(hipGraphLaunch(ens->exec, s), "hipGraphLaunch");
   return 0;

@@ -20,6 +20,7 @@
 
 using namespace pfc_amd;
 
+
 namespace pfc_amd {
 // message of the last failed ABI call on this thread (field_io.hip sets it too)
 thread_local std::string g_last_error;
@@ -130,6 +131,11 @@ struct pfc_hip_plan_s {
   int batch = 1;
   void *m_eps = nullptr, *m_nl = nullptr;
   bool members = false;
+  // Bumped by every call that changes a value the step's kernels receive as
+  // a launch argument (operators, eps or mean held as scalars, the
+  // mean-split mode, variants): a graph captured before such a change
+  // replays stale arguments and must be captured again.
+  unsigned gen = 0;
 
   size_t inbox() const { return (size_t)nx * ny * nz; }
   size_t outbox() const { return (size_t)W * ny * nz; }
@@ -343,23 +349,27 @@ int r2c_impl(pfc_hip_plan_s *p, const void *in, void *out, hipStream_t s) {
   return 0;
 }
 
+// C2R through the workspace `work` (outbox complex values; the plan's own,
+// allocated on first use, when null).
 template <typename T>
-int c2r_impl(pfc_hip_plan_s *p, const void *in, void *out, hipStream_t s) {
+int c2r_impl(pfc_hip_plan_s *p, const void *in, void *out, hipStream_t s,
+             void *work = nullptr) {
   const void *cur = in;
   hipError_t e;
-  if (p->nz > 1 || p->ny > 1) {
+  if (!work && (p->nz > 1 || p->ny > 1)) {
     if (int rc = dalloc(p, &p->work, p->outbox() * 2 * p->esize, "c2r workspace"))
       return rc;
+    work = p->work;
   }
   if (p->nz > 1) {
-    e = col_pass<T>(p, 2, +1, cur, p->work, nullptr, nullptr, 1, p->W, s);
+    e = col_pass<T>(p, 2, +1, cur, work, nullptr, nullptr, 1, p->W, s);
     PFC_HIP_TRY(e, "c2r z pass");
-    cur = p->work;
+    cur = work;
   }
   if (p->ny > 1) {
-    e = col_pass<T>(p, 1, +1, cur, p->work, nullptr, nullptr, 1, p->W, s);
+    e = col_pass<T>(p, 1, +1, cur, work, nullptr, nullptr, 1, p->W, s);
     PFC_HIP_TRY(e, "c2r y pass");
-    cur = p->work;
+    cur = work;
   }
   e = x_c2r_pass<T>(p, cur, out, p->W,
                     1.0 / ((double)p->nx * (double)p->ny * (double)p->nz), s);
@@ -990,6 +1000,7 @@ int pfc_hip_pfc_set_operators(pfc_hip_plan_t plan, int model_kind,
   if (int rc = step_buffers(plan)) return rc;
   plan->dt = dt;
   plan->ops_ready = true;
+  ++plan->gen;
   return 0;
 }
 
@@ -1343,6 +1354,7 @@ int pfc_hip_variant_set(pfc_hip_plan_t plan, int role, int index) {
         if (*b) { (void)hipFree(*b); *b = nullptr; plan->bytes -= nb; }
     }
   }
+  if (plan->var[role] != index) ++plan->gen;
   plan->var[role] = index;
   if (role == ROLE_STEP_LAYOUT && plan->ops_ready) return step_buffers(plan);
   return 0;
@@ -1470,13 +1482,21 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
         double pm[9], fastest = 1e300;
         rc = run_step(pm);
         for (int r = 0; r <= reps && !rc; ++r) {
-          PFC_HIP_TRY(hipEventRecord(ev[0], s), "autotune event");
+          hipError_t he = hipEventRecord(ev[0], s);
+          if (he != hipSuccess) {
+            rc = fail(PFC_HIP_EDEVICE, std::string("autotune event: ") + hipGetErrorString(he));
+            break;
+          }
           rc = plan->prec == PFC_HIP_F64
                    ? t_pass<double>(plan, pfc_t_args<double>(plan, psi), 5, s)
                    : t_pass<float>(plan, pfc_t_args<float>(plan, psi), 5, s);
           if (rc) break;
-          PFC_HIP_TRY(hipEventRecord(ev[1], s), "autotune event");
-          PFC_HIP_TRY(hipEventSynchronize(ev[1]), "autotune sync");
+          he = hipEventRecord(ev[1], s);
+          if (he == hipSuccess) he = hipEventSynchronize(ev[1]);
+          if (he != hipSuccess) {
+            rc = fail(PFC_HIP_EDEVICE, std::string("autotune sync: ") + hipGetErrorString(he));
+            break;
+          }
           float ms = 0;
           (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
           if (r > 0) fastest = std::min(fastest, (double)ms);
@@ -1513,12 +1533,20 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
   }
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
-  if (!rc) {
-    PFC_HIP_TRY(hipMemcpyAsync(psi, plan->work, rbytes, hipMemcpyDeviceToDevice, s),
-                "autotune restore");
-    PFC_HIP_TRY(hipStreamSynchronize(s), "autotune sync");
+  ++plan->gen; // the variants of every tuned role may have changed
+  // psi is restored on every exit path once the backup was enqueued; a
+  // failure of the sweep keeps its own code and message
+  const std::string sweep_error = rc ? g_last_error : std::string();
+  hipError_t he = hipMemcpyAsync(psi, plan->work, rbytes, hipMemcpyDeviceToDevice, s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  if (rc) {
+    g_last_error = sweep_error + (he == hipSuccess ? std::string(" (psi restored)")
+                                                   : std::string(" (psi not restored: ") +
+                                                         hipGetErrorString(he) + ")");
+    return rc;
   }
-  return rc;
+  PFC_HIP_TRY(he, "autotune restore");
+  return 0;
 }
 
 int pfc_hip_pfc_set_members(pfc_hip_plan_t plan, const double *eps, const double *mean,
@@ -1542,6 +1570,7 @@ int pfc_hip_pfc_set_members(pfc_hip_plan_t plan, const double *eps, const double
     plan->op.eps = e[0];
     plan->mean_split = mean_split != 0;
     plan->mean = m[0];
+    ++plan->gen; // launch arguments changed
     return 0;
   }
   if (int rc = dalloc(plan, &plan->m_eps, (size_t)nb * sizeof(double), "field eps")) return rc;
@@ -1570,6 +1599,9 @@ int pfc_hip_pfc_set_members(pfc_hip_plan_t plan, const double *eps, const double
     PFC_HIP_TRY(hipMemcpy(plan->m_nl, c2f.data(), c2f.size() * sizeof(float),
                           hipMemcpyHostToDevice),
                 "pfc_hip_pfc_set_members: mean upload");
+  // eps and the means are read from device memory; the mean-split mode
+  // (and the switch to the member arrays) is a launch argument
+  if (!plan->members || plan->mean_split != (mean_split != 0)) ++plan->gen;
   plan->members = true;
   plan->mean_split = mean_split != 0;
   plan->mean = 0.0;
@@ -1584,7 +1616,14 @@ int pfc_hip_pfc_set_mean(pfc_hip_plan_t plan, int enable, double mean) {
   if (!std::isfinite(mean)) return fail(PFC_HIP_EINVAL, "pfc_hip_pfc_set_mean: mean not finite");
   plan->mean_split = enable != 0;
   plan->mean = enable ? mean : 0.0;
+  ++plan->gen;
   return 0;
 }
 
 } // extern "C"
+
+namespace pfc_amd {
+// Launch-argument generation of a plan (ensemble.hip re-captures its graph
+// when a group plan's generation moved since the capture).
+unsigned plan_generation(const pfc_hip_plan_s *p) { return p ? p->gen : 0u; }
+} // namespace pfc_amd

@@ -61,6 +61,8 @@ def lib():
         L.orc_ic_single_seed.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 9 + [_dp]
         L.orc_add_counter_noise.argtypes = [_dp, ctypes.c_size_t, ctypes.c_uint64,
                                             ctypes.c_double, ctypes.c_double]
+        L.orc_splitmix64.restype = ctypes.c_uint64
+        L.orc_splitmix64.argtypes = [ctypes.c_uint64]
         L.orc_ic_single_seed_masked.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 9
                                                 + [ctypes.c_int, _dp])
         L.orc_pfc_free_energy.restype = ctypes.c_double
@@ -258,6 +260,12 @@ def add_counter_noise(a: np.ndarray, seed: int, lo: float, hi: float) -> np.ndar
     assert a.dtype == np.float64 and a.flags.c_contiguous
     lib().orc_add_counter_noise(_p(a), a.size, ctypes.c_uint64(seed), lo, hi)
     return a
+
+
+def splitmix64(z: int) -> int:
+    """The oracle's splitmix64 mixing function (one output of the generator
+    whose state is z before its increment)."""
+    return int(lib().orc_splitmix64(ctypes.c_uint64(z & ((1 << 64) - 1))))
 
 
 def ic_single_seed_masked(a: np.ndarray, origin, spacing, rho, amp, radius, keep_outside):

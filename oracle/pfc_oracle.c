@@ -786,6 +786,10 @@ static uint64_t splitmix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
+/* The mixing function itself, exported so the tests can pin it against the
+ * published splitmix64 known answers. */
+uint64_t orc_splitmix64(uint64_t z) { return splitmix64(z); }
+
 void orc_add_counter_noise(double *psi, size_t n, uint64_t seed, double lo,
                            double hi) {
   const uint64_t s0 = splitmix64(seed);

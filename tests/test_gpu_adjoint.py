@@ -168,3 +168,52 @@ def test_mixed_trajectory_gradients_close_to_fp64_central_differences(torch_gpu)
           f"r rel {abs(ge - fdr) / abs(fdr):.2e}, forward rel L2 {fwd:.2e}")
     assert abs(fd - ad) <= 1e-4 * abs(fd), (fd, ad)
     assert abs(fdr - ge) <= 1e-4 * abs(fdr), (fdr, ge)
+
+
+def test_binomial_schedule_gradients_equal_uniform(torch_gpu):
+    """The binomial (revolve) schedule applies the same adjoint steps to the
+    same recomputed states as the uniform one: gradients bit for bit."""
+    from openpfc_amd.autograd import pfc_trajectory
+    torch = torch_gpu
+    n, nsteps = (32, 16, 16), 23
+    psi0_np, target_np = _fields(torch, n, 19)
+    target = torch.from_numpy(target_np).cuda()
+    out = []
+    for every, snaps in ((5, 0), (1, 0), (5, 1), (5, 3), (5, 6), (5, 40)):
+        print(f"schedule every={every} snaps={snaps}", flush=True)
+        psi0 = torch.from_numpy(psi0_np).cuda().requires_grad_(True)
+        eps = torch.tensor(-0.25, dtype=torch.float64, requires_grad=True)
+        psiT = pfc_trajectory(psi0, eps, nsteps, DT, (DX,) * 3, every=every, snaps=snaps)
+        J = ((psiT - target) ** 2).sum()
+        J.backward()
+        out.append((psiT.detach().clone(), psi0.grad.clone(), float(eps.grad.item())))
+    for T, g, ge in out[1:]:
+        assert torch.equal(T, out[0][0])
+        assert torch.equal(g, out[0][1]) and ge == out[0][2]
+
+
+def test_trajectory_on_a_side_stream_matches_default_stream(torch_gpu):
+    """Every library call of the autograd Function is enqueued on torch's
+    current stream, so work under torch.cuda.stream(s) (a non-blocking pool
+    stream) gives the default stream's results."""
+    from openpfc_amd.autograd import pfc_trajectory
+    torch = torch_gpu
+    n, nsteps = (64, 32, 32), 12
+    psi0_np, target_np = _fields(torch, n, 21)
+    res = []
+    for side in (False, True):
+        print(f"trajectory side stream={side}", flush=True)
+        s = torch.cuda.Stream() if side else torch.cuda.current_stream()
+        with torch.cuda.stream(s):
+            # inputs produced on this stream right before the call
+            psi0 = torch.from_numpy(psi0_np).cuda().mul_(1.0).requires_grad_(True)
+            target = torch.from_numpy(target_np).cuda().add_(0.0)
+            eps = torch.tensor(-0.3, dtype=torch.float64, requires_grad=True)
+            psiT = pfc_trajectory(psi0, eps, nsteps, DT, (DX,) * 3, every=5, snaps=2)
+            J = ((psiT - target) ** 2).sum()
+            J.backward()
+            g = psi0.grad.clone()
+        s.synchronize()
+        res.append((g, float(eps.grad.item()), float(J.item())))
+    assert torch.equal(res[0][0], res[1][0])
+    assert res[0][1] == res[1][1] and res[0][2] == res[1][2]

@@ -224,6 +224,55 @@ def test_ensemble_members_changed_between_replays(torch_gpu):
     ens.close()
 
 
+def test_ensemble_single_run_group_and_split_switch_between_replays(torch_gpu):
+    """A single-run group holds eps and the mean as launch arguments of its
+    kernels: changing them (and switching the mean-split mode) between graph
+    replays must reach the replays, not only the eager path."""
+    import openpfc_amd as P
+    from openpfc_amd import Ensemble
+    torch = torch_gpu
+    runs = _runs(16, sizes=[32, 32, 48])
+    psi0 = _initial(runs, 600)
+    ens = Ensemble(runs)
+    _load(ens, psi0, False)
+    ens.step(3, graph=True)  # captured with the original eps
+    plan, batch, off = ctypes.c_void_p(), ctypes.c_int(), ctypes.c_size_t()
+    P.check(ens._lib.pfc_hip_ensemble_group(ens._h, 1, ctypes.byref(plan), ctypes.byref(batch),
+                                            ctypes.byref(off)))
+    assert batch.value == 1
+    torch.cuda.synchronize()
+    new_eps = -0.31
+    print("single-run group: eps changed between replays", flush=True)
+    P.check(ens._lib.pfc_hip_pfc_set_members(plan, (ctypes.c_double * 1)(new_eps), None, 0))
+    ens.step(3, graph=True)
+    torch.cuda.synchronize()
+    r2 = P.Run(runs[2].ext, runs[2].spacing, new_eps, runs[2].dt)
+    mid = _oracle(r2, _oracle(runs[2], psi0[2], 3), 3)
+    err = _rel_max(ens.field(2).cpu().numpy(), mid)
+    assert err <= 1e-10, f"eps change not replayed: rel max err {err:.3e}"
+    # switch the single-run group to the mean-split step: the field then
+    # holds the deviation from its (conserved) mean
+    f2 = ens.field(2)
+    m = float(f2.mean().item())
+    f2.sub_(m)
+    torch.cuda.synchronize()
+    print("single-run group: mean-split switched on between replays", flush=True)
+    P.check(ens._lib.pfc_hip_pfc_set_members(plan, (ctypes.c_double * 1)(new_eps),
+                                             (ctypes.c_double * 1)(m), 1))
+    ens.step(3, graph=True)
+    torch.cuda.synchronize()
+    ref = _oracle(r2, mid, 3)
+    got = ens.field(2).cpu().numpy() + m
+    err = _rel_max(got, ref)
+    assert err <= 1e-10, f"mean-split switch not replayed: rel max err {err:.3e}"
+    assert abs(float(ens.field(2).mean().item())) <= 1e-12
+    # the two-run group was untouched throughout
+    for i in range(2):
+        err = _rel_max(ens.field(i).cpu().numpy(), _oracle(runs[i], psi0[i], 9))
+        assert err <= 1e-10, f"run {i}: rel max err {err:.3e}"
+    ens.close()
+
+
 def test_ensemble_autotune_keeps_fields_and_parity(torch_gpu):
     from openpfc_amd import Ensemble
     torch = torch_gpu
