@@ -216,18 +216,113 @@ def time_steps(torch, st, psi, steps, warmup):
     return e0.elapsed_time(e1) / steps, wall
 
 
-def cpu_baseline(n, dx, dt, r, nsteps):
-    """One (or nsteps) oracle step(s) on the same workload; returns steps/s."""
+def cpu_info() -> dict:
+    """Host the CPU baseline ran on: CPU model, cores, OpenMP setting."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "cores_available": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(n, dx, dt, r, nsteps, warmup=3, n_single=256):
+    """The oracle (the C restatement of the reference path) on this host, with
+    the method of apps/tungsten/src/tungsten_scalability.cpp:95-135: set up,
+    `warmup` untimed steps, then `nsteps` timed steps, with the share of the
+    time spent in transforms.  The first row uses every OpenMP thread on the
+    n^3 workload; the second is the reference's literal one-rank,
+    single-threaded run, sampled at n_single^3 so that it stays within
+    seconds (its ns per DOF-step carries over to n^3 up to the log n of the
+    FFT)."""
     import numpy as np
     from oracle import oracle as O
-    a = np.full((n, n, n), -0.2)
-    O.add_counter_noise(a, 1234, -0.01, 0.01)
-    O.ic_single_seed_masked(a, (-0.5 * n * dx,) * 3, (dx, dx, dx), -0.2, SEED_AMP, 64 * dx, 1)
-    opL, opN = O.pfc_operators((n, n, n), (dx, dx, dx), r, dt)
-    t0 = time.perf_counter()
-    O.pfc_step(a, opL, opN, nsteps)
-    dt_s = (time.perf_counter() - t0) / nsteps
-    return 1.0 / dt_s, O.num_threads()
+
+    def run(N, steps):
+        a = np.full((N, N, N), -0.2)
+        O.add_counter_noise(a, 1234, -0.01, 0.01)
+        O.ic_single_seed_masked(a, (-0.5 * N * dx,) * 3, (dx, dx, dx), -0.2, SEED_AMP,
+                                64 * dx, 1)
+        opL, opN = O.pfc_operators((N, N, N), (dx, dx, dx), r, dt)
+        O.pfc_step(a, opL, opN, warmup)
+        O.fft_seconds(reset=True)
+        t0 = time.perf_counter()
+        O.pfc_step(a, opL, opN, steps)
+        el = time.perf_counter() - t0
+        return steps / el, O.fft_seconds(reset=True) / el
+
+    threads = O.num_threads()
+    sps, share = run(n, nsteps)
+    O.set_num_threads(1)
+    try:
+        sps1, share1 = run(n_single, 1)
+    finally:
+        O.set_num_threads(threads)
+    return {"value": sps, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{nsteps} timed step(s) after {warmup} warm-up steps of the same "
+                      f"{n}^3 FP64 workload, oracle C restatement (own Stockham FFT, "
+                      f"OpenMP {threads} threads)",
+            "fft_share": share, "warmup_steps": warmup, **cpu_info(),
+            "single_thread": {"grid": n_single, "steps_per_s": sps1,
+                              "ns_per_dof_step": 1e9 / sps1 / n_single ** 3,
+                              "fft_share": share1,
+                              "sample": f"1 timed step after {warmup} warm-up steps at "
+                                        f"{n_single}^3, 1 thread (1 rank)"}}
+
+
+def cpp_app_leg(P, torch, n, dx, dt, r, steps, entry):
+    """The C++ drop-in path at n^3: the settings-file App (openpfc_amd_app,
+    PFCModelHIP under the Simulator) with the variants of `entry` handed over
+    through its autotune cache (OPENPFC_GPU_AUTOTUNE_MODE=cache_only, so a
+    cache the App cannot apply fails the leg).  Two runs of 3 and 3 + steps
+    steps; their difference over `steps` is the App's time per step without
+    its first step (which allocates the step buffers)."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    app = os.path.join(ROOT, "openpfc_amd", "host", "openpfc_amd_app")
+    d = tempfile.mkdtemp(prefix="pfc_app_")
+    try:
+        key = P.step_key("pfc", P.F64, (n, n, n))
+        with open(os.path.join(d, f"{P.device_id()}_autotune_cache.json"), "w") as fh:
+            json.dump({"device_id": P.device_id(), "entries": {key: entry}}, fh)
+        env = dict(os.environ, OPENPFC_GPU_AUTOTUNE_CACHE_DIR=d,
+                   OPENPFC_GPU_AUTOTUNE_MODE="cache_only")
+        totals = {}
+        for k in (3, 3 + steps):
+            cfg = {"model": {"name": "pfc", "params": {"epsilon": r}},
+                   "domain": {"Lx": n, "Ly": n, "Lz": n, "dx": dx, "dy": dx, "dz": dx,
+                              "origin": "center"},
+                   "timestepping": {"t0": 0.0, "t1": k * dt, "dt": dt, "saveat": k * dt},
+                   "initial_conditions": [
+                       {"target": "density", "type": "constant", "n0": -0.2},
+                       {"target": "density", "type": "uniform_noise", "seed": 1234,
+                        "low": -0.01, "high": 0.01}]}
+            path = os.path.join(d, f"run{k}.json")
+            with open(path, "w") as fh:
+                json.dump(cfg, fh)
+            log(f"c++ app leg: {k} steps")
+            res = subprocess.run([app, path], capture_output=True, text=True, timeout=600,
+                                 env=env)
+            if res.returncode != 0:
+                raise RuntimeError(f"openpfc_amd_app failed: {res.stdout}{res.stderr}")
+            m = re.search(r"\[app\] (\d+) steps in ([0-9.]+) s", res.stdout)
+            if not m or int(m.group(1)) != k:
+                raise RuntimeError(f"openpfc_amd_app output not understood: {res.stdout}")
+            totals[k] = float(m.group(2))
+        ms = (totals[3 + steps] - totals[3]) * 1e3 / steps
+        return {"ms_per_step": ms, "steps_per_s": 1e3 / ms, "steps": steps,
+                "method": f"openpfc_amd_app wall time of {3 + steps} minus 3 steps, "
+                          "variants from the Python leg via the autotune cache"}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 def stream_copy(P, torch, nbytes, reps=5):
@@ -644,11 +739,15 @@ def main(argv=None):
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpp", action="store_true",
+                    help="skip the C++ App leg (openpfc_amd_app at the same size)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the (untimed) launch-variant autotuner")
     ap.add_argument("--variants", help="JSON file {role: variant name} to force instead of "
                                        "autotuning (e.g. a previous run's choice)")
     ap.add_argument("--save-variants", help="write the chosen variants to this JSON file")
+    ap.add_argument("--variants-fp32", help="as --variants, for the FP32-mixed leg")
+    ap.add_argument("--save-variants-fp32", help="as --save-variants, for the FP32-mixed leg")
     ap.add_argument("--traffic", default=None,
                     help="traffic table from openpfc_amd.profile_summary (default: "
                          "profiles/traffic_pfc_step_<n>_f64.json when present)")
@@ -697,6 +796,7 @@ def main(argv=None):
     if args.save_variants:
         with open(args.save_variants, "w") as fh:
             json.dump(chosen64, fh, indent=1)
+    entry64 = P.cache_entry(fft)
     ms64, wall64 = time_steps(torch, st, psi, args.steps, args.warmup)
     kms = st.step_timed(psi, 3)
     kf = st.step_timed_fused(psi, max(args.steps, 2))
@@ -782,7 +882,12 @@ def main(argv=None):
         d = (psi0 - mean0).float()
         del psi0
         st.set_mean(mean0)
-        chosen32 = choose_variants(P, fft, st, d, args, "fp32")
+        args32 = argparse.Namespace(**vars(args))
+        args32.variants = args.variants_fp32
+        chosen32 = choose_variants(P, fft, st, d, args32, "fp32")
+        if args.save_variants_fp32:
+            with open(args.save_variants_fp32, "w") as fh:
+                json.dump(chosen32, fh, indent=1)
         ms32, _ = time_steps(torch, st, d, args.steps, args.warmup)
         k32 = st.step_timed(d, 3)
         k32f = st.step_timed_fused(d, max(args.steps, 2))
@@ -818,14 +923,16 @@ def main(argv=None):
     out["roofline"]["frac_of_stream_copy"] = out["roofline"]["achieved"] / sc["GBps"]
     out["step_frac_of_stream_copy"] = step_bytes / (ms64 * 1e-3) / 1e9 / sc["GBps"]
 
+    # ---- the C++ drop-in path (App over PFCModelHIP), same variants -------
+    if not args.no_cpp:
+        cpp = cpp_app_leg(P, torch, n, dx, dt, r, args.steps, entry64)
+        cpp["vs_value"] = ms64 / cpp["ms_per_step"]
+        out["cpp_app"] = cpp
+
     # ---- CPU baseline (oracle, rank 0, N=1) --------------------------------
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) start")
-        sps, threads = cpu_baseline(n, dx, dt, r, args.cpu_steps)
-        out["cpu_baseline"] = {
-            "value": sps, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"{args.cpu_steps} step(s) of the same {n}^3 FP64 workload, "
-                      f"oracle C restatement (own Stockham FFT, OpenMP {threads} threads)"}
+        out["cpu_baseline"] = cpu_baseline(n, dx, dt, r, args.cpu_steps)
 
     if args.calibrate_copy:
         out["copy_calibration"] = copy_calibration(torch)

@@ -281,6 +281,44 @@ int pfc_hip_bc_fixed(pfc_hip_plan_t plan, void *psi, double x0, double dx, doubl
                      double rho_high, double xwidth, double alpha, void *stream);
 
 /*
+ * The field of a list of rotated seeds on the device.  seeds (host memory)
+ * holds nseeds rows of 24 doubles: location (3), the six rotated wave
+ * vectors q_i (18), rho, radius, amplitude.  At X = origin + i*spacing the
+ * first seed with |X - location|^2 < radius^2 gives
+ * rho + sum_i 2*amplitude*cos(q_i . X) (X absolute); outside every seed the
+ * field becomes 0.0, or is left unchanged with keep_outside.  Replaces the
+ * field::apply of SeedGrid and RandomSeeds
+ * (include/openpfc/kernel/simulation/initial_conditions/seed_grid.hpp:108-118,
+ * random_seeds.hpp:80-89) over Seed::is_inside / Seed::get_value
+ * (initial_conditions/seed.hpp:124-141).  The host builds the list (the
+ * seeds' rotations and their mt19937_64 draws).
+ */
+int pfc_hip_ic_seeds(pfc_hip_plan_t plan, void *psi, const double *origin3,
+                     const double *spacing3, const double *seeds, int nseeds, int keep_outside,
+                     void *stream);
+
+/*
+ * Column profile of MovingBC: xline[i] (host memory, nx doubles) = the
+ * maximum of the field over every (y, z) at x index i.  Synchronises the
+ * stream.  Replaces the xline loop and MPI_Reduce(MPI_MAX) of
+ * MovingBC::apply at one rank
+ * (include/openpfc/kernel/simulation/boundary_conditions/moving_bc.hpp:120-140).
+ */
+int pfc_hip_column_max(pfc_hip_plan_t plan, const void *psi, double *xline, void *stream);
+
+/*
+ * MovingBC's band on the device: with l = nx*dx and p = fmod(xpos, l), every
+ * point whose x = x0 + i*dx lies within xwidth of p, or of p - l / p + l
+ * where the band wraps round the periodic box, becomes
+ * rho_low*S + rho_high*(1 - S), S = 1/(1 + exp(-alpha*d)), d the signed
+ * distance; other points are unchanged.  Replaces MovingBC::fill_bc
+ * (moving_bc.hpp:187-219); the host keeps the boundary position.
+ */
+int pfc_hip_bc_moving(pfc_hip_plan_t plan, void *psi, double x0, double dx, double xpos,
+                      double rho_low, double rho_high, double xwidth, double alpha,
+                      void *stream);
+
+/*
  * Operator tables of the fused step, written in the outbox layout (W = nx/2+1
  * pitch) to opL and opN (device, outbox real values each), scaled by
  * 1/(nx*ny*nz).  The step itself evaluates them per mode on the fly with the

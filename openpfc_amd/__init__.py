@@ -12,11 +12,12 @@ std::invalid_argument) before anything is enqueued.
 from __future__ import annotations
 
 import ctypes
+import os
 import time
 
 from . import _lib
-from ._lib import (F32, F64, MODEL_PFC, MODEL_TUNGSTEN, TUNGSTEN_NPARAMS, PfcHipError,
-                   check, load)
+from ._lib import (F32, F64, MODEL_PFC, MODEL_TUNGSTEN, PFC_HIP_OK, TUNGSTEN_NPARAMS,
+                   PfcHipError, check, load)
 from .ensemble import Ensemble, Run
 from .tungsten import TungstenParams
 
@@ -176,6 +177,46 @@ def ic_single_seed(fft: HipFft, psi, origin, spacing, rho: float, amplitude: flo
     check(fft._lib.pfc_hip_ic_single_seed(fft.handle, _ptr(psi), o, s, float(rho),
                                           float(amplitude), float(radius),
                                           int(bool(keep_outside)), _stream_ptr(stream)))
+
+
+def ic_seeds(fft: HipFft, psi, origin, spacing, seeds, keep_outside: bool = False,
+             stream=None) -> None:
+    """The field of a rotated-seed list on the device (the field::apply of
+    SeedGrid / RandomSeeds, initial_conditions/seed_grid.hpp:108-118).  seeds:
+    rows of 24 doubles (location, six wave vectors, rho, radius, amplitude);
+    the first seed holding a point gives its value, 0.0 outside every seed
+    (or the field kept with keep_outside)."""
+    import numpy as np
+    _check_field(fft, psi, "ic_seeds")
+    rows = np.ascontiguousarray(seeds, dtype=np.float64).reshape(-1, 24)
+    o = (ctypes.c_double * 3)(*origin)
+    s = (ctypes.c_double * 3)(*spacing)
+    check(fft._lib.pfc_hip_ic_seeds(fft.handle, _ptr(psi), o, s,
+                                    rows.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                    rows.shape[0], int(bool(keep_outside)),
+                                    _stream_ptr(stream)))
+
+
+def column_max(fft: HipFft, psi, stream=None):
+    """MovingBC's column profile: the maximum over (y, z) at every x index
+    (numpy array of nx doubles; synchronises the stream)."""
+    import numpy as np
+    _check_field(fft, psi, "column_max")
+    out = np.empty(fft.n[0])
+    check(fft._lib.pfc_hip_column_max(fft.handle, _ptr(psi),
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      _stream_ptr(stream)))
+    return out
+
+
+def bc_moving(fft: HipFft, psi, x0: float, dx: float, xpos: float, rho_low: float,
+              rho_high: float, xwidth: float = 15.0, alpha: float = 1.0, stream=None) -> None:
+    """MovingBC::fill_bc (boundary_conditions/moving_bc.hpp:187-219) on the
+    device: the sigmoid band of half-width xwidth round fmod(xpos, nx*dx)."""
+    _check_field(fft, psi, "bc_moving")
+    check(fft._lib.pfc_hip_bc_moving(fft.handle, _ptr(psi), float(x0), float(dx), float(xpos),
+                                     float(rho_low), float(rho_high), float(xwidth),
+                                     float(alpha), _stream_ptr(stream)))
 
 
 def ic_add_noise(fft: HipFft, psi, seed: int, lo: float, hi: float, stream=None) -> None:
@@ -357,3 +398,84 @@ class TungstenStepper(_Stepper):
         check(self.fft._lib.pfc_hip_tungsten_operators(self.fft.handle, _ptr(f), _ptr(e),
                                                        _ptr(n), _stream_ptr(stream)))
         return f, e, n
+
+
+# -- autotune cache (the file format of the C++ layer's autotune.hpp) --------
+def device_id(device: int = 0) -> str:
+    """'<arch>_<CUs>cu' of a device, e.g. 'gfx950_256cu': the cache file's prefix."""
+    import torch
+    p = torch.cuda.get_device_properties(device)
+    return p.gcnArchName.split(":")[0] + f"_{p.multi_processor_count}cu"
+
+
+def step_key(model: str, precision: int, n, mean_split: bool = False) -> str:
+    """Cache key of a step: model/precision/extents[/mean_split]."""
+    return (f"{model}/{'f64' if precision == F64 else 'f32'}/{n[0]}x{n[1]}x{n[2]}"
+            + ("/mean_split" if mean_split else ""))
+
+
+def autotune_cache_path(cache_dir: str | None = None, device: int = 0) -> str:
+    """<cache_dir>/<device>_autotune_cache.json; cache_dir defaults to
+    OPENPFC_GPU_AUTOTUNE_CACHE_DIR, else the working directory."""
+    d = cache_dir or os.environ.get("OPENPFC_GPU_AUTOTUNE_CACHE_DIR", ".")
+    return os.path.join(d, f"{device_id(device)}_autotune_cache.json")
+
+
+def cache_entry(fft: HipFft) -> dict:
+    """The plan's current variant name per step role (keys: role numbers)."""
+    out = {}
+    for role in range(64):
+        c = ctypes.c_int()
+        if fft._lib.pfc_hip_variant_count(fft._h, role, ctypes.byref(c)) != PFC_HIP_OK:
+            break
+        if c.value > 0:
+            out[str(role)] = fft.variants(role)[fft.get_variant(role)]
+    return out
+
+
+def store_cache_entry(fft: HipFft, key: str, cache_dir: str | None = None) -> str:
+    """Add the plan's variants under `key` to the cache file (atomically
+    replaced); returns the file's path."""
+    import json
+    path = autotune_cache_path(cache_dir)
+    try:
+        with open(path) as fh:
+            cache = json.load(fh)
+        if not isinstance(cache, dict):
+            cache = {}
+    except (OSError, ValueError):
+        cache = {}
+    cache["device_id"] = device_id()
+    cache.setdefault("entries", {})[key] = cache_entry(fft)
+    tmp = f"{path}.tmp{os.getpid()}"
+    with open(tmp, "w") as fh:
+        json.dump(cache, fh, indent=1)
+    os.replace(tmp, path)
+    return path
+
+
+def apply_cache_entry(fft: HipFft, key: str, cache_dir: str | None = None) -> bool:
+    """Select the cached variants of `key`; False (nothing changed) when the
+    file has no such entry or it names a variant the plan does not offer."""
+    import json
+    try:
+        with open(autotune_cache_path(cache_dir)) as fh:
+            entry = json.load(fh).get("entries", {}).get(key)
+    except (OSError, ValueError, AttributeError):
+        return False
+    if not isinstance(entry, dict):
+        return False
+    picks = []
+    for role in range(64):
+        c = ctypes.c_int()
+        if fft._lib.pfc_hip_variant_count(fft._h, role, ctypes.byref(c)) != PFC_HIP_OK:
+            break
+        if c.value <= 0:
+            continue
+        names = fft.variants(role)
+        if entry.get(str(role)) not in names:
+            return False
+        picks.append((role, names.index(entry[str(role)])))
+    for role, idx in picks:
+        fft.set_variant(role, idx)
+    return True

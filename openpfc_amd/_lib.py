@@ -100,6 +100,12 @@ SIGNATURES = {
     "pfc_hip_ensemble_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp]),
     "pfc_hip_pfc_free_energy": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_double), _vp]),
     "pfc_hip_stream_copy": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
+    "pfc_hip_ic_seeds": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                        ctypes.c_int, _vp]),
+    "pfc_hip_column_max": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_double), _vp]),
+    "pfc_hip_bc_moving": (ctypes.c_int, [_vp, _vp] + [ctypes.c_double] * 7 + [_vp]),
 }
 
 _lib = None

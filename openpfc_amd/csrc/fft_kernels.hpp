@@ -784,4 +784,92 @@ template <typename T> __global__ void k_ic_single_seed(T *psi, SeedArgs a) {
   }
 }
 
+// One rotated seed (initial_conditions/seed.hpp:124-141) painted over the
+// box [i0, i0+bx) x [j0, j0+by) x [k0, k0+bz) that holds its sphere: where
+// (X - location)^2 < radius^2 the value is rho + sum_i 2*amp*cos(q_i . X),
+// X the absolute coordinate.  A seed list is painted last seed first, so a
+// point inside several seeds ends with the first one's value, as the
+// reference's first-match loop gives.
+struct SeedBoxArgs {
+  double x0, y0, z0, dx, dy, dz;
+  double loc[3], q[18], rho, r2, amp;
+  int nx, ny, i0, j0, k0, bx, by;
+  long nbox; // bx*by*bz
+};
+
+template <typename T> __global__ void k_ic_seed_box(T *psi, SeedBoxArgs a) {
+#pragma clang fp contract(off)
+  for (long b = blockIdx.x * (long)blockDim.x + threadIdx.x; b < a.nbox;
+       b += (long)gridDim.x * blockDim.x) {
+    const int i = a.i0 + (int)(b % a.bx);
+    const long jk = b / a.bx;
+    const int j = a.j0 + (int)(jk % a.by);
+    const int k = a.k0 + (int)(jk / a.by);
+    const double X = (a.x0 + ((double)i * a.dx));
+    const double Y = (a.y0 + ((double)j * a.dy));
+    const double Z = (a.z0 + ((double)k * a.dz));
+    const double x = X - a.loc[0], y = Y - a.loc[1], z = Z - a.loc[2];
+    if (!((((x * x) + (y * y)) + (z * z)) < a.r2)) continue;
+    double u = a.rho;
+    const double two_amp = (2.0 * a.amp);
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      u = (u + (two_amp * cos((((a.q[3 * q] * X) + (a.q[3 * q + 1] * Y)) +
+                               (a.q[3 * q + 2] * Z)))));
+    psi[((long)k * a.ny + j) * a.nx + i] = (T)u;
+  }
+}
+
+// MovingBC's column profile (boundary_conditions/moving_bc.hpp:120-127):
+// the maximum of the field over (y, z) for every x.  Stage one: block
+// (bx, by) reduces rows by, by + gridDim.y, ... for the 256 columns of bx
+// into part[by*nx + x]; stage two (k_column_max_final, next to its launch)
+// reduces the gridDim.y partials per column.  max is exact, so the result
+// does not depend on the split.
+template <typename T>
+__global__ void __launch_bounds__(256) k_column_max_part(const T *__restrict__ psi, int nx,
+                                                         long rows, double *__restrict__ part) {
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  if (x >= nx) return;
+  double m = -1.7976931348623157e308; // numeric_limits<double>::lowest()
+  for (long r = blockIdx.y; r < rows; r += gridDim.y) {
+    const double v = (double)psi[r * nx + x];
+    m = v > m ? v : m;
+  }
+  part[(long)blockIdx.y * nx + x] = m;
+}
+
+// MovingBC::fill_bc (moving_bc.hpp:187-219): with l = nx*dx and
+// p = fmod(xpos, l), columns whose x = x0 + i*dx lies within xwidth of p (or
+// of p -+ l where the band wraps round the periodic box) become
+// rho_low*S + rho_high*(1-S), S = 1/(1 + exp(-alpha*d)); the launch covers
+// the columns [i0, i0+bx) that can be in a band, every other value is kept.
+struct MovingBcArgs {
+  double x0, dx, xpos, l, xwidth, alpha, rho_low, rho_high;
+  int nx, i0, bx;
+  long nbox; // bx*ny*nz
+};
+
+template <typename T> __global__ void k_bc_moving(T *psi, MovingBcArgs a) {
+#pragma clang fp contract(off)
+  for (long b = blockIdx.x * (long)blockDim.x + threadIdx.x; b < a.nbox;
+       b += (long)gridDim.x * blockDim.x) {
+    const int i = a.i0 + (int)(b % a.bx);
+    const long row = b / a.bx;
+    const double x = (a.x0 + ((double)i * a.dx));
+    const double dist = x - a.xpos;
+    double d;
+    if (fabs(dist) < a.xwidth)
+      d = dist;
+    else if (a.xpos < a.xwidth && fabs(dist - a.l) < a.xwidth)
+      d = dist - a.l;
+    else if (a.xpos > a.l - a.xwidth && fabs(dist + a.l) < a.xwidth)
+      d = dist + a.l;
+    else
+      continue;
+    const double S = 1.0 / (1.0 + exp((-a.alpha) * d));
+    psi[row * a.nx + i] = (T)((a.rho_low * S) + (a.rho_high * (1.0 - S)));
+  }
+}
+
 } // namespace pfc_amd

@@ -114,6 +114,8 @@ struct pfc_hip_plan_s {
   // adjoint workspace: three outbox spectra, two real fields, block partials
   void *adj_lam = nullptr, *adj_P = nullptr, *adj_N = nullptr;
   void *adj_a = nullptr, *adj_b = nullptr, *adj_part = nullptr;
+  // column maxima (MovingBC): kColParts partial rows of nx, then the result
+  void *colmax = nullptr;
   bool ops_ready = false;
   int model_kind = -1;
   double dt = 0;
@@ -240,7 +242,7 @@ void free_all(pfc_hip_plan_s *p) {
   void **bufs[] = {&p->wn,      &p->work,  &p->A,     &p->B,     &p->A1,
                    &p->B1,      &p->A2,    &p->B2,    &p->adj_lam, &p->adj_P,
                    &p->adj_N,   &p->adj_a, &p->adj_b, &p->adj_part, &p->m_eps,
-                   &p->m_nl};
+                   &p->m_nl,    &p->colmax};
   for (void **b : bufs) {
     if (*b) (void)hipFree(*b);
     *b = nullptr;
@@ -687,6 +689,23 @@ constexpr size_t kPartials = 2 * kAdjBlocks + 2;
 // Adjoint of one plain PFC step (see adjoint_kernels.hpp).
 template <typename T>
 int adjoint_impl(pfc_hip_plan_s *p, const void *psi, void *lambda, double *dr, hipStream_t s);
+
+// partial rows of the column maxima (pfc_hip_column_max)
+constexpr long kColParts = 256;
+
+// second stage of the column maxima: the maximum of the nparts partials
+__global__ void __launch_bounds__(256) k_column_max_final(const double *__restrict__ part,
+                                                          int nx, int nparts,
+                                                          double *__restrict__ out) {
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  if (x >= nx) return;
+  double m = part[x];
+  for (int p = 1; p < nparts; ++p) {
+    const double v = part[(long)p * nx + x];
+    m = v > m ? v : m;
+  }
+  out[x] = m;
+}
 
 unsigned ic_blocks(long n) {
   return (unsigned)std::min<long>((n + 255) / 256, 1L << 20);
@@ -1298,6 +1317,162 @@ int pfc_hip_bc_fixed(pfc_hip_plan_t plan, void *psi, double x0, double dx, doubl
     hipLaunchKernelGGL(k_bc_fixed<float>, dim3(ic_blocks(n)), dim3(256), 0, s,
                        (float *)psi, a);
   PFC_HIP_TRY(hipGetLastError(), "bc_fixed launch");
+  return 0;
+}
+
+int pfc_hip_ic_seeds(pfc_hip_plan_t plan, void *psi, const double *origin3,
+                     const double *spacing3, const double *seeds, int nseeds, int keep_outside,
+                     void *stream) {
+  if (int rc = check_single(plan, "pfc_hip_ic_seeds")) return rc;
+  if (!psi || !origin3 || !spacing3 || (nseeds > 0 && !seeds))
+    return fail(PFC_HIP_EINVAL, "pfc_hip_ic_seeds: null argument");
+  if (nseeds < 0) return fail(PFC_HIP_EINVAL, "pfc_hip_ic_seeds: nseeds < 0");
+  for (int d = 0; d < 3; ++d)
+    if (!std::isfinite(origin3[d]) || !(spacing3[d] > 0) || !std::isfinite(spacing3[d]))
+      return fail(PFC_HIP_EINVAL, "pfc_hip_ic_seeds: non-finite origin or spacing");
+  for (long v = 0; v < 24L * nseeds; ++v)
+    if (!std::isfinite(seeds[v]))
+      return fail(PFC_HIP_EINVAL, "pfc_hip_ic_seeds: non-finite seed value");
+  if (int rc = set_device(plan)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const bool f64 = plan->prec == PFC_HIP_F64;
+  if (!keep_outside) {
+    const long n = (long)plan->inbox();
+    if (f64)
+      hipLaunchKernelGGL(k_ic_constant<double>, dim3(ic_blocks(n)), dim3(256), 0, s,
+                         (double *)psi, n, 0.0);
+    else
+      hipLaunchKernelGGL(k_ic_constant<float>, dim3(ic_blocks(n)), dim3(256), 0, s,
+                         (float *)psi, n, 0.0);
+    PFC_HIP_TRY(hipGetLastError(), "ic_seeds clear");
+  }
+  const int ext[3] = {plan->nx, plan->ny, plan->nz};
+  // last seed first: the first seed holding a point writes it last
+  for (int sd = nseeds - 1; sd >= 0; --sd) {
+    const double *v = seeds + 24L * sd;
+    SeedBoxArgs a{};
+    a.x0 = origin3[0];
+    a.y0 = origin3[1];
+    a.z0 = origin3[2];
+    a.dx = spacing3[0];
+    a.dy = spacing3[1];
+    a.dz = spacing3[2];
+    for (int d = 0; d < 3; ++d) a.loc[d] = v[d];
+    for (int q = 0; q < 18; ++q) a.q[q] = v[3 + q];
+    a.rho = v[21];
+    a.r2 = v[22] * v[22];
+    a.amp = v[23];
+    a.nx = plan->nx;
+    a.ny = plan->ny;
+    // index box of the sphere, one point of margin each side, clipped
+    int lo[3], hi[3];
+    bool empty = false;
+    for (int d = 0; d < 3; ++d) {
+      const double r = std::fabs(v[22]);
+      const double f0 = std::floor((v[d] - r - origin3[d]) / spacing3[d]) - 1.0;
+      const double f1 = std::ceil((v[d] + r - origin3[d]) / spacing3[d]) + 1.0;
+      lo[d] = (int)std::max(0.0, std::min(f0, (double)ext[d]));
+      hi[d] = (int)std::min((double)ext[d] - 1.0, std::max(f1, -1.0));
+      if (hi[d] < lo[d]) empty = true;
+    }
+    if (empty) continue;
+    a.i0 = lo[0];
+    a.j0 = lo[1];
+    a.k0 = lo[2];
+    a.bx = hi[0] - lo[0] + 1;
+    a.by = hi[1] - lo[1] + 1;
+    a.nbox = (long)a.bx * a.by * (hi[2] - lo[2] + 1);
+    if (f64)
+      hipLaunchKernelGGL(k_ic_seed_box<double>, dim3(ic_blocks(a.nbox)), dim3(256), 0, s,
+                         (double *)psi, a);
+    else
+      hipLaunchKernelGGL(k_ic_seed_box<float>, dim3(ic_blocks(a.nbox)), dim3(256), 0, s,
+                         (float *)psi, a);
+    PFC_HIP_TRY(hipGetLastError(), "ic_seeds launch");
+  }
+  return 0;
+}
+
+int pfc_hip_column_max(pfc_hip_plan_t plan, const void *psi, double *xline, void *stream) {
+  if (int rc = check_single(plan, "pfc_hip_column_max")) return rc;
+  if (!psi || !xline) return fail(PFC_HIP_EINVAL, "pfc_hip_column_max: null argument");
+  if (int rc = set_device(plan)) return rc;
+  const int nx = plan->nx;
+  const long rows = (long)plan->ny * plan->nz;
+  const int parts = (int)std::min<long>(rows, kColParts);
+  if (int rc = dalloc(plan, &plan->colmax, ((size_t)kColParts + 1) * nx * sizeof(double),
+                      "column maxima"))
+    return rc;
+  double *part = (double *)plan->colmax;
+  double *out = part + (size_t)kColParts * nx;
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned gx = (unsigned)((nx + 255) / 256);
+  if (plan->prec == PFC_HIP_F64)
+    hipLaunchKernelGGL(k_column_max_part<double>, dim3(gx, (unsigned)parts), dim3(256), 0, s,
+                       (const double *)psi, nx, rows, part);
+  else
+    hipLaunchKernelGGL(k_column_max_part<float>, dim3(gx, (unsigned)parts), dim3(256), 0, s,
+                       (const float *)psi, nx, rows, part);
+  PFC_HIP_TRY(hipGetLastError(), "column max (partials)");
+  hipLaunchKernelGGL(k_column_max_final, dim3(gx), dim3(256), 0, s, (const double *)part, nx,
+                     parts, out);
+  PFC_HIP_TRY(hipGetLastError(), "column max (final)");
+  PFC_HIP_TRY(hipMemcpyAsync(xline, out, (size_t)nx * sizeof(double), hipMemcpyDeviceToHost, s),
+              "column max copy");
+  PFC_HIP_TRY(hipStreamSynchronize(s), "column max sync");
+  return 0;
+}
+
+int pfc_hip_bc_moving(pfc_hip_plan_t plan, void *psi, double x0, double dx, double xpos,
+                      double rho_low, double rho_high, double xwidth, double alpha,
+                      void *stream) {
+  if (int rc = check_single(plan, "pfc_hip_bc_moving")) return rc;
+  if (!psi) return fail(PFC_HIP_EINVAL, "pfc_hip_bc_moving: null psi");
+  if (!std::isfinite(x0) || !std::isfinite(dx) || !(dx > 0) || !std::isfinite(xpos) ||
+      !std::isfinite(xwidth) || !std::isfinite(alpha))
+    return fail(PFC_HIP_EINVAL, "pfc_hip_bc_moving: non-finite geometry");
+  if (int rc = set_device(plan)) return rc;
+  MovingBcArgs a{};
+  a.x0 = x0;
+  a.dx = dx;
+  a.xwidth = xwidth;
+  a.alpha = alpha;
+  a.rho_low = rho_low;
+  a.rho_high = rho_high;
+  a.nx = plan->nx;
+  {
+#pragma clang fp contract(off)
+    const double Lx = (double)plan->nx;
+    a.l = Lx * dx;                 // moving_bc.hpp: l = Lx * dx
+    a.xpos = std::fmod(xpos, a.l); // xpos = fmod(m_xpos, l)
+  }
+  // the columns in a band, by the kernel's own test (host and device round
+  // these few operations identically)
+  int first = -1, last = -1;
+  for (int i = 0; i < plan->nx; ++i) {
+#pragma clang fp contract(off)
+    const double x = (x0 + ((double)i * dx));
+    const double dist = x - a.xpos;
+    const bool in = std::fabs(dist) < xwidth ||
+                    (a.xpos < xwidth && std::fabs(dist - a.l) < xwidth) ||
+                    (a.xpos > a.l - xwidth && std::fabs(dist + a.l) < xwidth);
+    if (in) {
+      if (first < 0) first = i;
+      last = i;
+    }
+  }
+  if (first < 0) return 0;
+  a.i0 = first;
+  a.bx = last - first + 1;
+  a.nbox = (long)a.bx * plan->ny * plan->nz;
+  hipStream_t s = (hipStream_t)stream;
+  if (plan->prec == PFC_HIP_F64)
+    hipLaunchKernelGGL(k_bc_moving<double>, dim3(ic_blocks(a.nbox)), dim3(256), 0, s,
+                       (double *)psi, a);
+  else
+    hipLaunchKernelGGL(k_bc_moving<float>, dim3(ic_blocks(a.nbox)), dim3(256), 0, s,
+                       (float *)psi, a);
+  PFC_HIP_TRY(hipGetLastError(), "bc_moving launch");
   return 0;
 }
 

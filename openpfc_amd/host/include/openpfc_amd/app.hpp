@@ -8,7 +8,8 @@
 // the model (constructed from the FFT and the World, "model.params" applied
 // as the reference's from_json does), the initial and boundary conditions
 // ("initial_conditions" / "boundary_conditions": constant, single_seed,
-// from_file, uniform_noise, fixed; field_modifier_registry.hpp:108-118), the
+// seed_grid, random_seeds, from_file, uniform_noise, fixed, moving;
+// field_modifier_registry.hpp:108-118), the
 // results writers ("fields": [{name, data}] written with BinaryWriter at
 // "saveat", simulation_wiring_writers.hpp) and the "simulator" section
 // (result_counter, increment: simulation_wiring_simulator_section.hpp), then
@@ -141,6 +142,15 @@ inline double num(const Value &j, const std::string &key, const std::string &ctx
   return v->as_double();
 }
 
+// A required numeric field of a modifier, with the reference's message
+// ("Invalid JSON input: missing or invalid 'X0' field.").
+inline const Value &required_number(const Value &j, const std::string &key) {
+  const Value *v = j.find(key);
+  if (!v || !v->is_number())
+    throw std::invalid_argument("Invalid JSON input: missing or invalid '" + key + "' field.");
+  return *v;
+}
+
 // Field modifiers by "type" (field_modifier_registry.hpp:108-118,
 // from_json_field_modifiers.hpp).
 inline std::unique_ptr<FieldModifier> field_modifier_from_json(const Value &j) {
@@ -170,10 +180,36 @@ inline std::unique_ptr<FieldModifier> field_modifier_from_json(const Value &j) {
                                        num(j, "high", "uniform_noise"));
   } else if (type == "fixed") {
     m = std::make_unique<FixedBC>(num(j, "rho_low", "fixed"), num(j, "rho_high", "fixed"));
+  } else if (type == "seed_grid") {
+    // from_json_field_modifiers.hpp:85-125
+    auto g = std::make_unique<SeedGrid>();
+    g->set_Ny((int)required_number(j, "Ny").as_double());
+    g->set_Nz((int)required_number(j, "Nz").as_double());
+    g->set_X0(required_number(j, "X0").as_double());
+    g->set_radius(required_number(j, "radius").as_double());
+    g->set_amplitude(required_number(j, "amplitude").as_double());
+    g->set_density(required_number(j, "rho").as_double());
+    m = std::move(g);
+  } else if (type == "random_seeds") {
+    // from_json_field_modifiers.hpp:65-83
+    auto r = std::make_unique<RandomSeeds>();
+    r->set_amplitude(required_number(j, "amplitude").as_double());
+    r->set_density(required_number(j, "rho").as_double());
+    m = std::move(r);
+  } else if (type == "moving") {
+    // from_json_field_modifiers.hpp:157-197 (threshold keeps its default 0.1)
+    auto b = std::make_unique<MovingBC>();
+    b->set_rho_low(required_number(j, "rho_low").as_double());
+    b->set_rho_high(required_number(j, "rho_high").as_double());
+    b->set_xwidth(required_number(j, "width").as_double());
+    b->set_alpha(required_number(j, "alpha").as_double());
+    b->set_disp(required_number(j, "disp").as_double());
+    b->set_xpos(required_number(j, "xpos").as_double());
+    m = std::move(b);
   } else {
     throw std::invalid_argument("unknown field modifier type '" + type +
-                                "' (known: constant, single_seed, from_file, uniform_noise, "
-                                "fixed)");
+                                "' (known: constant, single_seed, seed_grid, random_seeds, "
+                                "from_file, uniform_noise, fixed, moving)");
   }
   if (const Value *tg = j.find("target")) m->set_field_name(tg->as_string());
   return m;

@@ -21,6 +21,7 @@
 #pragma once
 
 #include "core.hpp"
+#include "decomposition.hpp"
 #include "world.hpp"
 
 #include <chrono>
@@ -32,12 +33,6 @@
 #include <vector>
 
 namespace pfc {
-
-struct Box3i {
-  Int3 low{0, 0, 0};
-  Int3 high{0, 0, 0};
-  Int3 size{0, 0, 0};
-};
 
 namespace fft {
 
@@ -214,12 +209,72 @@ inline HipFft create_hip(const World &w, int precision = PFC_HIP_F64) {
   return HipFft(w, precision);
 }
 
+namespace detail {
+// The HipFft behind an IFFT& (device-resident models need the library plan).
+inline HipFft &as_hip_fft(IFFT &f, const char *who) {
+  if (auto *h = dynamic_cast<HipFft *>(&f)) return *h;
+  throw std::invalid_argument(std::string(who) + ": the FFT must be a pfc::fft::HipFft "
+                              "(fft::create_hip or create_with_backend(..., Backend::HIP))");
+}
+
+// The one part a plan of this library covers: part rank_id of a one-part
+// decomposition on a one-rank communicator.
+inline void require_single_part(const Decomposition &d, int rank_id, Comm comm,
+                                const char *who) {
+  pfc::detail::require_one_rank(comm, who);
+  if (decomposition::get_num_domains(d) != 1)
+    throw std::logic_error(std::string(who) + ": the decomposition has " +
+                           std::to_string(decomposition::get_num_domains(d)) +
+                           " parts; this build plans one part per GPU");
+  if (rank_id != 0)
+    throw std::invalid_argument(std::string(who) + ": rank_id " + std::to_string(rank_id) +
+                                " outside a one-part decomposition");
+}
+} // namespace detail
+
+// create_hip(const Decomposition&, int rank_id, MPI_Comm)
+// (runtime/hip/fft_hip.hpp:65-66): the HIP FFT of part rank_id.
+[[nodiscard]] inline HipFft create_hip(const Decomposition &d, int rank_id,
+                                       MPI_Comm comm = MPI_COMM_WORLD) {
+  detail::require_single_part(d, rank_id, comm, "fft::create_hip");
+  return HipFft(decomposition::get_world(d));
+}
+
+// create_hip(const Decomposition&, MPI_Comm) (fft_hip.hpp:77-78): the rank
+// is taken from the communicator, whose size must match the decomposition.
+[[nodiscard]] inline HipFft create_hip(const Decomposition &d, MPI_Comm comm = MPI_COMM_WORLD) {
+  return create_hip(d, 0, comm);
+}
+
 // create_with_backend (src/openpfc/runtime/cpu/fft.cpp:131-191): this build
 // provides the HIP backend; FFTW and CUDA requests are rejected as the
 // reference rejects backends it was not built with.
-inline std::unique_ptr<IFFT> create_with_backend(const World &w, Backend b) {
+[[nodiscard]] inline std::unique_ptr<IFFT> create_with_backend(const World &w, Backend b) {
   if (b != Backend::HIP) throw std::runtime_error("Unsupported FFT backend requested");
   return std::make_unique<HipFft>(w);
+}
+
+// create_with_backend(const FFTLayout&, int rank_id, const plan_options&,
+// Backend, MPI_Comm) (kernel/fft/fft.hpp:52-55; fft.cpp:131-166): the real and
+// complex boxes of part rank_id must be the whole grid and its half spectrum.
+[[nodiscard]] inline std::unique_ptr<IFFT>
+create_with_backend(const FFTLayout &layout, int rank_id, const plan_options &options,
+                    Backend backend, MPI_Comm comm = MPI_COMM_WORLD) {
+  (void)options; // reshape choices of a distributed plan
+  if (backend != Backend::HIP) throw std::runtime_error("Unsupported FFT backend requested");
+  detail::require_single_part(layout.m_decomposition, rank_id, comm,
+                              "fft::create_with_backend");
+  return std::make_unique<HipFft>(decomposition::get_world(layout.m_decomposition));
+}
+
+// create_with_backend(const Decomposition&, int rank_id, Backend, MPI_Comm)
+// (kernel/fft/fft.hpp:56-57; fft.cpp:168-191): layout with r2c_direction 0
+// and the backend's default options.
+[[nodiscard]] inline std::unique_ptr<IFFT>
+create_with_backend(const Decomposition &d, int rank_id, Backend backend,
+                    MPI_Comm comm = MPI_COMM_WORLD) {
+  if (backend != Backend::HIP) throw std::runtime_error("Unsupported FFT backend requested");
+  return create_with_backend(layout::create(d, 0), rank_id, default_options(), backend, comm);
 }
 
 // Backend names of the JSON configs ("hip" / "rocm"; runtime/common/backend_from_string.hpp:24-40).

@@ -30,7 +30,11 @@ struct DeviceField {
 
 class Model {
 public:
-  Model(fft::IFFT &fft, const World &world) : m_fft(fft), m_world(world) {}
+  // comm: the reference's defaulted MPI_Comm (model.hpp:151); one rank here.
+  Model(fft::IFFT &fft, const World &world, MPI_Comm comm = MPI_COMM_WORLD)
+      : m_fft(fft), m_world(world), m_comm(comm) {
+    pfc::detail::require_one_rank(comm, "Model");
+  }
   virtual ~Model() = default;
   Model(const Model &) = delete;
   Model &operator=(const Model &) = delete;
@@ -48,6 +52,7 @@ public:
   virtual bool get_device_field(const std::string &, DeviceField &) { return false; }
   virtual void device_field_modified(const std::string &) {}
 
+  [[nodiscard]] MPI_Comm mpi_comm() const noexcept { return m_comm; }
   fft::IFFT &get_fft() noexcept { return m_fft; }
   const fft::IFFT &get_fft() const noexcept { return m_fft; }
   const World &get_world() const noexcept { return m_world; }
@@ -64,6 +69,7 @@ public:
 private:
   fft::IFFT &m_fft;
   World m_world;
+  MPI_Comm m_comm;
   std::map<std::string, RealField *> m_real;
 };
 

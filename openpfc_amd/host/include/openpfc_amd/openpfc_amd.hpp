@@ -1,7 +1,10 @@
 // openpfc_amd.hpp -- umbrella header of the C++ host layer.
 #pragma once
 
+#include "autotune.hpp"
+#include "comm.hpp"
 #include "core.hpp"
+#include "decomposition.hpp"
 #include "fft.hpp"
 #include "field_modifier.hpp"
 #include "io.hpp"

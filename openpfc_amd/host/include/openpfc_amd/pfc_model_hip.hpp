@@ -3,21 +3,26 @@
 // Drop-in for PFCModel of docs/api/examples/10_complete_pfc_model.cpp:62-214:
 // same constructor arguments (FFT, domain, epsilon), initialize(dt) builds
 // opL = exp(L dt), opN = (opL-1)/L*(-k^2) with L = -k^2(eps + (1-k^2)^2)
-// (evaluated per mode on the device), step() advances psi by one ETD1 step
-// psi_hat <- opL*F(psi) + opN*F(psi^3), psi <- F^-1(psi_hat).
+// (evaluated per mode on the device) and picks the step's launch variants
+// (autotune cache, or tuned on the spot), step() advances psi by one ETD1
+// step psi_hat <- opL*F(psi) + opN*F(psi^3), psi <- F^-1(psi_hat).
 // psi lives on the device; the registered host field "density" (and
 // "default") is a mirror refreshed by the residency hooks, so field
 // modifiers, writers and get_real_field() keep their reference meaning.
 #pragma once
 
+#include "autotune.hpp"
 #include "model.hpp"
 
 namespace pfc {
 
 class PFCModelHIP : public Model {
 public:
-  PFCModelHIP(fft::HipFft &fft, const World &world, double epsilon)
-      : Model(fft, world), m_hfft(fft), m_epsilon(epsilon) {}
+  // fft must be a HipFft (any IFFT& the HIP factories return)
+  PFCModelHIP(fft::IFFT &fft, const World &world, double epsilon,
+              MPI_Comm comm = MPI_COMM_WORLD)
+      : Model(fft, world, comm), m_hfft(fft::detail::as_hip_fft(fft, "PFCModelHIP")),
+        m_epsilon(epsilon) {}
 
   void initialize(double dt) override {
     m_dt = dt;
@@ -32,7 +37,13 @@ public:
         "PFCModelHIP::initialize");
     m_host_stale = false;
     m_psi.copy_from_host(m_density);
+    // step variants from the autotune cache, or tuned now (autotune.hpp)
+    m_tuned = gpu::tune_step(m_hfft.plan(), m_psi.data(),
+                             gpu::step_key("pfc", PFC_HIP_F64, w.size, false), m_hfft.stream());
   }
+
+  // How initialize() chose the step's launch variants.
+  gpu::TuneOutcome autotune_outcome() const noexcept { return m_tuned; }
 
   void step(double) override {
     pfc::detail::check_abi(pfc_hip_pfc_step(m_hfft.plan(), m_psi.data(), 1, m_hfft.stream()),
@@ -91,6 +102,7 @@ private:
   RealField m_density;
   core::DataBuffer<backend::HipTag, double> m_psi;
   bool m_host_stale = false;
+  gpu::TuneOutcome m_tuned = gpu::TuneOutcome::defaults;
 };
 
 } // namespace pfc

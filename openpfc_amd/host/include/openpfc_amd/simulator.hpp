@@ -20,7 +20,16 @@ namespace pfc {
 
 class Simulator {
 public:
-  Simulator(Model &m, Time &t) : m_model(m), m_time(t) {}
+  // comm: the reference's defaulted MPI_Comm (simulator.hpp:152); one rank here.
+  Simulator(Model &m, Time &t, MPI_Comm comm = MPI_COMM_WORLD)
+      : m_model(m), m_time(t), m_comm(comm) {
+    pfc::detail::require_one_rank(comm, "Simulator");
+  }
+  void set_mpi_comm(MPI_Comm comm) {
+    pfc::detail::require_one_rank(comm, "Simulator");
+    m_comm = comm;
+  }
+  [[nodiscard]] MPI_Comm mpi_comm() const noexcept { return m_comm; }
 
   void add_initial_conditions(std::unique_ptr<FieldModifier> ic) { m_ics.push_back(std::move(ic)); }
   void add_boundary_conditions(std::unique_ptr<FieldModifier> bc) { m_bcs.push_back(std::move(bc)); }
@@ -85,6 +94,7 @@ private:
 
   Model &m_model;
   Time &m_time;
+  MPI_Comm m_comm;
   std::vector<std::unique_ptr<FieldModifier>> m_ics, m_bcs;
   std::vector<std::function<void(Model &, const Time &, int)>> m_writers;
   int m_result_counter = 0;

@@ -15,6 +15,7 @@
 // field "psi" is a mirror refreshed by the residency hooks.
 #pragma once
 
+#include "autotune.hpp"
 #include "model.hpp"
 
 #include <cmath>
@@ -91,9 +92,13 @@ template <typename RealType = double> class TungstenHIP : public Model {
 public:
   TungstenParams params;
 
-  TungstenHIP(fft::HipFft &fft, const World &world) : Model(fft, world), m_hfft(fft) {
+  // TungstenHIP(fft::IFFT&, const World&, MPI_Comm = MPI_COMM_WORLD)
+  // (apps/tungsten/include/tungsten/hip/tungsten_model.hpp:131-136); fft must
+  // be a HipFft of RealType's precision.
+  explicit TungstenHIP(fft::IFFT &fft, const World &world, MPI_Comm comm = MPI_COMM_WORLD)
+      : Model(fft, world, comm), m_hfft(fft::detail::as_hip_fft(fft, "TungstenHIP")) {
     const int want = std::is_same_v<RealType, double> ? PFC_HIP_F64 : PFC_HIP_F32;
-    if (fft.precision() != want)
+    if (m_hfft.precision() != want)
       throw std::invalid_argument("TungstenHIP: the HipFft plan precision does not "
                                   "match RealType");
   }
@@ -110,7 +115,16 @@ public:
                                                      p, PFC_HIP_TUNGSTEN_NPARAMS, dt),
                            "TungstenHIP::initialize");
     finalize_after_field_modifiers();
+    // step variants from the autotune cache, or tuned now (autotune.hpp)
+    m_tuned = gpu::tune_step(
+        m_hfft.plan(), m_psi.data(),
+        gpu::step_key("tungsten", std::is_same_v<RealType, double> ? PFC_HIP_F64 : PFC_HIP_F32,
+                      get_world().size, false),
+        m_hfft.stream());
   }
+
+  // How initialize() chose the step's launch variants.
+  gpu::TuneOutcome autotune_outcome() const noexcept { return m_tuned; }
 
   void step(double) override { step_n(1); }
 
@@ -165,6 +179,7 @@ private:
   core::DataBuffer<backend::HipTag, RealType> m_psi;
   core::DataBuffer<backend::HipTag, RealType> m_psiMF;
   bool m_host_stale = false;
+  gpu::TuneOutcome m_tuned = gpu::TuneOutcome::defaults;
 };
 
 } // namespace pfc

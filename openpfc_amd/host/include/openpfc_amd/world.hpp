@@ -16,6 +16,14 @@ struct GridSize { Int3 value; explicit GridSize(Int3 v) : value(v) {} };
 struct PhysicalOrigin { Real3 value; explicit PhysicalOrigin(Real3 v) : value(v) {} };
 struct GridSpacing { Real3 value; explicit GridSpacing(Real3 v) : value(v) {} };
 
+// Index box {low, high, size} (inclusive bounds), the reference's Box3i
+// (include/openpfc/kernel/data/box3i.hpp).
+struct Box3i {
+  Int3 low{0, 0, 0};
+  Int3 high{0, 0, 0};
+  Int3 size{0, 0, 0};
+};
+
 struct World {
   Int3 size{1, 1, 1};
   Real3 origin{0, 0, 0};

@@ -11,6 +11,11 @@
 //                                      CPU-vs-HIP recipe (32^3, spacing 1,
 //                                      T = 0.5, dt = 0.01, -0.4 + 0.1 sin(2 pi i/n));
 //                                      writes psi and psi_MF (raw f64)
+//   test_host seeds DIR                host SeedGrid / RandomSeeds / MovingBC
+//                                      (see run_seeds)
+//   test_host gpu-callsites N STEPS    the reference's factory and constructor
+//                                      signatures (see run_gpu_callsites)
+//   test_host gpu-autotune DIR         the autotune cache (see run_gpu_autotune)
 #include <openpfc_amd/app.hpp>
 #include <openpfc_amd/openpfc_amd.hpp>
 
@@ -19,6 +24,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <functional>
 #include <stdexcept>
@@ -193,6 +199,85 @@ static int run_cpu() {
   CHECK(fft::backend_from_string("hip") == fft::Backend::HIP);
   CHECK(fft::backend_from_string("rocm") == fft::Backend::HIP);
   CHECK(throws<std::invalid_argument>([] { fft::backend_from_string("opencl"); }));
+  // Decomposition / FFTLayout / factories with the reference's signatures
+  // (decomposition.hpp, fft_layout.hpp, fft_hip.hpp:65-78, fft.hpp:52-57):
+  // the refusals happen before any device work
+  {
+    World w = world::create(GridSize({16, 8, 12}), PhysicalOrigin({0, 0, 0}),
+                            GridSpacing({1, 1, 1}));
+    auto d1 = decomposition::create(w, 1);
+    CHECK(decomposition::get_num_domains(d1) == 1);
+    CHECK(decomposition::get_local_box(d1, 0).size == Int3({16, 8, 12}));
+    auto d4 = decomposition::create(w, 4);
+    CHECK(decomposition::get_grid(d4) == (std::array<int, 3>{1, 1, 4}));
+    CHECK(decomposition::get_local_box(d4, 3).low[2] == 9 &&
+          decomposition::get_local_box(d4, 3).high[2] == 11);
+    auto l1 = fft::layout::create(d1, 0);
+    CHECK(fft::layout::get_complex_box(l1, 0).size == Int3({9, 8, 12}));
+    CHECK(fft::layout::get_real_box(l1, 0).high == Int3({15, 7, 11}));
+    CHECK(fft::layout::get_r2c_direction(l1) == 0);
+    auto l4 = fft::layout::create(d4, 0);
+    CHECK(fft::layout::get_complex_box(l4, 2).low[2] == 6 &&
+          fft::layout::get_complex_box(l4, 2).size[0] == 9);
+    CHECK(throws<std::invalid_argument>([&] { (void)fft::layout::create(d1, 1); }));
+    CHECK(throws<std::logic_error>([&] { (void)fft::create_hip(d4, 0, MPI_COMM_WORLD); },
+                                   "one part per GPU"));
+    CHECK(throws<std::invalid_argument>([&] { (void)fft::create_hip(d1, 1); }));
+    CHECK(throws<std::runtime_error>([&] {
+      (void)fft::create_with_backend(l1, 0, fft::default_options(), fft::Backend::FFTW,
+                                     MPI_COMM_WORLD);
+    }, "Unsupported FFT backend"));
+    CHECK(throws<std::logic_error>([&] {
+      (void)fft::create_with_backend(d4, 0, fft::Backend::HIP);
+    }, "one part per GPU"));
+    // a Model over an IFFT that is not a HipFft is refused by the device models
+    BoxOnlyFft bf(w.size);
+    CHECK(throws<std::invalid_argument>([&] { PFCModelHIP m(bf, w, -0.25); }, "HipFft"));
+    CHECK(throws<std::invalid_argument>([&] { TungstenHIP<double> m(bf, w, MPI_COMM_WORLD); },
+                                        "HipFft"));
+    CountingModel cm(bf, w);
+    Time t({0.0, 1.0, 0.5}, 0.0);
+    Simulator sim(cm, t, MPI_COMM_WORLD);
+    CHECK(sim.mpi_comm() == MPI_COMM_WORLD && cm.mpi_comm() == MPI_COMM_WORLD);
+  }
+  return g_fail;
+}
+
+// Reference call sites on the device: the FFT from create_hip(decomposition,
+// rank, MPI_COMM_WORLD) and from create_with_backend(layout, rank, options,
+// Backend::HIP, MPI_COMM_WORLD), the model and the Simulator with their
+// communicator arguments; both runs must give the field of the World-based
+// construction (gpu-sim) bit for bit.
+static int run_gpu_callsites(int n, int steps) {
+  const double dx = M_PI / 4, dt = 0.5;
+  World w = domain::create_centered(GridSize({n, n, n}), GridSpacing({dx, dx, dx}));
+  auto run = [&](fft::IFFT &f) {
+    PFCModelHIP model(f, w, -0.25, MPI_COMM_WORLD);
+    initialize(model, dt);
+    Time time({0.0, steps * dt, dt}, 0.0);
+    Simulator sim(model, time, MPI_COMM_WORLD);
+    auto c = std::make_unique<Constant>(-0.2);
+    c->set_field_name("density");
+    sim.add_initial_conditions(std::move(c));
+    auto noise = std::make_unique<UniformNoise>(1234, -0.01, 0.01);
+    noise->set_field_name("density");
+    sim.add_initial_conditions(std::move(noise));
+    while (!done(sim)) pfc::step(sim);
+    model.prepare_for_field_modifiers();
+    return get_real_field(model, "density");
+  };
+  auto decomp = decomposition::create(w, 1);
+  fft::HipFft f0(w);
+  const RealField base = run(f0);
+  auto f1 = fft::create_hip(decomp, 0, MPI_COMM_WORLD);
+  CHECK(run(f1) == base);
+  auto layout = fft::layout::create(decomp, 0);
+  auto f2 = fft::create_with_backend(layout, 0, fft::default_options(), fft::Backend::HIP,
+                                     MPI_COMM_WORLD);
+  CHECK(run(*f2) == base);
+  auto f3 = fft::create_with_backend(decomp, 0, fft::Backend::HIP);
+  CHECK(run(*f3) == base);
+  std::printf("callsites %d^3 x %d steps: %zu values\n", n, steps, base.size());
   return g_fail;
 }
 
@@ -365,6 +450,141 @@ static int run_config(const char *path) {
   return 0;
 }
 
+// Host paths of the seed modifiers and MovingBC, for the oracle comparison
+// in tests/test_seeds_host.py.  Writes into DIR (raw little-endian f64):
+//   grid_rows.bin    SeedGrid rows (24 doubles each) on 128x64x64, dx 1.1107...
+//   random_rows.bin  RandomSeeds rows (150 x 24)
+//   draws.bin        the first 16 std::uniform_real_distribution(-3, 5) draws
+//                    of std::mt19937_64(42)
+//   grid_field.bin   SeedGrid field (host field::apply path) on that grid
+//   random_field.bin RandomSeeds field on 64^3, spacing 4, origin -128
+//   moving_%d.bin    a field after MovingBC call d = 0, 1, 2 (xline and xpos
+//                    printed), from the SeedGrid field shifted each call
+static int run_seeds(const std::string &dir) {
+  auto wr = [&](const std::string &name, const std::vector<double> &v) {
+    std::ofstream f(dir + "/" + name, std::ios::binary);
+    f.write(reinterpret_cast<const char *>(v.data()), (std::streamsize)(v.size() * sizeof(double)));
+  };
+  const double dx = 1.1107207345395915;
+  World w = domain::create(GridSize({128, 64, 64}), PhysicalOrigin({0, 0, 0}),
+                           GridSpacing({dx, dx, dx}));
+  SeedGrid g;
+  g.set_Ny(2);
+  g.set_Nz(2);
+  g.set_X0(30.0);
+  g.set_radius(25.0);
+  g.set_amplitude(0.215936);
+  g.set_density(-0.047);
+  std::vector<double> rows;
+  for (const Seed &sd : g.seeds(w)) {
+    double r[24];
+    sd.to_row(r);
+    rows.insert(rows.end(), r, r + 24);
+  }
+  wr("grid_rows.bin", rows);
+  RandomSeeds rs;
+  rs.set_amplitude(0.215936);
+  rs.set_density(-0.047);
+  rows.clear();
+  for (const Seed &sd : rs.seeds()) {
+    double r[24];
+    sd.to_row(r);
+    rows.insert(rows.end(), r, r + 24);
+  }
+  wr("random_rows.bin", rows);
+  {
+    std::mt19937_64 re(42);
+    std::uniform_real_distribution<double> u(-3.0, 5.0);
+    std::vector<double> d(16);
+    for (double &x : d) x = u(re);
+    wr("draws.bin", d);
+  }
+  {
+    BoxOnlyFft f(w.size);
+    CountingModel m(f, w);
+    m.initialize(1.0);
+    m.psi.assign(m.psi.size(), -0.1);
+    g.set_field_name("default");
+    g.apply(m, 0.0);
+    wr("grid_field.bin", m.psi);
+    MovingBC b(-0.464, -0.1);
+    b.set_xwidth(15.0);
+    b.set_alpha(1.0);
+    b.set_disp(40.0);
+    b.set_xpos(0.0);
+    b.set_field_name("default");
+    RealField base = m.psi;
+    for (int call = 0; call < 3; ++call) {
+      // move the seeded columns 9 cells along x per call
+      for (size_t r = 0; r < base.size() / 128; ++r)
+        for (int i = 0; i < 128; ++i) m.psi[r * 128 + i] = base[r * 128 + (i + 128 - 9 * call) % 128];
+      b.apply(m, 0.0);
+      std::printf("moving %d idx %d xpos %.17g\n", call, b.get_index(), b.get_xpos());
+      wr("moving_" + std::to_string(call) + ".bin", m.psi);
+    }
+  }
+  {
+    World w2 = domain::create(GridSize({64, 64, 64}), PhysicalOrigin({-128, -128, -128}),
+                              GridSpacing({4, 4, 4}));
+    BoxOnlyFft f(w2.size);
+    CountingModel m(f, w2);
+    m.initialize(1.0);
+    rs.set_field_name("default");
+    rs.apply(m, 0.0);
+    wr("random_field.bin", m.psi);
+  }
+  return g_fail;
+}
+
+// The autotune cache of the C++ drop-in (autotune.hpp) in directory DIR:
+// the first model of a size tunes and stores its choice, the next one loads
+// it; cache_only refuses a size without an entry; DISABLE keeps defaults;
+// an entry naming unknown variants counts as missing.
+static int run_gpu_autotune(const std::string &dir) {
+  setenv("OPENPFC_GPU_AUTOTUNE_CACHE_DIR", dir.c_str(), 1);
+  unsetenv("OPENPFC_GPU_AUTOTUNE_DISABLE");
+  setenv("OPENPFC_GPU_AUTOTUNE_MODE", "auto", 1);
+  const double dx = M_PI / 4;
+  World w = domain::create_centered(GridSize({64, 32, 32}), GridSpacing({dx, dx, dx}));
+  fft::HipFft f(w);
+  PFCModelHIP a(f, w, -0.25);
+  a.initialize(0.5);
+  CHECK(a.autotune_outcome() == gpu::TuneOutcome::tuned);
+  const std::string path = gpu::cache_file();
+  CHECK(std::filesystem::exists(path));
+  const json::Value cache = gpu::load_cache(path);
+  const json::Value *e = cache.find("entries");
+  CHECK(e && e->find(gpu::step_key("pfc", PFC_HIP_F64, w.size, false)));
+  const json::Value chosen = gpu::current_entry(f.plan());
+  PFCModelHIP b(f, w, -0.25);
+  b.initialize(0.5);
+  CHECK(b.autotune_outcome() == gpu::TuneOutcome::cached);
+  CHECK(gpu::current_entry(f.plan()).dump() == chosen.dump());
+  World w2 = domain::create_centered(GridSize({32, 32, 32}), GridSpacing({dx, dx, dx}));
+  fft::HipFft f2(w2);
+  setenv("OPENPFC_GPU_AUTOTUNE_MODE", "cache_only", 1);
+  CHECK(throws<std::runtime_error>([&] {
+    PFCModelHIP c(f2, w2, -0.25);
+    c.initialize(0.5);
+  }, "No cached configuration"));
+  setenv("OPENPFC_GPU_AUTOTUNE_DISABLE", "1", 1);
+  PFCModelHIP d(f2, w2, -0.25);
+  d.initialize(0.5);
+  CHECK(d.autotune_outcome() == gpu::TuneOutcome::defaults);
+  unsetenv("OPENPFC_GPU_AUTOTUNE_DISABLE");
+  // an entry with a variant name the library does not offer is a miss
+  json::Value bad = gpu::load_cache(path);
+  bad["entries"][gpu::step_key("pfc", PFC_HIP_F64, w.size, false)]["4"] =
+      json::Value::string("no-such-variant");
+  gpu::store_cache(path, bad);
+  setenv("OPENPFC_GPU_AUTOTUNE_MODE", "auto", 1);
+  PFCModelHIP g(f, w, -0.25);
+  g.initialize(0.5);
+  CHECK(g.autotune_outcome() == gpu::TuneOutcome::tuned);
+  std::printf("autotune cache %s\n", path.c_str());
+  return g_fail;
+}
+
 int main(int argc, char **argv) {
   const std::string mode = argc > 1 ? argv[1] : "cpu";
   int rc = 0;
@@ -375,9 +595,14 @@ int main(int argc, char **argv) {
     else if (mode == "gpu-tung" && argc == 5) rc = run_gpu_tung(std::atoi(argv[2]), argv[3], argv[4]);
     else if (mode == "parse" && argc == 3) return run_parse(argv[2]);
     else if (mode == "config" && argc == 3) rc = run_config(argv[2]);
+    else if (mode == "seeds" && argc == 3) rc = run_seeds(argv[2]);
+    else if (mode == "gpu-autotune" && argc == 3) rc = run_gpu_autotune(argv[2]);
+    else if (mode == "gpu-callsites" && argc == 4)
+      rc = run_gpu_callsites(std::atoi(argv[2]), std::atoi(argv[3]));
     else {
       std::fprintf(stderr, "usage: test_host cpu | gpu-fft | gpu-sim N STEPS OUT | "
-                           "gpu-tung STEPS OUT MFOUT | parse FILE | config FILE\n");
+                           "gpu-tung STEPS OUT MFOUT | parse FILE | config FILE | seeds DIR | "
+                           "gpu-callsites N STEPS | gpu-autotune DIR\n");
       return 2;
     }
   } catch (const std::exception &e) {

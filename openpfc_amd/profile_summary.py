@@ -42,7 +42,8 @@ def short_name(name: str) -> str:
     """pfc_amd kernels -> 'k_tz<double, r16x16x4/E16, G8, split>' style names."""
     m = re.match(r"void pfc_amd::(k_\w+)<(.*)>\(", name)
     if not m:
-        return name.split("(")[0][:80]
+        plain = name.replace("(anonymous namespace)::", "")
+        return plain.split("(")[0][:80]
     args = m.group(2)
     t = "double" if args.startswith("double") else ("float" if args.startswith("float") else "")
     # one or two (paired) line plans: "N512/E8,r8x8x8[+N1024/E16,r16x16x4]"

@@ -72,6 +72,21 @@ def lib():
         L.orc_sum_sq.restype = ctypes.c_double
         L.orc_sum_sq.argtypes = [_dp, ctypes.c_size_t]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_fft_seconds.restype = ctypes.c_double
+        L.orc_fft_seconds.argtypes = [ctypes.c_int]
+        L.orc_set_num_threads.argtypes = [ctypes.c_int]
+        L.orc_uniform_real_seq.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int, _dp]
+        L.orc_seed_make.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, _dp]
+        L.orc_seed_grid.argtypes = ([ctypes.c_int] * 2 + [ctypes.c_double] * 2
+                                    + [ctypes.c_int] * 2 + [ctypes.c_double] * 4 + [_dp])
+        L.orc_random_seeds.argtypes = [ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_ic_seeds.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 6
+                                   + [_dp, ctypes.c_int, ctypes.c_int, _dp])
+        L.orc_column_max.argtypes = [ctypes.c_int] * 3 + [_dp, _dp]
+        L.orc_bc_moving.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 8
+                                    + [ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp])
         _lib = L
     return _lib
 
@@ -255,6 +270,15 @@ def num_threads() -> int:
     return int(lib().orc_num_threads())
 
 
+def set_num_threads(n: int) -> None:
+    lib().orc_set_num_threads(int(n))
+
+
+def fft_seconds(reset: bool = True) -> float:
+    """Seconds pfc_step spent in transforms since the last reset."""
+    return float(lib().orc_fft_seconds(int(reset)))
+
+
 def add_counter_noise(a: np.ndarray, seed: int, lo: float, hi: float) -> np.ndarray:
     """psi += lo + (hi-lo)*u_i with the counter-based splitmix64 stream."""
     assert a.dtype == np.float64 and a.flags.c_contiguous
@@ -273,3 +297,82 @@ def ic_single_seed_masked(a: np.ndarray, origin, spacing, rho, amp, radius, keep
     lib().orc_ic_single_seed_masked(nx, ny, nz, *origin, *spacing, rho, amp, radius,
                                     int(keep_outside), _p(a))
     return a
+
+
+# -- rotated seeds, SeedGrid, RandomSeeds, MovingBC ----------------------------
+# A seed is a row of 24 doubles: location (3), six rotated wave vectors (18),
+# rho, radius, amplitude.
+
+def uniform_real_seq(seed: int, a: float, b: float, n: int) -> np.ndarray:
+    """First n draws of std::uniform_real_distribution<double>(a, b) on
+    std::mt19937_64(seed), as libstdc++ evaluates them."""
+    out = np.empty(n)
+    lib().orc_uniform_real_seq(ctypes.c_uint64(seed), a, b, n, _p(out))
+    return out
+
+
+def seed_make(location, orientation, radius: float, rho: float, amp: float) -> np.ndarray:
+    loc = np.ascontiguousarray(location, dtype=np.float64)
+    o = np.ascontiguousarray(orientation, dtype=np.float64)
+    out = np.empty(24)
+    lib().orc_seed_make(_p(loc), _p(o), radius, rho, amp, _p(out))
+    return out
+
+
+def seed_grid(size, spacing, Ny: int, Nz: int, X0: float, radius: float, rho: float,
+              amp: float) -> np.ndarray:
+    out = np.empty((Ny * Nz, 24))
+    lib().orc_seed_grid(int(size[1]), int(size[2]), float(spacing[1]), float(spacing[2]),
+                        Ny, Nz, X0, radius, rho, amp, _p(out))
+    return out
+
+
+def random_seeds(rho: float, amp: float) -> np.ndarray:
+    out = np.empty((150, 24))
+    lib().orc_random_seeds(rho, amp, _p(out))
+    return out
+
+
+def ic_seeds(a: np.ndarray, origin, spacing, seeds: np.ndarray, keep_outside=False):
+    """The field of a seed list, in place on a (nz, ny, nx) array."""
+    nz, ny, nx = a.shape
+    sd = np.ascontiguousarray(seeds, dtype=np.float64).reshape(-1, 24)
+    lib().orc_ic_seeds(nx, ny, nz, *map(float, origin), *map(float, spacing), _p(sd),
+                       sd.shape[0], int(keep_outside), _p(a))
+    return a
+
+
+def column_max(a: np.ndarray) -> np.ndarray:
+    nz, ny, nx = a.shape
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    out = np.empty(nx)
+    lib().orc_column_max(nx, ny, nz, _p(a), _p(out))
+    return out
+
+
+class MovingBC:
+    """MovingBC (one rank) applied in place to (nz, ny, nx) arrays; keeps the
+    reference's state (first call, column index, boundary position)."""
+
+    def __init__(self, rho_low, rho_high, xwidth=15.0, alpha=1.0, disp=40.0, xpos=0.0,
+                 threshold=0.1):
+        self.p = (rho_low, rho_high, xwidth, alpha, disp, threshold)
+        self.si = (ctypes.c_int * 2)(1, 0)
+        self.sd = np.array([xpos], dtype=np.float64)
+
+    @property
+    def xpos(self) -> float:
+        return float(self.sd[0])
+
+    @property
+    def idx(self) -> int:
+        return int(self.si[1])
+
+    def apply(self, a: np.ndarray, x0: float, dx: float) -> np.ndarray:
+        nz, ny, nx = a.shape
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+        xline = np.empty(nx)
+        rl, rh, w, al, disp, thr = self.p
+        lib().orc_bc_moving(nx, ny, nz, x0, dx, rl, rh, w, al, disp, thr, self.si,
+                            _p(self.sd), _p(xline), _p(a))
+        return xline

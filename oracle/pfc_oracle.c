@@ -30,6 +30,13 @@
  *     PDE; no reference code)
  *   - FixedBC boundary condition:
  *       include/openpfc/kernel/simulation/boundary_conditions/fixed_bc.hpp
+ *   - rotated seeds, SeedGrid, RandomSeeds (and the libstdc++ evaluation of
+ *     std::uniform_real_distribution over mt19937_64 they draw from):
+ *       include/openpfc/kernel/simulation/initial_conditions/seed.hpp:39-143
+ *       include/openpfc/kernel/simulation/initial_conditions/seed_grid.hpp:76-119
+ *       include/openpfc/kernel/simulation/initial_conditions/random_seeds.hpp:54-90
+ *   - MovingBC boundary condition (one rank):
+ *       include/openpfc/kernel/simulation/boundary_conditions/moving_bc.hpp:107-219
  *   - Tungsten mean-field PFC operators and step:
  *       apps/tungsten/include/tungsten/common/tungsten_spectral.hpp:83-104
  *       apps/tungsten/include/tungsten/cpu/tungsten_model.hpp:223-366
@@ -44,6 +51,7 @@
  * is compiled without FMA contraction by default).
  */
 #include <complex.h>
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -54,6 +62,16 @@
 #endif
 
 typedef double complex cdbl;
+
+/* wall clock and the transform-time meter of orc_pfc_step */
+static double g_fft_seconds = 0.0;
+static double orc_wtime(void) {
+#ifdef _OPENMP
+  return omp_get_wtime();
+#else
+  return 0.0;
+#endif
+}
 
 /* ------------------------------------------------------------------------ */
 /* mt19937_64 (the C++ standard's std::mt19937_64 engine, [rand.predef]).    */
@@ -517,17 +535,42 @@ int orc_pfc_step(int nx, int ny, int nz, const double *opL, const double *opN,
     return -2;
   }
   for (int s = 0; s < nsteps; ++s) {
+    double t0 = orc_wtime();
     orc_r2c3d(nx, ny, nz, psi, (double *)psiF);
+    double t1 = orc_wtime();
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < (long)nr; ++i) psiN[i] = psi[i] * psi[i] * psi[i];
+    double t2 = orc_wtime();
     orc_r2c3d(nx, ny, nz, psiN, (double *)NF);
+    double t3 = orc_wtime();
     etd_combine(psiF, NF, opL, opN, nc);
+    double t4 = orc_wtime();
     orc_c2r3d(nx, ny, nz, (const double *)psiF, psi, (double *)NF);
+    g_fft_seconds += (t1 - t0) + (t3 - t2) + (orc_wtime() - t4);
   }
   free(psiF);
   free(NF);
   free(psiN);
   return 0;
+}
+
+/* Seconds spent in the transforms of orc_pfc_step since the last reset (the
+ * FFT share the reference's scalability driver reports,
+ * apps/tungsten/src/tungsten_scalability.cpp:95-135). */
+double orc_fft_seconds(int reset) {
+  const double v = g_fft_seconds;
+  if (reset) g_fft_seconds = 0.0;
+  return v;
+}
+
+/* OpenMP threads of the following oracle calls (a CPU-baseline row on one
+ * thread). */
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
 }
 
 /* Adjoint of one plain PFC step (no reference counterpart; the mathematics
@@ -824,6 +867,214 @@ void orc_ic_single_seed_masked(int nx, int ny, int nz, double x0, double y0,
           u += 2.0 * amp *
                cos((q[qi][0] * x) + (q[qi][1] * y) + (q[qi][2] * z));
         psi[idx] = u;
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Rotated seeds, SeedGrid, RandomSeeds and MovingBC.                        */
+/* ------------------------------------------------------------------------ */
+
+/* std::uniform_real_distribution<double>(a, b) over std::mt19937_64 as
+ * libstdc++ evaluates it: generate_canonical<double, 53> takes
+ * k = max(1, ceil(53 / 64)) = 1 engine output x and returns x / 2^64 (x
+ * rounded to double first), pulled below 1 when the rounding reaches 1
+ * ([rand.util.canonical]); the distribution returns u * (b - a) + a. */
+double orc_uniform_real(orc_mt64 *g, double a, double b) {
+  double u = (double)orc_mt64_next(g) / 18446744073709551616.0;
+  if (u >= 1.0) u = nextafter(1.0, 0.0);
+  return (u * (b - a)) + a;
+}
+
+/* The first n draws of uniform_real_distribution(a, b) on mt19937_64(seed). */
+void orc_uniform_real_seq(uint64_t seed, double a, double b, int n, double *out) {
+  orc_mt64 g;
+  orc_mt64_seed(&g, seed);
+  for (int i = 0; i < n; ++i) out[i] = orc_uniform_real(&g, a, b);
+}
+
+/* Seed (initial_conditions/seed.hpp:39-143).  A seed is stored as 24
+ * doubles: location (3), the six rotated wave vectors (18), rho, radius,
+ * amplitude.  R = yaw(o0) * (pitch(o1) * roll(o2)); q_i = R q_i with the
+ * <110>/sqrt(2) vectors in the reference's order; every 3x3 product sums
+ * from 0.0 over k = 0, 1, 2 as mult3 does. */
+static void mat3_mul(const double A[3][3], const double B[3][3], double C[3][3]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      double c = 0.0;
+      for (int k = 0; k < 3; k++) c += A[i][k] * B[k][j];
+      C[i][j] = c;
+    }
+}
+
+void orc_seed_make(const double loc[3], const double orient[3], double radius, double rho,
+                   double amp, double out[24]) {
+  const double ca = cos(orient[0]), sa = sin(orient[0]);
+  const double cb = cos(orient[1]), sb = sin(orient[1]);
+  const double cc = cos(orient[2]), sc = sin(orient[2]);
+  const double Ra[3][3] = {{ca, -sa, 0.0}, {sa, ca, 0.0}, {0.0, 0.0, 1.0}};
+  const double Rb[3][3] = {{cb, 0.0, sb}, {0.0, 1.0, 0.0}, {-sb, 0.0, cb}};
+  const double Rc[3][3] = {{1.0, 0.0, 0.0}, {0.0, cc, -sc}, {0.0, sc, cc}};
+  double Rbc[3][3], R[3][3];
+  mat3_mul(Rb, Rc, Rbc);
+  mat3_mul(Ra, Rbc, R);
+  const double s = 1.0 / sqrt(2.0);
+  const double q[6][3] = {{s, s, 0}, {s, 0, s},  {0, s, s},
+                          {s, 0, -s}, {s, -s, 0}, {0, s, -s}};
+  for (int d = 0; d < 3; ++d) out[d] = loc[d];
+  for (int qi = 0; qi < 6; ++qi)
+    for (int i = 0; i < 3; ++i) {
+      double c = 0.0;
+      for (int j = 0; j < 3; ++j) c += R[i][j] * q[qi][j];
+      out[3 + 3 * qi + i] = c;
+    }
+  out[21] = rho;
+  out[22] = radius;
+  out[23] = amp;
+}
+
+/* SeedGrid::apply (initial_conditions/seed_grid.hpp:76-119): Ny*Nz seeds,
+ * j outer, k inner; location {X0 + rt, Y0 + Dy*j + rt, Z0 + Dz*k + rt} and
+ * orientation {rr, rr, rr} drawn left to right from mt19937_64(42), with
+ * rt ~ U(-0.2 radius, 0.2 radius), rr ~ U(0, 8 atan(1)),
+ * Dy = dy*Ly/Ny, Dz = dz*Lz/Nz, Y0 = Dy/2, Z0 = Dz/2.  Writes Ny*Nz seeds. */
+void orc_seed_grid(int Ly, int Lz, double dy, double dz, int Ny, int Nz, double X0,
+                   double radius, double rho, double amp, double *out) {
+  const double Dy = dy * (double)Ly / (double)Ny;
+  const double Dz = dz * (double)Lz / (double)Nz;
+  const double Y0 = Dy / 2.0, Z0 = Dz / 2.0;
+  orc_mt64 g;
+  orc_mt64_seed(&g, 42);
+  const double ta = -0.2 * radius, tb = 0.2 * radius;
+  const double rb = 8.0 * atan(1.0);
+  int n = 0;
+  for (int j = 0; j < Ny; j++)
+    for (int k = 0; k < Nz; k++) {
+      double loc[3], o[3];
+      loc[0] = X0 + orc_uniform_real(&g, ta, tb);
+      loc[1] = Y0 + (Dy * j) + orc_uniform_real(&g, ta, tb);
+      loc[2] = Z0 + (Dz * k) + orc_uniform_real(&g, ta, tb);
+      for (int d = 0; d < 3; ++d) o[d] = orc_uniform_real(&g, 0.0, rb);
+      orc_seed_make(loc, o, radius, rho, amp, out + 24 * n++);
+    }
+}
+
+/* RandomSeeds::apply (initial_conditions/random_seeds.hpp:54-90): 150 seeds
+ * of radius 20; location {U(-108, -68), U(-128, 128), U(-128, 128)} and
+ * orientation {U(0, 2 pi)} x 3 from mt19937_64(42).  Writes 150 seeds. */
+void orc_random_seeds(double rho, double amp, double *out) {
+  const double radius = 20.0;
+  const double lx = -128.0 + radius, ux = -128.0 + (3 * radius);
+  const double rb = 8.0 * atan(1.0);
+  orc_mt64 g;
+  orc_mt64_seed(&g, 42);
+  for (int i = 0; i < 150; i++) {
+    double loc[3], o[3];
+    loc[0] = orc_uniform_real(&g, lx, ux);
+    loc[1] = orc_uniform_real(&g, -128.0, 128.0);
+    loc[2] = orc_uniform_real(&g, -128.0, 128.0);
+    for (int d = 0; d < 3; ++d) o[d] = orc_uniform_real(&g, 0.0, rb);
+    orc_seed_make(loc, o, radius, rho, amp, out + 24 * i);
+  }
+}
+
+/* The field of a seed list (seed_grid.hpp:108-118, random_seeds.hpp:80-89):
+ * at X = origin + i*spacing the first seed with |X - location|^2 < radius^2
+ * gives rho + sum_i 2*amp*cos(q_i . X) (seed.hpp:124-141, X absolute);
+ * outside every seed 0.0, or the field unchanged with keep_outside. */
+void orc_ic_seeds(int nx, int ny, int nz, double x0, double y0, double z0, double dx,
+                  double dy, double dz, const double *seeds, int nseeds, int keep_outside,
+                  double *psi) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const size_t idx = ((size_t)k * ny + j) * nx + i;
+        const double X = x0 + i * dx, Y = y0 + j * dy, Z = z0 + k * dz;
+        int hit = -1;
+        for (int s = 0; s < nseeds && hit < 0; ++s) {
+          const double *sd = seeds + 24 * s;
+          const double x = X - sd[0], y = Y - sd[1], z = Z - sd[2];
+          const double r = sd[22];
+          if (x * x + y * y + z * z < r * r) hit = s;
+        }
+        if (hit < 0) {
+          if (!keep_outside) psi[idx] = 0.0;
+          continue;
+        }
+        const double *sd = seeds + 24 * hit;
+        double u = sd[21];
+        const double a = sd[23];
+        for (int qi = 0; qi < 6; ++qi) {
+          const double *q = sd + 3 + 3 * qi;
+          u += 2.0 * a * cos((q[0] * X) + (q[1] * Y) + (q[2] * Z));
+        }
+        psi[idx] = u;
+      }
+}
+
+/* MovingBC's column profile (boundary_conditions/moving_bc.hpp:120-127):
+ * out[i] = max over (j, k) of psi, starting from the lowest double. */
+void orc_column_max(int nx, int ny, int nz, const double *psi, double *out) {
+  for (int i = 0; i < nx; ++i) out[i] = -DBL_MAX;
+  size_t idx = 0;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i, ++idx)
+        if (psi[idx] > out[i]) out[i] = psi[idx];
+}
+
+/* MovingBC::apply and fill_bc (moving_bc.hpp:107-219) on one rank.  State:
+ * state_i = {first, idx}, state_d = {xpos}.  On the first call idx is the
+ * last column whose maximum exceeds threshold (unchanged if none); later
+ * calls advance idx while column idx % nx exceeds it, at most nx columns.
+ * xpos = max(x0 + idx*dx + disp, xpos).  Then, with l = nx*dx and
+ * p = fmod(xpos, l), every point whose x lies within xwidth of p (or of
+ * p -+ l where the band wraps) becomes rho_low*S + rho_high*(1-S),
+ * S = 1/(1 + exp(-alpha*d)).  The column maxima go to xline (nx). */
+void orc_bc_moving(int nx, int ny, int nz, double x0, double dx, double rho_low,
+                   double rho_high, double xwidth, double alpha, double disp,
+                   double threshold, int *state_i, double *state_d, double *xline,
+                   double *psi) {
+  orc_column_max(nx, ny, nz, psi, xline);
+  int idx_ = state_i[1];
+  if (state_i[0]) {
+    for (int i = nx - 1; i >= 0; i--)
+      if (xline[i] > threshold) {
+        idx_ = i;
+        break;
+      }
+  } else {
+    int scanned = 0;
+    while (xline[idx_ % nx] > threshold && scanned < nx) {
+      idx_ += 1;
+      scanned += 1;
+    }
+  }
+  const double new_xpos = x0 + (idx_ * dx) + disp;
+  const double old = state_d[0];
+  state_d[0] = (new_xpos < old) ? old : new_xpos;
+  state_i[0] = 0;
+  state_i[1] = idx_;
+  const double Lx = (double)nx;
+  const double l = Lx * dx;
+  const double xpos = fmod(state_d[0], l);
+  size_t n = 0;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i, ++n) {
+        const double x = x0 + i * dx;
+        const double dist = x - xpos;
+        double d;
+        if (fabs(dist) < xwidth)
+          d = dist;
+        else if (xpos < xwidth && fabs(dist - l) < xwidth)
+          d = dist - l;
+        else if (xpos > l - xwidth && fabs(dist + l) < xwidth)
+          d = dist + l;
+        else
+          continue;
+        const double S = 1.0 / (1.0 + exp(-alpha * d));
+        psi[n] = (rho_low * S) + (rho_high * (1.0 - S));
       }
 }
 

@@ -19,3 +19,14 @@ def torch_gpu():
     if not torch.cuda.is_available():
         pytest.fail("GPU test selected but no HIP device is visible")
     return torch
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _autotune_cache_dir(tmp_path_factory):
+    """Device-resident models of the C++ layer tune their step on
+    initialize() and cache the choice in OPENPFC_GPU_AUTOTUNE_CACHE_DIR
+    (default: the working directory); the test session keeps that cache in a
+    temporary directory, shared by the programs it starts."""
+    if "OPENPFC_GPU_AUTOTUNE_CACHE_DIR" not in os.environ:
+        os.environ["OPENPFC_GPU_AUTOTUNE_CACHE_DIR"] = str(tmp_path_factory.mktemp("autotune"))
+    yield

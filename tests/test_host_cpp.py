@@ -77,3 +77,26 @@ def test_host_tungsten_hip_cpu_vs_hip_recipe(torch_gpu, tmp_path):
     ref, ref_mf = O.tungsten_step(psi0, fMF, expL, nW, prm.nonlinear_params(), 10)
     assert float(np.abs(got - ref).max()) < 1e-10
     assert float(np.abs(mf - ref_mf).max()) < 1e-10
+
+
+@pytest.mark.gpu
+def test_host_reference_call_sites(torch_gpu):
+    """create_hip(decomposition, rank, MPI_COMM_WORLD), create_with_backend
+    (layout, rank, options, Backend::HIP, MPI_COMM_WORLD), the model and the
+    Simulator with their communicator arguments: the same field, bit for bit,
+    as the World-based construction."""
+    r = _run("gpu-callsites", "32", "12")
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+def test_host_autotune_cache(torch_gpu, tmp_path):
+    """PFCModelHIP::initialize tunes and stores its step variants in
+    <dir>/<device>_autotune_cache.json, the next model of that size loads
+    them; OPENPFC_GPU_AUTOTUNE_MODE=cache_only refuses a size without an
+    entry, OPENPFC_GPU_AUTOTUNE_DISABLE=1 keeps the defaults, and an entry
+    naming an unknown variant is tuned again."""
+    r = _run("gpu-autotune", str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    files = list(tmp_path.glob("gfx*_autotune_cache.json"))
+    assert len(files) == 1, files
