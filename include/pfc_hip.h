@@ -307,6 +307,22 @@ int pfc_hip_ic_seeds(pfc_hip_plan_t plan, void *psi, const double *origin3,
 int pfc_hip_column_max(pfc_hip_plan_t plan, const void *psi, double *xline, void *stream);
 
 /*
+ * No counterpart in the reference: pfc_hip_column_max for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It allocates nothing:
+ * the partial maxima live in a static device array of the library (4 MiB,
+ * held by each GPU the library runs on until the process ends), and the
+ * result goes to xline_dev, a device buffer of nx doubles the caller owns.
+ * Asynchronous on `stream` (no synchronisation, no host copy).  The scratch
+ * holds min(ny*nz, 256) partial rows of nx columns, so nx <= 2048; a larger
+ * nx gives PFC_HIP_EINVAL.  Every call on one GPU shares that scratch:
+ * calls that overlap on one GPU (different streams, no ordering between
+ * them) race and give wrong maxima.
+ */
+int pfc_hip_column_max_noalloc(pfc_hip_plan_t plan, const void *psi, double *xline_dev,
+                               void *stream);
+
+/*
  * MovingBC's band on the device: with l = nx*dx and p = fmod(xpos, l), every
  * point whose x = x0 + i*dx lies within xwidth of p, or of p - l / p + l
  * where the band wraps round the periodic box, becomes

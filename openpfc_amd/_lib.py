@@ -105,6 +105,7 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_double), ctypes.c_int,
                                         ctypes.c_int, _vp]),
     "pfc_hip_column_max": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_double), _vp]),
+    "pfc_hip_column_max_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "pfc_hip_bc_moving": (ctypes.c_int, [_vp, _vp] + [ctypes.c_double] * 7 + [_vp]),
 }
 

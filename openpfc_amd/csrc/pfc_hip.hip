@@ -15,6 +15,7 @@
 #include <new>
 #include <string>
 #include <map>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -24,6 +25,13 @@ using namespace pfc_amd;
 namespace pfc_amd {
 // message of the last failed ABI call on this thread (field_io.hip sets it too)
 thread_local std::string g_last_error;
+
+// Scratch of pfc_hip_column_max_noalloc: the partial maxima of up to 256
+// rows of nx <= 2048 columns (4 MiB), one copy per GPU for the life of the
+// process.  Its name is a host-side stand-in for the device array, so host
+// code uses only the address col_scratch() obtains on each device.
+constexpr long kColScratchDoubles = 256L * 2048;
+__device__ __align__(256) double scratch[kColScratchDoubles];
 } // namespace pfc_amd
 
 namespace {
@@ -693,6 +701,30 @@ int adjoint_impl(pfc_hip_plan_s *p, const void *psi, void *lambda, double *dr, h
 // partial rows of the column maxima (pfc_hip_column_max)
 constexpr long kColParts = 256;
 
+constexpr long kColScratch = pfc_amd::kColScratchDoubles;
+using pfc_amd::scratch;
+
+// Device address of `scratch` on device `dev` (current), looked up on the
+// first call on that device and kept for the process.
+constexpr int kMaxDevices = 64;
+int col_scratch(int dev, double **out) {
+  static std::mutex mu;
+  static double *addr[kMaxDevices] = {};
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(PFC_HIP_EUNSUPPORTED, "column-max scratch: device index " + std::to_string(dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(PFC_HIP_EDEVICE, std::string("hipGetSymbolAddress(column-max scratch): ") +
+                                       hipGetErrorString(e));
+    addr[dev] = (double *)p;
+  }
+  *out = addr[dev];
+  return 0;
+}
+
 // second stage of the column maxima: the maximum of the nparts partials
 __global__ void __launch_bounds__(256) k_column_max_final(const double *__restrict__ part,
                                                           int nx, int nparts,
@@ -706,6 +738,7 @@ __global__ void __launch_bounds__(256) k_column_max_final(const double *__restri
   }
   out[x] = m;
 }
+
 
 unsigned ic_blocks(long n) {
   return (unsigned)std::min<long>((n + 255) / 256, 1L << 20);
@@ -1420,6 +1453,40 @@ int pfc_hip_column_max(pfc_hip_plan_t plan, const void *psi, double *xline, void
   PFC_HIP_TRY(hipMemcpyAsync(xline, out, (size_t)nx * sizeof(double), hipMemcpyDeviceToHost, s),
               "column max copy");
   PFC_HIP_TRY(hipStreamSynchronize(s), "column max sync");
+  return 0;
+}
+
+// The same maxima without allocating: partials in the library's static
+// scratch, result in the caller's device buffer, nothing waited for.
+int pfc_hip_column_max_noalloc(pfc_hip_plan_t plan, const void *psi, double *xline_dev,
+                               void *stream) {
+  if (int rc = check_single(plan, "pfc_hip_column_max_noalloc")) return rc;
+  if (!psi || !xline_dev)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_column_max_noalloc: null argument");
+  const int nx = plan->nx;
+  const long rows = (long)plan->ny * plan->nz;
+  const int parts = (int)std::min<long>(rows, kColParts);
+  if ((long)parts * nx > kColScratch)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_column_max_noalloc: nx = " + std::to_string(nx) +
+                                    " needs " + std::to_string((long)parts * nx) +
+                                    " doubles of scratch, more than its " +
+                                    std::to_string(kColScratch) + " (nx <= " +
+                                    std::to_string(kColScratch / kColParts) + ")");
+  if (int rc = set_device(plan)) return rc;
+  double *part = nullptr;
+  if (int rc = col_scratch(plan->device, &part)) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned gx = (unsigned)((nx + 255) / 256);
+  if (plan->prec == PFC_HIP_F64)
+    hipLaunchKernelGGL(k_column_max_part<double>, dim3(gx, (unsigned)parts), dim3(256), 0, s,
+                       (const double *)psi, nx, rows, part);
+  else
+    hipLaunchKernelGGL(k_column_max_part<float>, dim3(gx, (unsigned)parts), dim3(256), 0, s,
+                       (const float *)psi, nx, rows, part);
+  PFC_HIP_TRY(hipGetLastError(), "column max (partials)");
+  hipLaunchKernelGGL(k_column_max_final, dim3(gx), dim3(256), 0, s, (const double *)part, nx,
+                     parts, xline_dev);
+  PFC_HIP_TRY(hipGetLastError(), "column max (final)");
   return 0;
 }
 
