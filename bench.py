@@ -95,7 +95,8 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
                 f"{pb[2][1:]},{'true' if pb[3] == 'split' else 'false'}>")
     parts = variant.split("/")
     nt = "nt" in parts  # non-temporal lines: the kernel's last template argument
-    parts = [p for p in parts if p != "nt"]
+    xp = "xp" in parts  # a plan with double butterflies
+    parts = [p for p in parts if p not in ("nt", "xp")]
     if kernel == "k_tx_fwd" and "r2c2" in parts:
         kernel, n = "k_tx_fwd2", n // 2
     if kernel == "k_tx_fwd" and "wide" in parts:
@@ -133,7 +134,8 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
     if kernel in NT_KERNELS:
         rest.append("true" if nt else "false")
     t = "double" if prec == "f64" else "float"
-    return f"{kernel}<{t},N{nn}/E{e},r{radices}" + ("," + ",".join(rest) if rest else "") + ">"
+    return (f"{kernel}<{t},N{nn}/E{e}{'/xp' if xp else ''},r{radices}"
+            + ("," + ",".join(rest) if rest else "") + ">")
 
 
 def pass_bytes(n: int, esize: int):
@@ -380,16 +382,20 @@ def copy_calibration(torch, nbytes=4 << 30, reps=5):
 
 
 def accuracy_run(P, torch, n, dx, dt, r, steps, every=100):
-    """Accuracy run: FP64 and FP32-mixed advanced in lockstep from the
-    same field; relative L2 / max error of FP32-mixed against FP64 every
-    `every` steps (the FP64 GPU step is held to 1e-10 of the CPU oracle)."""
+    """Accuracy run: FP64 and the two FP32-mixed modes (FP32 butterflies;
+    FP64 butterflies on FP32 fields) advanced in lockstep from the same
+    field; relative L2 / max error of each against FP64 every `every` steps
+    (the FP64 GPU step is held to 1e-10 of the CPU oracle)."""
     fft64, st64, psi = setup(P, torch, n, P.F64, dx, dt, r)
-    fft32 = P.HipFft(n, n, n, precision=P.F32)
-    st32 = P.PfcStepper(fft32, (dx, dx, dx), r)
-    st32.initialize(dt)
     mean0 = float(psi.mean().item())
-    d = (psi - mean0).float()
-    st32.set_mean(mean0)
+    mixed = {}
+    for key, flags in (("fp32_mixed", 0), ("fp32_mixed_f64_butterflies",
+                                            P.FLAG_F64_BUTTERFLIES)):
+        f = P.HipFft(n, n, n, precision=P.F32, flags=flags)
+        st = P.PfcStepper(f, (dx, dx, dx), r)
+        st.initialize(dt)
+        st.set_mean(mean0)
+        mixed[key] = (f, st, (psi - mean0).float())
     # control: FP64 from the initial field rounded to FP32 (one FP32-sized
     # perturbation of the state, no FP32 arithmetic afterwards)
     fftc = P.HipFft(n, n, n, precision=P.F64)
@@ -401,21 +407,28 @@ def accuracy_run(P, torch, n, dx, dt, r, steps, every=100):
     while done < steps:
         k = min(every, steps - done)
         st64.step(psi, k)
-        st32.step(d, k)
+        for f, st, d in mixed.values():
+            st.step(d, k)
         stc.step(pc, k)
         done += k
         nrm = torch.linalg.vector_norm(psi)
-        diff = d.double() + mean0 - psi
-        l2 = (torch.linalg.vector_norm(diff) / nrm).item()
-        mx = (diff.abs().max() / psi.abs().max()).item()
-        drift = abs((d.double().mean() + mean0).item() - mean0)
-        cl2 = (torch.linalg.vector_norm(pc - psi) / nrm).item()
-        del diff
-        rows.append({"step": done, "rel_l2": l2, "rel_max": mx, "mean_drift": drift,
-                     "fp64_from_fp32_rounded_start_rel_l2": cl2})
-        log(f"accuracy step {done}: rel L2 {l2:.3e} rel max {mx:.3e} (fp64 control {cl2:.3e})")
+        mx0 = psi.abs().max()
+        row = {"step": done}
+        for key, (f, st, d) in mixed.items():
+            diff = d.double() + mean0 - psi
+            row[key] = {"rel_l2": (torch.linalg.vector_norm(diff) / nrm).item(),
+                        "rel_max": (diff.abs().max() / mx0).item(),
+                        "mean_drift": abs((d.double().mean() + mean0).item() - mean0)}
+            del diff
+        row["fp64_from_fp32_rounded_start_rel_l2"] = \
+            (torch.linalg.vector_norm(pc - psi) / nrm).item()
+        rows.append(row)
+        log(f"accuracy step {done}: " + ", ".join(
+            f"{key} rel L2 {row[key]['rel_l2']:.3e} rel max {row[key]['rel_max']:.3e}"
+            for key in mixed) + f" (fp64 control {row['fp64_from_fp32_rounded_start_rel_l2']:.3e})")
     return {"metric": "fp32-mixed error vs fp64", "grid": [n, n, n], "steps": steps,
             "control": "fp64 run started from the fp32-rounded initial field",
+            "bar": {"rel_l2": 1e-4, "after_steps": 1000},
             "rows": rows}
 
 
@@ -748,6 +761,9 @@ def main(argv=None):
     ap.add_argument("--save-variants", help="write the chosen variants to this JSON file")
     ap.add_argument("--variants-fp32", help="as --variants, for the FP32-mixed leg")
     ap.add_argument("--save-variants-fp32", help="as --save-variants, for the FP32-mixed leg")
+    ap.add_argument("--variants-fp32xp", help="as --variants, for the FP32-mixed leg with "
+                                              "FP64 butterflies")
+    ap.add_argument("--save-variants-fp32xp", help="as --save-variants, for that leg")
     ap.add_argument("--traffic", default=None,
                     help="traffic table from openpfc_amd.profile_summary (default: "
                          "profiles/traffic_pfc_step_<n>_f64.json when present)")
@@ -871,48 +887,62 @@ def main(argv=None):
         "checks": {"finite": finite, "mean_after": mean64},
     }
 
-    # ---- FP32-mixed (same workload, same step sequence) ---------------------
+    # ---- FP32-mixed (same workload, same step sequence): FP32 butterflies,
+    # and FP64 butterflies on the same FP32 fields ---------------------------
     log(f"fp64 done: {ms64:.3f} ms/step")
     if not args.no_fp32:
-        log("fp32-mixed: setup")
-        fft = P.HipFft(n, n, n, precision=P.F32)
-        st = P.PfcStepper(fft, (dx, dx, dx), r)
-        st.initialize(dt)
+        legs = [("fp32_mixed", 0, args.variants_fp32, args.save_variants_fp32,
+                 "fp32 field and transforms; fp64 conserved mean (mean-split step, k=0 pinned)"),
+                ("fp32_mixed_f64_butterflies", P.FLAG_F64_BUTTERFLIES, args.variants_fp32xp,
+                 args.save_variants_fp32xp,
+                 "fp32 field and spectra, fp64 butterflies (twiddle products and in-register "
+                 "DFTs), one fp32 rounding per radix pass; fp64 conserved mean")]
         mean0 = float(psi0.double().mean().item())
-        d = (psi0 - mean0).float()
+        for key, flags, vfile, sfile, mode in legs:
+            log(f"{key}: setup")
+            fft = P.HipFft(n, n, n, precision=P.F32, flags=flags)
+            st = P.PfcStepper(fft, (dx, dx, dx), r)
+            st.initialize(dt)
+            d = (psi0 - mean0).float()
+            st.set_mean(mean0)
+            a32 = argparse.Namespace(**vars(args))
+            a32.variants = vfile
+            chosen32 = choose_variants(P, fft, st, d, a32, key)
+            if sfile:
+                with open(sfile, "w") as fh:
+                    json.dump(chosen32, fh, indent=1)
+            ms32, _ = time_steps(torch, st, d, args.steps, args.warmup)
+            k32 = st.step_timed(d, 3)
+            k32f = st.step_timed_fused(d, max(args.steps, 2))
+            got = d.double() + mean0
+            diff = got - psi64
+            l2 = (torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(psi64)).item()
+            mx = (diff.abs().max() / psi64.abs().max()).item()
+            drift = abs(got.mean().item() - mean0)
+            del d, got, diff, st, fft
+            torch.cuda.empty_cache()
+            log(f"{key}: {ms32:.3f} ms/step")
+            out[key] = {
+                "mode": mode,
+                "steps_per_s": 1000.0 / ms32, "ms_per_step": ms32,
+                "ns_per_dof_step": ms32 * 1e6 / n ** 3,
+                "speedup_vs_fp64": ms64 / ms32,
+                "step_hbm_frac": sum(pass_bytes(n, 4).values()) / (ms32 * 1e-3) / HBM_PEAK,
+                "run_hbm_frac": ((sum(fused_pass_bytes(n, 4).values())
+                                  + 2 * 4 * n ** 3 / args.steps)
+                                 if k32f[5] > 0 else sum(pass_bytes(n, 4).values()))
+                                / (ms32 * 1e-3) / HBM_PEAK,
+                "kernels_ms": dict(zip(PASS_NAMES, k32)),
+                "fused_run_ms": dict(zip(FUSED_NAMES, k32f)) if k32f[5] > 0 else None,
+                "variants": chosen32,
+                "error_vs_fp64_after": {"steps": args.warmup + 2 * args.steps + 3
+                                        if args.steps >= 2 else args.warmup + args.steps + 5,
+                                        "rel_l2": l2, "rel_max": mx, "mean_drift": drift,
+                                        "note": "a short run; the 1000-step error is "
+                                                "bench.py --accuracy 1000"},
+                "accuracy_bar": {"rel_l2_after_1000_steps": 1e-4,
+                                 "measured_by": "bench.py --accuracy 1000"}}
         del psi0
-        st.set_mean(mean0)
-        args32 = argparse.Namespace(**vars(args))
-        args32.variants = args.variants_fp32
-        chosen32 = choose_variants(P, fft, st, d, args32, "fp32")
-        if args.save_variants_fp32:
-            with open(args.save_variants_fp32, "w") as fh:
-                json.dump(chosen32, fh, indent=1)
-        ms32, _ = time_steps(torch, st, d, args.steps, args.warmup)
-        k32 = st.step_timed(d, 3)
-        k32f = st.step_timed_fused(d, max(args.steps, 2))
-        got = d.double() + mean0
-        diff = got - psi64
-        l2 = (torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(psi64)).item()
-        mx = (diff.abs().max() / psi64.abs().max()).item()
-        drift = abs(got.mean().item() - mean0)
-        del d, got, diff, st, fft
-        torch.cuda.empty_cache()
-        out["fp32_mixed"] = {
-            "mode": "fp32 field and transforms; fp64 conserved mean (mean-split step, k=0 pinned)",
-            "steps_per_s": 1000.0 / ms32, "ms_per_step": ms32,
-            "ns_per_dof_step": ms32 * 1e6 / n ** 3,
-            "speedup_vs_fp64": ms64 / ms32,
-            "step_hbm_frac": sum(pass_bytes(n, 4).values()) / (ms32 * 1e-3) / HBM_PEAK,
-            "run_hbm_frac": ((sum(fused_pass_bytes(n, 4).values()) + 2 * 4 * n ** 3 / args.steps)
-                             if k32f[5] > 0 else sum(pass_bytes(n, 4).values()))
-                            / (ms32 * 1e-3) / HBM_PEAK,
-            "kernels_ms": dict(zip(PASS_NAMES, k32)),
-            "fused_run_ms": dict(zip(FUSED_NAMES, k32f)) if k32f[5] > 0 else None,
-            "variants": chosen32,
-            "error_vs_fp64_after": {"steps": args.warmup + 2 * args.steps + 3 if args.steps >= 2
-                                    else args.warmup + args.steps + 5,
-                                    "rel_l2": l2, "rel_max": mx, "mean_drift": drift}}
     del psi64
     torch.cuda.empty_cache()
 

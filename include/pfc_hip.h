@@ -47,6 +47,19 @@ enum {
 
 enum { PFC_HIP_F64 = 0, PFC_HIP_F32 = 1 };
 
+/*
+ * Plan flag (pfc_hip_plan_create): with PFC_HIP_F32, the passes of the PFC
+ * step keep FP32 fields and spectra in HBM and LDS but compute every
+ * butterfly (twiddle product and in-register DFT) in FP64 from FP64
+ * twiddles, rounding to FP32 once per radix pass.  No reference
+ * counterpart: it lowers the FP32-mixed step's arithmetic error (the
+ * mean-split mode, pfc_hip_pfc_set_mean).  Needs the transposed step
+ * layout (power-of-two extents, every axis >= 128, nx >= 256); such plans run
+ * the PFC model only, and their plain transforms, free energy and adjoint use
+ * FP32 arithmetic.
+ */
+#define PFC_HIP_FLAG_F64_BUTTERFLIES 1
+
 enum { PFC_HIP_MODEL_PFC = 0, PFC_HIP_MODEL_TUNGSTEN = 1 };
 
 /* Number of parameters of PFC_HIP_MODEL_TUNGSTEN (see pfc_hip_pfc_set_operators). */
@@ -66,7 +79,7 @@ const char *pfc_hip_last_error(void);
  *   src/openpfc/runtime/hip/fft_hip.cpp:19-37 (create_hip) and
  *   src/openpfc/runtime/cpu/fft.cpp:114-121,131-191 (create / create_with_backend)
  * at one rank: inbox [0,nx-1]x[0,ny-1]x[0,nz-1], outbox [0,nx/2]x[0,ny-1]x[0,nz-1].
- * flags: reserved, pass 0.
+ * flags: 0, or PFC_HIP_FLAG_F64_BUTTERFLIES with PFC_HIP_F32.
  */
 int pfc_hip_plan_create(int nx, int ny, int nz, int precision, int flags,
                         pfc_hip_plan_t *plan);

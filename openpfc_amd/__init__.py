@@ -16,13 +16,13 @@ import os
 import time
 
 from . import _lib
-from ._lib import (F32, F64, MODEL_PFC, MODEL_TUNGSTEN, PFC_HIP_OK, TUNGSTEN_NPARAMS,
-                   PfcHipError, check, load)
+from ._lib import (F32, F64, FLAG_F64_BUTTERFLIES, MODEL_PFC, MODEL_TUNGSTEN, PFC_HIP_OK,
+                   TUNGSTEN_NPARAMS, PfcHipError, check, load)
 from .ensemble import Ensemble, Run
 from .tungsten import TungstenParams
 
 __all__ = ["HipFft", "PfcStepper", "TungstenStepper", "TungstenParams", "Ensemble", "Run",
-           "PfcHipError", "F32", "F64", "load"]
+           "PfcHipError", "F32", "F64", "FLAG_F64_BUTTERFLIES", "load"]
 
 
 def _stream_ptr(stream) -> ctypes.c_void_p:
@@ -43,13 +43,14 @@ class HipFft:
     (nz, ny, nx//2+1) complex tensors.  ``precision`` is F64 or F32.
     """
 
-    def __init__(self, nx: int, ny: int, nz: int, precision: int = F64):
+    def __init__(self, nx: int, ny: int, nz: int, precision: int = F64, flags: int = 0):
         self._lib = load()
         h = ctypes.c_void_p()
-        check(self._lib.pfc_hip_plan_create(nx, ny, nz, precision, 0, ctypes.byref(h)))
+        check(self._lib.pfc_hip_plan_create(nx, ny, nz, precision, int(flags), ctypes.byref(h)))
         self._h = h
         self.n = (nx, ny, nz)
         self.precision = precision
+        self.flags = int(flags)
         self._fft_time = 0.0
 
     # -- lifetime -----------------------------------------------------------
@@ -254,6 +255,8 @@ class _Stepper:
         layouts = self.fft.variants("step_layout")
         if "transposed" not in layouts:
             roles = roles[:self.NATURAL_ROLES]
+        elif "natural" not in layouts:  # transposed only (FP64-butterfly plans)
+            roles = roles[self.NATURAL_ROLES:]
         names = {r: self.fft.variants(r) for r in roles}
         total = sum(len(v) for v in names.values()) + 2
         out = (ctypes.c_double * total)()
@@ -264,7 +267,7 @@ class _Stepper:
             res[r] = [(n, out[i + k]) for k, n in enumerate(names[r])]
             i += len(names[r])
         res["layout_ms"] = {"transposed": out[i], "natural": out[i + 1]}
-        res["chosen"] = {r: names[r][self.fft.get_variant(r)] for r in roles}
+        res["chosen"] = {r: names[r][self.fft.get_variant(r)] for r in roles if names[r]}
         res["chosen"]["step_layout"] = self.fft.variants("step_layout")[
             self.fft.get_variant("step_layout")]
         return res

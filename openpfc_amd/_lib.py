@@ -24,6 +24,8 @@ IO_COMPLEX = 1
 
 F64 = 0
 F32 = 1
+# plan flag: FP64 butterflies on FP32 fields (pfc_hip.h PFC_HIP_FLAG_F64_BUTTERFLIES)
+FLAG_F64_BUTTERFLIES = 1
 MODEL_PFC = 0
 MODEL_TUNGSTEN = 1
 TUNGSTEN_NPARAMS = 17

@@ -44,7 +44,8 @@ def _run(cmd, quiet=True):
 
 
 SOURCES = ["pfc_hip.hip", "variants_common.hip", "variants_f64.hip", "variants_f32.hip",
-           "variants_t_f64.hip", "variants_t_f32.hip", "field_io.hip", "ensemble.hip"]
+           "variants_t_f64.hip", "variants_t_f32.hip", "variants_t_xp.hip", "field_io.hip",
+           "ensemble.hip"]
 OBJDIR = os.path.join(HERE, "_build")
 
 

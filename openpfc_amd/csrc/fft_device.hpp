@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace pfc_amd {
 
@@ -247,12 +248,18 @@ template <int DIR, typename T> struct Dft<12, DIR, T> {
 // ---------------------------------------------------------------------------
 template <int... Rs> struct RadixList {};
 
-template <int N_, int E_, typename List> struct FftPlan;
-template <int N_, int E_, int... Rs> struct FftPlan<N_, E_, RadixList<Rs...>> {
+// XP_: the butterflies (twiddle products and the in-register DFTs) run in
+// double whatever the value type, from a twiddle table of double entries;
+// values are rounded back to the value type after each radix pass.  Used by
+// single-precision fields whose transforms must keep FP64 arithmetic error
+// (the FP64-butterfly FP32-mixed step).
+template <int N_, int E_, typename List, bool XP_ = false> struct FftPlan;
+template <int N_, int E_, int... Rs, bool XP_> struct FftPlan<N_, E_, RadixList<Rs...>, XP_> {
   static constexpr int N = N_;
   static constexpr int E = E_;
   static constexpr int TL = N_ / E_; // threads per line
   static constexpr int NPASS = sizeof...(Rs);
+  static constexpr bool XP = XP_;
   using List = RadixList<Rs...>;
   static_assert((Rs * ... * 1) == N_, "radices must multiply to N");
   static_assert(((E_ % Rs == 0) && ...), "every radix must divide E");
@@ -319,34 +326,34 @@ struct Pass {
   static constexpr int NB = E / R; // butterflies per thread
   static constexpr bool LAST = (NS * R == N);
 
+  // arithmetic type of the butterflies (double for XP plans)
+  using C = std::conditional_t<Plan::XP, double, T>;
+
   __device__ __forceinline__ static void run(cplx<T> (&v)[E], int t,
                                              const LdsLine<T> &lds,
                                              const cplx<T> *__restrict__ tw) {
     // butterflies; element q of butterfly b is v[b + NB*q]
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      cplx<T> a[R];
+      cplx<C> a[R];
 #pragma unroll
-      for (int q = 0; q < R; ++q) a[q] = v[b + NB * q];
+      for (int q = 0; q < R; ++q) a[q] = cplx<C>{(C)v[b + NB * q].x, (C)v[b + NB * q].y};
       const int j = t + TL * b;
       const int k = j % NS;
       if constexpr (NS > 1) {
-        const cplx<T> *w = tw + TWOFF + k * (R - 1);
+        // XP plans read a table of double entries through the same pointer
+        const cplx<C> *w = reinterpret_cast<const cplx<C> *>(tw) + TWOFF + k * (R - 1);
 #pragma unroll
         for (int q = 1; q < R; ++q) {
-          cplx<T> wq = w[q - 1];
+          cplx<C> wq = w[q - 1];
           a[q] = (DIR < 0) ? c_mul(a[q], wq) : c_mulc(a[q], wq);
         }
       }
-      Dft<R, DIR, T>::run(a);
-      if constexpr (LAST) {
+      Dft<R, DIR, C>::run(a);
+      // outputs go back to slot b + NB*q (parked there until the exchange
+      // unless this is the last pass)
 #pragma unroll
-        for (int q = 0; q < R; ++q) v[b + NB * q] = a[q];
-      } else {
-        // park outputs in v (slot b + NB*q) until the exchange
-#pragma unroll
-        for (int q = 0; q < R; ++q) v[b + NB * q] = a[q];
-      }
+      for (int q = 0; q < R; ++q) v[b + NB * q] = cplx<T>{(T)a[q].x, (T)a[q].y};
     }
     if constexpr (!LAST) {
       exchange(v, t, lds);

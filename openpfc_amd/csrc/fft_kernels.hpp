@@ -247,8 +247,13 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_x_c2r(XArgs<T> a) {
 template <typename T> struct Nonlin {
   T c1, c2; // 3m, 3m^2 (used when split != 0)
   int split;
+  double c1d, c2d; // the same constants unrounded (the double evaluation)
   __device__ __forceinline__ T operator()(T u) const {
     return split ? u * (c2 + (c1 + u) * u) : u * u * u;
+  }
+  // in double, from the unrounded constants (FP64-butterfly kernels)
+  __device__ __forceinline__ double exact(double u) const {
+    return split ? u * (c2d + (c1d + u) * u) : u * u * u;
   }
 };
 

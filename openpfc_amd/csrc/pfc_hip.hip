@@ -16,6 +16,7 @@
 #include <string>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -124,6 +125,8 @@ struct pfc_hip_plan_s {
   void *adj_a = nullptr, *adj_b = nullptr, *adj_part = nullptr;
   // column maxima (MovingBC): kColParts partial rows of nx, then the result
   void *colmax = nullptr;
+  // exp(-2 pi i k/nx) in double for the C2R merge of FP64-butterfly plans
+  void *wn64 = nullptr;
   bool ops_ready = false;
   int model_kind = -1;
   double dt = 0;
@@ -169,6 +172,9 @@ struct pfc_hip_plan_s {
   bool tungsten() const { return model_kind == PFC_HIP_MODEL_TUNGSTEN; }
   int step_passes() const { return tungsten() ? 9 : 5; }
   bool t_ok = false; // transposed step supported for these extents
+  // PFC_HIP_FLAG_F64_BUTTERFLIES: single-precision fields whose transposed
+  // PFC passes run their butterflies in double (t_variants_xp lists)
+  bool xp = false;
 };
 
 namespace {
@@ -217,6 +223,26 @@ int upload_line_tables(pfc_hip_plan_s *p, int n) {
   return 0;
 }
 
+// Twiddles of the double-butterfly plans of length n: double entries at
+// plan ids kXpPid + i, whatever the plan's precision.
+int upload_line_tables_xp(pfc_hip_plan_s *p, int n) {
+  for (int i = 0; i < plan_count_xp(n); ++i) {
+    const int pid = kXpPid + i;
+    auto key = std::make_pair(n, pid);
+    if (p->tw.count(key)) continue;
+    const std::vector<long double> v = line_twiddles(n, pid);
+    void *d = nullptr;
+    if (int rc = dalloc(p, &d, v.size() * sizeof(double), "line twiddles (double)")) return rc;
+    p->tw[key] = d;
+    if (!v.empty()) {
+      std::vector<double> h(v.begin(), v.end());
+      PFC_HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice),
+                  "line twiddles (double)");
+    }
+  }
+  return 0;
+}
+
 template <typename T> const cplx<T> *twp(pfc_hip_plan_s *p, int n, int pid) {
   auto it = p->tw.find(std::make_pair(n, pid));
   return it == p->tw.end() ? nullptr : (const cplx<T> *)it->second;
@@ -240,6 +266,8 @@ template <typename T> Nonlin<T> nonlin(const pfc_hip_plan_s *p) {
   n.split = p->mean_split ? 1 : 0;
   n.c1 = (T)(3.0 * p->mean);
   n.c2 = (T)(3.0 * p->mean * p->mean);
+  n.c1d = 3.0 * p->mean;
+  n.c2d = 3.0 * p->mean * p->mean;
   return n;
 }
 
@@ -250,7 +278,7 @@ void free_all(pfc_hip_plan_s *p) {
   void **bufs[] = {&p->wn,      &p->work,  &p->A,     &p->B,     &p->A1,
                    &p->B1,      &p->A2,    &p->B2,    &p->adj_lam, &p->adj_P,
                    &p->adj_N,   &p->adj_a, &p->adj_b, &p->adj_part, &p->m_eps,
-                   &p->m_nl,    &p->colmax};
+                   &p->m_nl,    &p->colmax, &p->wn64};
   for (void **b : bufs) {
     if (*b) (void)hipFree(*b);
     *b = nullptr;
@@ -258,12 +286,22 @@ void free_all(pfc_hip_plan_s *p) {
   p->bytes = 0;
 }
 
+// Variant list of a transposed-step kind: the double-butterfly passes on a
+// PFC_HIP_FLAG_F64_BUTTERFLIES plan (none for the Tungsten kinds and the fused
+// boundary there), the plan precision's list otherwise.
+template <typename T> const std::vector<TV<T>> &t_list(const pfc_hip_plan_s *p, int kind, int n) {
+  if constexpr (std::is_same_v<T, float>) {
+    if (p->xp) return t_variants_xp(kind, n);
+  }
+  return t_variants<T>(kind, n);
+}
+
 template <typename T> int variant_count(pfc_hip_plan_s *p, int role) {
   const int n = p->role_len(role);
-  if (role == ROLE_STEP_LAYOUT) return p->t_ok ? 2 : 1;
+  if (role == ROLE_STEP_LAYOUT) return p->t_ok && !p->xp ? 2 : 1;
   if (role == ROLE_STEP_FUSE)
-    return p->t_ok && !t_variants<T>(TK_X_FUSE, p->role_len(ROLE_T_X_FUSE)).empty() ? 2 : 1;
-  if (t_kind_of_role(role) >= 0) return (int)t_variants<T>(t_kind_of_role(role), n).size();
+    return p->t_ok && !t_list<T>(p, TK_X_FUSE, p->role_len(ROLE_T_X_FUSE)).empty() ? 2 : 1;
+  if (t_kind_of_role(role) >= 0) return (int)t_list<T>(p, t_kind_of_role(role), n).size();
   switch (role) {
   case ROLE_X_R2C: return (int)xr2c_variants<T>(n).size();
   case ROLE_X_C2R: return (int)xc2r_variants<T>(n).size();
@@ -282,7 +320,7 @@ template <typename T> const std::string *variant_name(pfc_hip_plan_s *p, int rol
   if (v < 0 || v >= variant_count<T>(p, role)) return nullptr;
   if (role == ROLE_STEP_LAYOUT) return &layouts[p->t_ok ? v : 1];
   if (role == ROLE_STEP_FUSE) return &fuse[v];
-  if (t_kind_of_role(role) >= 0) return &t_variants<T>(t_kind_of_role(role), n)[v].name;
+  if (t_kind_of_role(role) >= 0) return &t_list<T>(p, t_kind_of_role(role), n)[v].name;
   switch (role) {
   case ROLE_X_R2C: return &xr2c_variants<T>(n)[v].name;
   case ROLE_X_C2R: return &xc2r_variants<T>(n)[v].name;
@@ -475,7 +513,8 @@ template <typename T> TStepArgs<T> pfc_t_args(pfc_hip_plan_s *p, void *psi) {
   a.B1 = (cplx<T> *)p->B1;
   a.A2 = (cplx<T> *)p->A2;
   a.B2 = (cplx<T> *)p->B2;
-  a.wn = (const cplx<T> *)p->wn;
+  // FP64-butterfly kernels read their merge twiddles as double entries
+  a.wn = (const cplx<T> *)(p->xp ? p->wn64 : p->wn);
   a.op = p->op;
   a.nl = nonlin<T>(p);
   a.zero_k0 = p->mean_split ? 1 : 0;
@@ -499,7 +538,7 @@ int t_pass(pfc_hip_plan_s *p, TStepArgs<T> a, int k, hipStream_t s) {
                                 "step y inverse", "step x inverse", "step boundary x"};
   const int role = kTPassRoles[k];
   const int n = p->role_len(role);
-  const auto &v = t_variants<T>(t_kind_of_role(role), n)[p->var[role]];
+  const auto &v = t_list<T>(p, t_kind_of_role(role), n)[p->var[role]];
   const cplx<T> *tw = twp<T>(p, v.half ? n / 2 : n, v.plan_id);
   a.tw_x = a.tw_xm = a.tw_y = a.tw_z = tw;
   if (v.plan_id2 >= 0) a.tw_x = twp<T>(p, 2 * n, v.plan_id2);
@@ -524,7 +563,7 @@ int step_transposed(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev)
 // Whether multi-step PFC runs fuse their step boundaries (role 16).
 template <typename T> bool fused_steps(pfc_hip_plan_s *p) {
   return p->transposed() && !p->tungsten() && p->var[ROLE_STEP_FUSE] == 1 &&
-         !t_variants<T>(TK_X_FUSE, p->role_len(ROLE_T_X_FUSE)).empty();
+         !t_list<T>(p, TK_X_FUSE, p->role_len(ROLE_T_X_FUSE)).empty();
 }
 
 // nsteps >= 1 PFC steps with fused boundaries: forward x once, then per
@@ -663,7 +702,7 @@ int step_tung_transposed(pfc_hip_plan_s *p, void *psi, void *mf, hipStream_t s,
   for (int k = 0; k < 9; ++k) {
     const TPass &ps = passes[k];
     const int n = p->role_len(ps.role);
-    const auto &v = t_variants<T>(t_kind_of_role(ps.role), n)[p->var[ps.role]];
+    const auto &v = t_list<T>(p, t_kind_of_role(ps.role), n)[p->var[ps.role]];
     TStepArgs<T> b = a;
     b.mode = ps.mode;
     if (ps.on_b) {
@@ -868,8 +907,11 @@ int pfc_hip_plan_create(int nx, int ny, int nz, int precision, int flags,
   if (precision != PFC_HIP_F64 && precision != PFC_HIP_F32)
     return fail(PFC_HIP_EINVAL, "pfc_hip_plan_create: unknown precision " +
                                     std::to_string(precision));
-  if (flags != 0)
-    return fail(PFC_HIP_EINVAL, "pfc_hip_plan_create: flags must be 0");
+  if (flags & ~PFC_HIP_FLAG_F64_BUTTERFLIES)
+    return fail(PFC_HIP_EINVAL, "pfc_hip_plan_create: unknown flags " + std::to_string(flags));
+  if ((flags & PFC_HIP_FLAG_F64_BUTTERFLIES) && precision != PFC_HIP_F32)
+    return fail(PFC_HIP_EINVAL,
+                "pfc_hip_plan_create: PFC_HIP_FLAG_F64_BUTTERFLIES needs PFC_HIP_F32");
   if (nx % 2 != 0 || !supported_len(nx / 2) || !supported_len(ny) ||
       !supported_len(nz))
     return fail(PFC_HIP_EUNSUPPORTED,
@@ -908,6 +950,34 @@ int pfc_hip_plan_create(int nx, int ny, int nz, int precision, int flags,
       ok = ok && c > 0;
     }
     p->t_ok = ok;
+  }
+  if (!rc && (flags & PFC_HIP_FLAG_F64_BUTTERFLIES)) {
+    p->xp = true;
+    bool ok = p->t_ok;
+    for (int r = ROLE_T_X_FWD; r <= ROLE_T_X_INV && ok; ++r)
+      ok = variant_count<float>(p, r) > 0;
+    if (!ok) {
+      free_all(p);
+      delete p;
+      return fail(PFC_HIP_EUNSUPPORTED,
+                  "pfc_hip_plan_create: PFC_HIP_FLAG_F64_BUTTERFLIES needs the transposed "
+                  "step (power-of-two extents, every axis >= 128, nx >= 256), got " +
+                      std::to_string(nx) + "x" + std::to_string(ny) + "x" + std::to_string(nz));
+    }
+    for (int n : {nx, nx / 2, ny, nz})
+      if (!rc) rc = upload_line_tables_xp(p, n);
+    if (!rc) {
+      const std::vector<long double> w = real_twiddles(nx);
+      rc = dalloc(p, &p->wn64, w.size() * sizeof(double), "real twiddles (double)");
+      if (!rc) {
+        std::vector<double> h(w.begin(), w.end());
+        hipError_t e = hipMemcpy(p->wn64, h.data(), h.size() * sizeof(double),
+                                 hipMemcpyHostToDevice);
+        if (e != hipSuccess)
+          rc = fail(PFC_HIP_EDEVICE, std::string("real twiddles (double): ") +
+                                         hipGetErrorString(e));
+      }
+    }
   }
   if (rc) {
     free_all(p);
@@ -1003,6 +1073,9 @@ int pfc_hip_pfc_set_operators(pfc_hip_plan_t plan, int model_kind,
   if (model_kind == PFC_HIP_MODEL_TUNGSTEN && plan->batch > 1)
     return fail(PFC_HIP_EUNSUPPORTED,
                 "pfc_hip_pfc_set_operators: batched plans run the PFC model only");
+  if (model_kind == PFC_HIP_MODEL_TUNGSTEN && plan->xp)
+    return fail(PFC_HIP_EUNSUPPORTED, "pfc_hip_pfc_set_operators: PFC_HIP_FLAG_F64_BUTTERFLIES "
+                                      "plans run the PFC model only");
   if (model_kind == PFC_HIP_MODEL_TUNGSTEN && (!params || nparams != PFC_HIP_TUNGSTEN_NPARAMS))
     return fail(PFC_HIP_EINVAL,
                 "pfc_hip_pfc_set_operators: Tungsten needs " +
@@ -1667,10 +1740,13 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
   double layout_ms[2] = {0, 0};
   double role_best[ROLE_COUNT];
   for (double &b : role_best) b = 1e300;
-  const int layouts = plan->t_ok ? 2 : 1;
+  // both layouts when the plan has both (natural first, slot order); a
+  // double-butterfly plan has the transposed one only
+  const bool two = plan->t_ok && !plan->xp;
+  const int layouts = two ? 2 : 1;
   for (int li = 0; li < layouts && !rc; ++li) {
-    const int layout = plan->t_ok ? 1 - li : 1; // natural first (slot order)
-    rc = pfc_hip_variant_set(plan, ROLE_STEP_LAYOUT, plan->t_ok ? layout : 0);
+    const int layout = two ? 1 - li : (plan->t_ok ? 0 : 1);
+    rc = pfc_hip_variant_set(plan, ROLE_STEP_LAYOUT, two ? layout : 0);
     for (int si = 0; si < nspans && !rc; ++si) {
       const RoleSpan &sp = spans[si];
       if (sp.layout != layout) continue;
@@ -1765,7 +1841,7 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
     }
     layout_ms[layout] = fastest;
   }
-  if (!rc && plan->t_ok) {
+  if (!rc && two) {
     rc = pfc_hip_variant_set(plan, ROLE_STEP_LAYOUT,
                              layout_ms[0] <= layout_ms[1] ? 0 : 1);
   }

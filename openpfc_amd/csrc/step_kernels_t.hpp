@@ -89,6 +89,20 @@ template <typename T> struct TStepArgs {
   int nx, ny, nz, W;
 };
 
+// Kernels on a plan with double butterflies (Plan::XP) also evaluate the
+// pointwise arithmetic that would otherwise carry a rounded constant in
+// double -- the mean-split nonlinearity (from the unrounded 3m, 3m^2), the
+// ETD combine (opL, opN unrounded) and the C2R merge (exp(2 pi i k/nx) from
+// a double table) -- so that no FP32-rounded coefficient acts on the field
+// step after step; each result is rounded once.
+template <typename Plan, typename T>
+__device__ __forceinline__ T nl_value(const Nonlin<T> &nl, T u) {
+  if constexpr (Plan::XP)
+    return (T)nl.exact((double)u);
+  else
+    return nl(u);
+}
+
 // ----------------------------------------------------------------------------
 // psi -> T1: G consecutive rows y0..y0+G-1 of plane z per block.
 // ----------------------------------------------------------------------------
@@ -109,7 +123,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd(TStepArgs<T> a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const T u = row[t + TL * e];
-    v[e] = cplx<T>{u, a.nl(u)};
+    v[e] = cplx<T>{u, nl_value<Plan>(a.nl, u)};
   }
   T *re = lds + g * RP;
   T *im = lds + (SPLIT ? 0 : PL) + g * RP;
@@ -282,15 +296,17 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
     // division need many temporaries), then the lines are loaded.  PRE =
     // true: both lines are requested first and the multipliers computed
     // while the loads are in flight.
-    T opl[E], opn[E];
+    // XP plans keep the multipliers unrounded (double) and combine in double
+    using M = std::conditional_t<Plan::XP, double, T>;
+    M opl[E], opn[E];
     cplx<T> v[E];
     auto multipliers = [&] {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         double ma, mb;
         z_mode_ops<MODE>(a.op, a.top, kxy, pfc_kc(t + TL * e, a.op.nz, a.op.fz), ma, mb);
-        opl[e] = (T)ma;
-        opn[e] = (T)mb;
+        opl[e] = (M)ma;
+        opn[e] = (M)mb;
       }
     };
     if constexpr (PRE) {
@@ -311,11 +327,19 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
     if constexpr (MODE == ZM_PFC) {
       // A = Fz^-1(opL*Fz(A) + opN*Fz(B))
       LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(acc, t, ll, a.tw_z);
+      if constexpr (!Plan::XP) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) acc[e] = c_scale(acc[e], opl[e]);
+        for (int e = 0; e < E; ++e) acc[e] = c_scale(acc[e], opl[e]);
+      }
       LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_z);
 #pragma unroll
-      for (int e = 0; e < E; ++e) acc[e] = c_add(acc[e], c_scale(v[e], opn[e]));
+      for (int e = 0; e < E; ++e) {
+        if constexpr (Plan::XP)
+          acc[e] = cplx<T>{(T)((double)acc[e].x * opl[e] + (double)v[e].x * opn[e]),
+                           (T)((double)acc[e].y * opl[e] + (double)v[e].y * opn[e])};
+        else
+          acc[e] = c_add(acc[e], c_scale(v[e], opn[e]));
+      }
       if (a.zero_k0 && l == 0 && t == 0) acc[0] = cplx<T>{0, 0};
     } else {
       // A = Fz^-1(expL*A + nW*Fz(B)), A already a full spectrum
@@ -323,7 +347,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const cplx<T> h = PRE ? acc[e] : ld_line<NT>(la + (t + TL * e));
-        acc[e] = c_add(c_scale(h, opl[e]), c_scale(v[e], opn[e]));
+        acc[e] = c_add(c_scale(h, (T)opl[e]), c_scale(v[e], (T)opn[e]));
       }
     }
   }
@@ -406,18 +430,21 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
     // direct strided loads of X[k] and X[M-k] of this line
     const int yy = y < a.ny ? y : y0;
     const cplx<T> *__restrict__ col = a.A1 + (long)z * a.W * a.ny + yy;
+    using C = std::conditional_t<Plan::XP, double, T>; // merge arithmetic
+    const cplx<C> *wn = reinterpret_cast<const cplx<C> *>(a.wn);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = t + TL * e;
-      cplx<T> xa = col[(long)k * a.ny];
-      cplx<T> xb = c_conj(col[(long)(M - k) * a.ny]);
+      const cplx<T> ca = col[(long)k * a.ny], cb = col[(long)(M - k) * a.ny];
+      cplx<C> xa{(C)ca.x, (C)ca.y};
+      cplx<C> xb{(C)cb.x, -(C)cb.y};
       if (k == 0) {
-        xa.y = (T)0;
-        xb.y = (T)0;
+        xa.y = (C)0;
+        xb.y = (C)0;
       }
-      const cplx<T> ev = c_add(xa, xb);
-      const cplx<T> od = c_mulc(c_sub(xa, xb), a.wn[k]);
-      v[e] = cplx<T>{ev.x - od.y, ev.y + od.x};
+      const cplx<C> ev = c_add(xa, xb);
+      const cplx<C> od = c_mulc(c_sub(xa, xb), wn[k]);
+      v[e] = cplx<T>{(T)(ev.x - od.y), (T)(ev.y + od.x)};
     }
   } else {
   // stage: sr/si[kx*(G+1) + line]
@@ -430,18 +457,20 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
     si[tile_idx<G>(kx, gg)] = w.y;
   }
   __syncthreads();
+  using C = std::conditional_t<Plan::XP, double, T>; // merge arithmetic
+  const cplx<C> *wn = reinterpret_cast<const cplx<C> *>(a.wn);
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = t + TL * e;
-    cplx<T> xa{sr[tile_idx<G>(k, g)], si[tile_idx<G>(k, g)]};
-    cplx<T> xb{sr[tile_idx<G>(M - k, g)], -si[tile_idx<G>(M - k, g)]};
+    cplx<C> xa{(C)sr[tile_idx<G>(k, g)], (C)si[tile_idx<G>(k, g)]};
+    cplx<C> xb{(C)sr[tile_idx<G>(M - k, g)], -(C)si[tile_idx<G>(M - k, g)]};
     if (k == 0) {
-      xa.y = (T)0;
-      xb.y = (T)0;
+      xa.y = (C)0;
+      xb.y = (C)0;
     }
-    const cplx<T> ev = c_add(xa, xb);
-    const cplx<T> od = c_mulc(c_sub(xa, xb), a.wn[k]);
-    v[e] = cplx<T>{ev.x - od.y, ev.y + od.x};
+    const cplx<C> ev = c_add(xa, xb);
+    const cplx<C> od = c_mulc(c_sub(xa, xb), wn[k]);
+    v[e] = cplx<T>{(T)(ev.x - od.y), (T)(ev.y + od.x)};
   }
   __syncthreads(); // the staging area becomes the FFT exchange area
   }
@@ -756,7 +785,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w(TStepArgs<T> a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const T u = ld_val<NT>(row + (t + TL * e));
-    v[e] = cplx<T>{u, a.nl(u)};
+    v[e] = cplx<T>{u, nl_value<Plan>(a.nl, u)};
   }
   fwd_split_store_wide<T, Plan, G, SPLIT, NT>(v, lds, t, g, z, y0, a);
 }

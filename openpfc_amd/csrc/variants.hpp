@@ -99,8 +99,17 @@ PFC_AMD_DECLARE_LISTS(float)
 
 // Radices of line plan `plan_id` of length n (host twiddle tables are built
 // per (length, plan) since the table layout follows the pass sequence).
+// Plan ids from kXpPid on are the double-butterfly plans (their tables hold
+// double entries whatever the field precision).
+constexpr int kXpPid = 64;
 std::vector<int> radices_for(int n, int plan_id);
 int plan_count(int n);
+int plan_count_xp(int n);
 bool supported_len(int n);
+
+// Transposed PFC passes (kinds TK_X_FWD .. TK_X_INV) of single-precision
+// plans made with PFC_HIP_FLAG_F64_BUTTERFLIES (empty for other kinds and
+// for lengths without such plans).
+const std::vector<TV<float>> &t_variants_xp(int kind, int n);
 
 } // namespace pfc_amd

@@ -21,7 +21,16 @@ int plan_count(int n) {
   return 0;
 }
 
+int plan_count_xp(int n) {
+  PFC_AMD_SWITCH_LEN(n, return count_of(typename PlansXP<N_>::L{});)
+  return 0;
+}
+
 std::vector<int> radices_for(int n, int pid) {
+  if (pid >= kXpPid) {
+    PFC_AMD_SWITCH_LEN(n, return radices_in(typename PlansXP<N_>::L{}, pid - kXpPid);)
+    return {};
+  }
   PFC_AMD_SWITCH_LEN(n, return radices_in(typename Plans<N_>::L{}, pid);)
   return {};
 }

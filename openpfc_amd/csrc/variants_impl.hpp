@@ -59,6 +59,30 @@ template <> struct Plans<1024> {
                      FftPlan<1024, 32, RadixList<16, 16, 4>>>;
 };
 
+// Plans with double butterflies (FftPlan XP = true) for the transposed PFC
+// passes of single-precision plans made with PFC_HIP_FLAG_F64_BUTTERFLIES;
+// their twiddle tables hold double entries and sit at plan id kXpPid + i.
+template <int N> struct PlansXP { using L = PlanList<>; };
+template <> struct PlansXP<128> {
+  using L = PlanList<FftPlan<128, 8, RadixList<8, 4, 4>, true>,
+                     FftPlan<128, 16, RadixList<16, 8>, true>>;
+};
+template <> struct PlansXP<256> {
+  using L = PlanList<FftPlan<256, 8, RadixList<8, 8, 4>, true>,
+                     FftPlan<256, 16, RadixList<16, 16>, true>,
+                     FftPlan<256, 4, RadixList<4, 4, 4, 4>, true>>;
+};
+template <> struct PlansXP<512> {
+  using L = PlanList<FftPlan<512, 8, RadixList<8, 8, 8>, true>,
+                     FftPlan<512, 16, RadixList<16, 16, 2>, true>,
+                     FftPlan<512, 4, RadixList<4, 4, 4, 4, 2>, true>>;
+};
+template <> struct PlansXP<1024> {
+  using L = PlanList<FftPlan<1024, 8, RadixList<8, 8, 4, 4>, true>,
+                     FftPlan<1024, 16, RadixList<16, 16, 4>, true>,
+                     FftPlan<1024, 4, RadixList<4, 4, 4, 4, 4>, true>>;
+};
+
 template <typename List> struct RadixOf;
 template <int... Rs> struct RadixOf<RadixList<Rs...>> {
   static std::vector<int> get() { return {Rs...}; }
@@ -68,7 +92,7 @@ template <typename Plan> std::string plan_name() {
   std::string s = "r";
   auto r = RadixOf<typename Plan::List>::get();
   for (size_t i = 0; i < r.size(); ++i) s += (i ? "x" : "") + std::to_string(r[i]);
-  return s + "/E" + std::to_string(Plan::E);
+  return s + "/E" + std::to_string(Plan::E) + (Plan::XP ? "/xp" : "");
 }
 
 constexpr int kMaxLds = 147456; // bytes of static LDS a block may use
@@ -231,6 +255,9 @@ template <typename T, int N, typename F, typename... Ps>
 void for_t_plans(PlanList<Ps...>, F &&f) {
   int pid = 0;
   ((call_plan<T, true, Ps>(f, pid), ++pid), ...);
+}
+template <typename F, typename... Ps> void for_t_plans_xp(PlanList<Ps...>, F &&f) {
+  (f(Ps{}), ...);
 }
 
 // Lengths with several plans and tile shapes get the full sweep; the rest a
@@ -704,6 +731,64 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
           add_t_tung_wide_o<T, P, 8, 4>(v, pid);
       });
     }
+  }
+  return v;
+}
+
+// Transposed PFC passes on the double-butterfly plans: the launch shapes
+// the single-precision autotuner favours, a few per kind (16 or 8 lines per
+// block for the row and transposing passes, 4 when 16 lines exceed a block;
+// the z pass with the fewest lines that fill a wave, and twice that).
+template <int N, int KIND> std::vector<TV<float>> make_t_kind_xp() {
+  using T = float;
+  std::vector<TV<T>> v;
+  int pid = kXpPid;
+  auto each = [&](auto f) {
+    pid = kXpPid;
+    for_t_plans_xp(typename PlansXP<N>::L{}, [&](auto p) { f(p, pid++); });
+  };
+  if constexpr (KIND == TK_X_FWD) {
+    each([&](auto p, int id) {
+      using P = decltype(p);
+      add_t_fwd_w<T, P, 16, true>(v, id);
+      add_t_fwd_w<T, P, 16, false>(v, id);
+      add_t_fwd_w<T, P, 8, true>(v, id);
+      add_t_fwd_w<T, P, 16, true, true>(v, id);
+      add_t<T, P, 16, true, true, KIND>(v, id);
+      if constexpr (16 * P::TL > kMaxThreads) add_t_fwd_w<T, P, 4, true>(v, id);
+    });
+  }
+  if constexpr (KIND == TK_Y_FWD || KIND == TK_Y_INV) {
+    each([&](auto p, int id) {
+      using P = decltype(p);
+      add_t_y_w<T, P, 16, true, 1, KIND>(v, id);
+      add_t_y_w<T, P, 16, true, 2, KIND>(v, id);
+      add_t_y_w<T, P, 8, true, 2, KIND>(v, id);
+      add_t_y_w<T, P, 16, true, 2, KIND, true>(v, id);
+      add_t<T, P, 16, true, true, KIND>(v, id);
+      if constexpr (16 * P::TL > kMaxThreads) add_t_y_w<T, P, 4, true, 2, KIND>(v, id);
+    });
+  }
+  if constexpr (KIND == TK_Z) {
+    each([&](auto p, int id) {
+      using P = decltype(p);
+      constexpr int G = P::TL >= 64 ? 1 : 64 / P::TL;
+      add_t<T, P, G, true, true, KIND>(v, id);
+      add_t<T, P, G, false, true, KIND>(v, id);
+      add_t_z_pre<T, P, G, false>(v, id);
+      add_t_z_nt<T, P, G, false, true>(v, id);
+      add_t_z_nt<T, P, G, true, false>(v, id);
+      add_t<T, P, 2 * G, true, true, KIND>(v, id);
+    });
+  }
+  if constexpr (KIND == TK_X_INV) {
+    each([&](auto p, int id) {
+      using P = decltype(p);
+      add_t_x_inv_nt<T, P, 16>(v, id);
+      add_t<T, P, 16, false, true, KIND>(v, id);
+      add_t<T, P, 8, false, true, KIND>(v, id);
+      if constexpr (16 * P::TL > kMaxThreads) add_t<T, P, 4, false, true, KIND>(v, id);
+    });
   }
   return v;
 }

@@ -46,11 +46,13 @@ def short_name(name: str) -> str:
         return plain.split("(")[0][:80]
     args = m.group(2)
     t = "double" if args.startswith("double") else ("float" if args.startswith("float") else "")
-    # one or two (paired) line plans: "N512/E8,r8x8x8[+N1024/E16,r16x16x4]"
-    plans = [f"N{n}/E{e},r" + r.replace(", ", "x")
-             for n, e, r in re.findall(r"FftPlan<(\d+), (\d+), pfc_amd::RadixList<([\d, ]+)> >",
-                                       args)]
-    rest = re.sub(r".*RadixList<[\d, ]+> >,?\s*", "", args)
+    # one or two (paired) line plans: "N512/E8,r8x8x8[+N1024/E16,r16x16x4]"; a
+    # plan with double butterflies (its last template argument true) is
+    # "N1024/E8/xp,r8x8x4x4"
+    plan_re = r"FftPlan<(\d+), (\d+), pfc_amd::RadixList<([\d, ]+)>(?:, (true|false))? ?>"
+    plans = [f"N{n}/E{e}{'/xp' if xp == 'true' else ''},r" + r.replace(", ", "x")
+             for n, e, r, xp in re.findall(plan_re, args)]
+    rest = re.sub(r".*RadixList<[\d, ]+>(?:, (?:true|false))? ?>,?\s*", "", args)
     desc = []
     if plans:
         desc.append("+".join(plans))

@@ -236,13 +236,43 @@ def test_modifier_factory_and_backend(tmp_path):
     bad = json.loads(json.dumps(doc))
     del bad["initial_conditions"][1]["amp_eq"]
     assert _config(tmp_path, bad).returncode != 0
-    bad = json.loads(json.dumps(doc))
-    bad["plan_options"] = {"backend": "fftw"}
-    r = _config(tmp_path, bad)
-    assert r.returncode != 0 and "backend" in r.stderr
-    ok = json.loads(json.dumps(doc))
-    ok["plan_options"] = {"backend": "rocm"}
-    assert _config(tmp_path, ok).returncode == 0
+    # the device models transform on the device whatever the base FFT is, as
+    # the reference's tungsten_hip does with tungsten_performance.json's fftw
+    for name in ("rocm", "hip", "fftw"):
+        ok = json.loads(json.dumps(doc))
+        ok["plan_options"] = {"backend": name, "use_reorder": True}
+        assert _config(tmp_path, ok).returncode == 0, name
+    for name, needle in (("cuda", "CUDA FFT backend"), ("opencl", "unknown FFT backend")):
+        bad = json.loads(json.dumps(doc))
+        bad["plan_options"] = {"backend": name}
+        r = _config(tmp_path, bad)
+        assert r.returncode != 0 and needle in r.stderr, (name, r.stderr)
+
+
+def test_seed_grid_random_seeds_and_moving_modifiers(tmp_path):
+    """The modifier types of the reference's registry beyond round one's:
+    seed_grid, random_seeds and moving, with the reference's field checks
+    (from_json_field_modifiers.hpp:65-197)."""
+    doc = tomli.loads(TOML_CONFIG)
+    doc["initial_conditions"] = [
+        {"target": "psi", "type": "constant", "n0": -0.10},
+        {"target": "psi", "type": "seed_grid", "X0": 30.0, "Ny": 2, "Nz": 2, "radius": 25.0,
+         "amplitude": 0.215936, "rho": -0.047},
+        {"target": "psi", "type": "random_seeds", "amplitude": 0.2, "rho": -0.05}]
+    doc["boundary_conditions"] = [
+        {"target": "psi", "type": "moving", "rho_low": -0.464, "rho_high": -0.10,
+         "width": 15.0, "alpha": 1.0, "disp": 40.0, "xpos": 127.17225402106772}]
+    r = _config(tmp_path, doc)
+    assert r.returncode == 0, r.stderr
+    for typ, key in (("seed_grid", "X0"), ("seed_grid", "Nz"), ("random_seeds", "rho"),
+                     ("moving", "xpos"), ("moving", "width")):
+        bad = json.loads(json.dumps(doc))
+        mods = bad["initial_conditions"] + bad["boundary_conditions"]
+        m = next(x for x in mods if x["type"] == typ)
+        del m[key]
+        r = _config(tmp_path, bad)
+        assert r.returncode != 0, (typ, key)
+        assert f"missing or invalid '{key}' field" in r.stderr, r.stderr
 
 
 def test_oracle_fixed_bc_closed_form():

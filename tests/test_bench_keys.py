@@ -12,10 +12,12 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 import bench  # noqa: E402
 from openpfc_amd.profile_summary import short_name  # noqa: E402
 
-PLAN8 = "pfc_amd::FftPlan<1024, 8, pfc_amd::RadixList<8, 8, 4, 4> >"
-PLAN16 = "pfc_amd::FftPlan<1024, 16, pfc_amd::RadixList<16, 16, 4> >"
-HALF4 = "pfc_amd::FftPlan<512, 4, pfc_amd::RadixList<4, 4, 4, 4, 2> >"
-HALF8 = "pfc_amd::FftPlan<512, 8, pfc_amd::RadixList<8, 8, 8> >"
+PLAN8 = "pfc_amd::FftPlan<1024, 8, pfc_amd::RadixList<8, 8, 4, 4>, false>"
+PLAN16 = "pfc_amd::FftPlan<1024, 16, pfc_amd::RadixList<16, 16, 4>, false>"
+HALF4 = "pfc_amd::FftPlan<512, 4, pfc_amd::RadixList<4, 4, 4, 4, 2>, false>"
+HALF8 = "pfc_amd::FftPlan<512, 8, pfc_amd::RadixList<8, 8, 8>, false>"
+PLAN8XP = "pfc_amd::FftPlan<1024, 8, pfc_amd::RadixList<8, 8, 4, 4>, true>"
+HALF8XP = "pfc_amd::FftPlan<512, 8, pfc_amd::RadixList<8, 8, 8>, true>"
 
 
 def demangled(kernel, t, plan, rest):
@@ -45,6 +47,12 @@ def demangled(kernel, t, plan, rest):
      demangled("k_tx_tung", "double", HALF8, "4, true, true")),
     ("k_tx_tung", "r8x8x8/E8/G8/wide/w4", 0,
      demangled("k_tx_tung_o", "double", HALF8, "8, true, true, 4")),
+    ("k_ty_fwd", "r8x8x4x4/E8/xp/G16/split/wide2/nt", 0,
+     demangled("k_ty_fwd_w", "float", PLAN8XP, "16, true, 2, true")),
+    ("k_tx_inv", "r8x8x8/E8/xp/G16/nt", 0,
+     demangled("k_tx_inv", "float", HALF8XP, "16, true, true")),
+    ("k_tz", "r8x8x4x4/E8/xp/G1/split/nt", 0,
+     demangled("k_tz", "float", PLAN8XP, "1, true, 0, false, true")),
 ])
 def test_variant_key_matches_profile_name(kernel, variant, mode, name):
     prec = "f32" if "<float" in name else "f64"
