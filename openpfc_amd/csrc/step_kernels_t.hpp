@@ -256,6 +256,37 @@ template <bool NT, typename T> __device__ __forceinline__ void st_line(cplx<T> *
   }
 }
 
+// Block-wide load of a ROWS x G transpose tile (sr/si at tile_idx<G>(row,
+// line)) from HBM, NTH threads, element i = row*G + line.  Every load of a
+// thread is issued before the first LDS write, so a thread keeps
+// ceil(ROWS*G / NTH) HBM requests in flight instead of one per loop trip:
+// the addresses of lines >= nvalid and of the trip past the tile's end are
+// clamped to element (0, 0) of the tile (always valid) and their values
+// dropped, which keeps the issue loop free of branches.  (The stores of a
+// tile stay a plain loop: a store does not stall the thread, and holding
+// the values of several trips costs registers the wide kernels do not have.)
+template <int ROWS, int G, int NTH, bool NT, typename T, typename Src>
+__device__ __forceinline__ void tile_load(T *sr, T *si, int nvalid, Src src) {
+  constexpr int TOT = ROWS * G, IT = (TOT + NTH - 1) / NTH;
+  cplx<T> w[IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int i = (int)threadIdx.x + it * NTH;
+    const int r = i / G, gg = i % G;
+    const bool ok = (TOT % NTH == 0 || i < TOT) && gg < nvalid;
+    const cplx<T> v = ld_line<NT>(ok ? src(r, gg) : src(0, 0));
+    w[it] = ok ? v : cplx<T>{(T)0, (T)0};
+  }
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int i = (int)threadIdx.x + it * NTH;
+    if (TOT % NTH == 0 || i < TOT) {
+      sr[tile_idx<G>(i / G, i % G)] = w[it].x;
+      si[tile_idx<G>(i / G, i % G)] = w[it].y;
+    }
+  }
+}
+
 template <typename T, typename Plan, int G, bool SPLIT, int MODE, bool PRE = false,
           bool NT = false>
 __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
@@ -449,13 +480,9 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
   } else {
   // stage: sr/si[kx*(G+1) + line]
   T *sr = lds, *si = lds + W * (G + 1);
-  for (int i = threadIdx.x; i < W * G; i += G * TL) {
-    const int kx = i / G, gg = i % G;
-    cplx<T> w{0, 0};
-    if (y0 + gg < a.ny) w = ld_line<NT>(a.A1 + (((long)z * a.W + kx) * a.ny + y0 + gg));
-    sr[tile_idx<G>(kx, gg)] = w.x;
-    si[tile_idx<G>(kx, gg)] = w.y;
-  }
+  const cplx<T> *__restrict__ src = a.A1 + (long)z * a.W * a.ny + y0;
+  tile_load<W, G, G * TL, NT>(sr, si, a.ny - y0,
+                              [&](int kx, int gg) { return src + ((long)kx * a.ny + gg); });
   __syncthreads();
   using C = std::conditional_t<Plan::XP, double, T>; // merge arithmetic
   const cplx<C> *wn = reinterpret_cast<const cplx<C> *>(a.wn);
@@ -621,13 +648,9 @@ __device__ __forceinline__ void tx_tung_body(const TStepArgs<T> &a, T *lds) {
     }
   } else {
     T *sr = lds, *si = lds + W * (G + 1);
-    for (int i = threadIdx.x; i < W * G; i += G * TL) {
-      const int kx = i / G, gg = i % G;
-      cplx<T> w{0, 0};
-      if (y0 + gg < a.ny) w = spec[((long)z * a.W + kx) * a.ny + y0 + gg];
-      sr[tile_idx<G>(kx, gg)] = w.x;
-      si[tile_idx<G>(kx, gg)] = w.y;
-    }
+    const cplx<T> *__restrict__ src = spec + (long)z * a.W * a.ny + y0;
+    tile_load<W, G, G * TL, false>(sr, si, a.ny - y0,
+                                   [&](int kx, int gg) { return src + ((long)kx * a.ny + gg); });
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -865,13 +888,9 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv_w(TStepArgs<T> a) {
   T *tr = lds, *ti = lds + ROWS * (G + 1);
 #pragma unroll
   for (int h = 0; h < NR; ++h) {
-    for (int i = threadIdx.x; i < ROWS * G; i += G * TL) {
-      const int row = i / G, gg = i % G;
-      cplx<T> w{0, 0};
-      if (z0 + gg < a.nz) w = ld_line<NT>(a.A2 + (((long)kx * N + h * ROWS + row) * a.nz + z0 + gg));
-      tr[tile_idx<G>(row, gg)] = w.x;
-      ti[tile_idx<G>(row, gg)] = w.y;
-    }
+    const cplx<T> *__restrict__ src = a.A2 + ((long)kx * N + h * ROWS) * a.nz + z0;
+    tile_load<ROWS, G, G * TL, NT>(tr, ti, a.nz - z0,
+                                   [&](int row, int gg) { return src + ((long)row * a.nz + gg); });
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < EPR; ++q) {
@@ -925,13 +944,9 @@ __global__ void __launch_bounds__(G *PF::TL) k_tx_fuse(TStepArgs<T> a) {
   cplx<T> v[EH];
   {
     T *sr = lds, *si = lds + W * (G + 1);
-    for (int i = threadIdx.x; i < W * G; i += G * TL) {
-      const int kx = i / G, gg = i % G;
-      cplx<T> w{0, 0};
-      if (y0 + gg < a.ny) w = a.A1[((long)z * a.W + kx) * a.ny + y0 + gg];
-      sr[tile_idx<G>(kx, gg)] = w.x;
-      si[tile_idx<G>(kx, gg)] = w.y;
-    }
+    const cplx<T> *__restrict__ src = a.A1 + (long)z * a.W * a.ny + y0;
+    tile_load<W, G, G * TL, false>(sr, si, a.ny - y0,
+                                   [&](int kx, int gg) { return src + ((long)kx * a.ny + gg); });
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < EH; ++e) {
