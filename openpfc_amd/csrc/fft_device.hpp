@@ -260,6 +260,8 @@ template <int N_, int E_, int... Rs, bool XP_> struct FftPlan<N_, E_, RadixList<
   static constexpr int TL = N_ / E_; // threads per line
   static constexpr int NPASS = sizeof...(Rs);
   static constexpr bool XP = XP_;
+  // every radix a power of two (the XOR-swizzled row exchanges need it)
+  static constexpr bool POW2 = (((Rs & (Rs - 1)) == 0) && ...);
   using List = RadixList<Rs...>;
   static_assert((Rs * ... * 1) == N_, "radices must multiply to N");
   static_assert(((E_ % Rs == 0) && ...), "every radix must divide E");
@@ -368,6 +370,67 @@ struct Pass {
     return (j / NS) * NS * R + (j % NS) + q * NS;
   }
 
+  // LDS slot of line position pos in this exchange.  The LDS serves a
+  // wavefront's 4-byte accesses in halves of 32 lanes over 32 banks, its
+  // 8-byte accesses in quarters of 16 lanes over 16 bank pairs.  One pad
+  // slot per 16 positions suits the 8-byte planes: a quarter reads 16
+  // consecutive positions and the writes of stride R <= 16 spread (only the
+  // middle exchanges of R <= 8 plans pair up).  For 4-byte planes of
+  // power-of-two plans whose line fills at least half a wavefront, where a
+  // half reads 32 consecutive positions (aligned to 32) and writes 32 / NS
+  // groups of NS consecutive positions A = NS*R apart:
+  //  - NS == 1 or NS >= 32: one pad slot per 32 positions (writes of stride R
+  //    <= 16 and contiguous writes spread; reads shift as a block);
+  //  - R == 16 (A >= 32): NS pad slots per A positions, which moves the
+  //    groups NS banks apart.
+  // Both stay within row_padded(N) and give every read and write address as
+  // a per-lane part plus a compile-time offset.  The remaining exchanges
+  // (1 < NS < 32 with R <= 8) keep one pad slot per 16 positions: an XOR
+  // swizzle would spread them as well, but its write addresses differ per
+  // output in their low bits, and the compiler then holds them in registers
+  // across the transforms of a pass.  Column layouts are unpadded.
+  static constexpr bool SWZ = ROWPAD && sizeof(T) == 4 && Plan::POW2 && TL >= 32 && N % 32 == 0;
+  static constexpr int SWZ_KIND = (NS == 1 || NS >= 32) ? 0 : (R == 16 ? 1 : 2);
+  __device__ __forceinline__ static int slot(int pos) {
+    if constexpr (!SWZ || SWZ_KIND == 2)
+      return pad_pos<ROWPAD>(pos);
+    else if constexpr (SWZ_KIND == 0)
+      return pos + (pos >> 5);
+    else
+      return pos + (pos / (NS * R)) * NS;
+  }
+  // slot(t + TL*e), t < TL, as a per-lane part plus a per-e constant (TL is
+  // a multiple of 32 where SWZ holds), so that the reads of all e share one
+  // address register and take the rest as an immediate offset.
+  __device__ __forceinline__ static int read_slot(int t, int e) {
+    if constexpr (!SWZ || SWZ_KIND == 2) {
+      return pad_pos<ROWPAD>(t + TL * e);
+    } else if constexpr (SWZ_KIND == 0) {
+      return t + (t >> 5) + e * (TL + TL / 32);
+    } else {
+      constexpr int A = NS * R;
+      if constexpr (TL % A == 0)
+        return t + (t / A) * NS + e * (TL + (TL / A) * NS);
+      else // A a multiple of TL: t + TL*e stays within one block of A
+        return t + TL * e + ((TL * e) / A) * NS;
+    }
+  }
+
+  // slot(dst_pos(t, b, q)) as a per-(lane, b) part plus q times a constant.
+  __device__ __forceinline__ static int write_slot(int t, int b, int q) {
+    if constexpr (!SWZ || SWZ_KIND == 2) {
+      return pad_pos<ROWPAD>(dst_pos(t, b, q));
+    } else {
+      const int d0 = dst_pos(t, b, 0);
+      if constexpr (SWZ_KIND == 1) // dst / A = d0 / A for every q
+        return d0 + (d0 / (NS * R)) * NS + q * NS;
+      else if constexpr (NS == 1) // d0 = R*j: d0 mod 32 + q < 32
+        return d0 + (d0 >> 5) + q;
+      else // NS >= 32: q*NS is a multiple of 32
+        return d0 + (d0 >> 5) + q * (NS + NS / 32);
+    }
+  }
+
   __device__ __forceinline__ static void exchange(cplx<T> (&v)[E], int t,
                                                   const LdsLine<T> &lds) {
     if constexpr (SPLIT) {
@@ -376,35 +439,35 @@ struct Pass {
       for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int q = 0; q < R; ++q)
-          lds.base[lds.idx(pad_pos<ROWPAD>(dst_pos(t, b, q)))] = v[b + NB * q].x;
+          lds.base[lds.idx(write_slot(t, b, q))] = v[b + NB * q].x;
       lds_sync<WAVE>();
 #pragma unroll
       for (int e = 0; e < E; ++e)
-        v[e].x = lds.base[lds.idx(pad_pos<ROWPAD>(t + TL * e))];
+        v[e].x = lds.base[lds.idx(read_slot(t, e))];
       lds_sync<WAVE>();
 #pragma unroll
       for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int q = 0; q < R; ++q)
-          lds.base[lds.idx(pad_pos<ROWPAD>(dst_pos(t, b, q)))] = v[b + NB * q].y;
+          lds.base[lds.idx(write_slot(t, b, q))] = v[b + NB * q].y;
       lds_sync<WAVE>();
 #pragma unroll
       for (int e = 0; e < E; ++e)
-        v[e].y = lds.base[lds.idx(pad_pos<ROWPAD>(t + TL * e))];
+        v[e].y = lds.base[lds.idx(read_slot(t, e))];
       lds_sync<WAVE>();
     } else {
 #pragma unroll
       for (int b = 0; b < NB; ++b)
 #pragma unroll
         for (int q = 0; q < R; ++q) {
-          const int i = lds.idx(pad_pos<ROWPAD>(dst_pos(t, b, q)));
+          const int i = lds.idx(write_slot(t, b, q));
           lds.base[i] = v[b + NB * q].x;
           lds.base[i + lds.plane] = v[b + NB * q].y;
         }
       lds_sync<WAVE>();
 #pragma unroll
       for (int e = 0; e < E; ++e) {
-        const int i = lds.idx(pad_pos<ROWPAD>(t + TL * e));
+        const int i = lds.idx(read_slot(t, e));
         v[e].x = lds.base[i];
         v[e].y = lds.base[i + lds.plane];
       }

@@ -38,10 +38,22 @@ template <typename Plan, int G> constexpr int t_plane() {
   return G * row_padded(Plan::N);
 }
 
-// Transpose tile: TL positions x G lines, one padding column per row so the
-// position-major writes of a line hit distinct banks.
-template <int G> __device__ __forceinline__ int tile_idx(int pos, int g) {
-  return pos * (G + 1) + g;
+// Transpose tile: positions x G lines (callers size it (G + 1) per position).
+// A line is written position-major (consecutive positions of one line
+// across the lanes) and the tile is read row-major (S / G positions x G
+// lines per group of S lanes).  The LDS serves a wavefront's 4-byte accesses
+// in halves of 32 lanes over 32 banks and its 8-byte accesses in quarters of
+// 16 lanes over 16 bank pairs, so S = 128 / sizeof(T) lanes must fall on S
+// distinct slots.  For power-of-two G <= S the line index is XORed with
+// (pos / (S / G)) mod G within a pitch of G, which does that for both
+// patterns (a padding column, pitch G + 1, spreads the writes, but the reads
+// of S / G rows then wrap onto the first).
+template <int G, typename T> __device__ __forceinline__ int tile_idx(int pos, int g) {
+  constexpr int S = 128 / (int)sizeof(T);
+  if constexpr (G <= S && (G & (G - 1)) == 0)
+    return pos * G + (g ^ ((pos / (S / G)) & (G - 1)));
+  else
+    return pos * (G + 1) + g;
 }
 
 // Fetch, for every owned element, the value at line position pp(k) of the
@@ -167,13 +179,13 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd(TStepArgs<T> a) {
 #pragma unroll
     for (int r = 0; r < E / 2; ++r) {
       const cplx<T> w = f ? pz[r] : v[r];
-      tr[tile_idx<G>(t, g)] = w.x;
-      ti[tile_idx<G>(t, g)] = w.y;
+      tr[tile_idx<G, T>(t, g)] = w.x;
+      ti[tile_idx<G, T>(t, g)] = w.y;
       __syncthreads();
       const int j = threadIdx.x / G, gg = threadIdx.x % G;
       if (y0 + gg < a.ny) {
         const long o = ((long)z * a.W + TL * r + j) * a.ny + y0 + gg;
-        out[o] = cplx<T>{tr[tile_idx<G>(j, gg)], ti[tile_idx<G>(j, gg)]};
+        out[o] = cplx<T>{tr[tile_idx<G, T>(j, gg)], ti[tile_idx<G, T>(j, gg)]};
       }
       __syncthreads();
     }
@@ -215,13 +227,13 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd(TStepArgs<T> a) {
   T *tr = lds, *ti = lds + TL * (G + 1);
 #pragma unroll
   for (int r = 0; r < E; ++r) {
-    tr[tile_idx<G>(t, g)] = v[r].x;
-    ti[tile_idx<G>(t, g)] = v[r].y;
+    tr[tile_idx<G, T>(t, g)] = v[r].x;
+    ti[tile_idx<G, T>(t, g)] = v[r].y;
     __syncthreads();
     const int j = threadIdx.x / G, gg = threadIdx.x % G;
     if (z0 + gg < a.nz) {
       const long o = ((long)kx * N + TL * r + j) * a.nz + z0 + gg;
-      out[o] = cplx<T>{tr[tile_idx<G>(j, gg)], ti[tile_idx<G>(j, gg)]};
+      out[o] = cplx<T>{tr[tile_idx<G, T>(j, gg)], ti[tile_idx<G, T>(j, gg)]};
     }
     __syncthreads();
   }
@@ -256,7 +268,7 @@ template <bool NT, typename T> __device__ __forceinline__ void st_line(cplx<T> *
   }
 }
 
-// Block-wide load of a ROWS x G transpose tile (sr/si at tile_idx<G>(row,
+// Block-wide load of a ROWS x G transpose tile (sr/si at tile_idx<G, T>(row,
 // line)) from HBM, NTH threads, element i = row*G + line.  Every load of a
 // thread is issued before the first LDS write, so a thread keeps
 // ceil(ROWS*G / NTH) HBM requests in flight instead of one per loop trip:
@@ -281,8 +293,8 @@ __device__ __forceinline__ void tile_load(T *sr, T *si, int nvalid, Src src) {
   for (int it = 0; it < IT; ++it) {
     const int i = (int)threadIdx.x + it * NTH;
     if (TOT % NTH == 0 || i < TOT) {
-      sr[tile_idx<G>(i / G, i % G)] = w[it].x;
-      si[tile_idx<G>(i / G, i % G)] = w[it].y;
+      sr[tile_idx<G, T>(i / G, i % G)] = w[it].x;
+      si[tile_idx<G, T>(i / G, i % G)] = w[it].y;
     }
   }
 }
@@ -417,10 +429,10 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv(TStepArgs<T> a) {
     const int j = threadIdx.x / G, gg = threadIdx.x % G;
     cplx<T> w{0, 0};
     if (z0 + gg < a.nz) w = a.A2[((long)kx * N + TL * r + j) * a.nz + z0 + gg];
-    tr[tile_idx<G>(j, gg)] = w.x;
-    ti[tile_idx<G>(j, gg)] = w.y;
+    tr[tile_idx<G, T>(j, gg)] = w.x;
+    ti[tile_idx<G, T>(j, gg)] = w.y;
     __syncthreads();
-    v[r] = cplx<T>{tr[tile_idx<G>(t, g)], ti[tile_idx<G>(t, g)]};
+    v[r] = cplx<T>{tr[tile_idx<G, T>(t, g)], ti[tile_idx<G, T>(t, g)]};
     __syncthreads();
   }
   const LdsLine<T> ll{lds, g * RP, 1, PL};
@@ -489,8 +501,8 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int k = t + TL * e;
-    cplx<C> xa{(C)sr[tile_idx<G>(k, g)], (C)si[tile_idx<G>(k, g)]};
-    cplx<C> xb{(C)sr[tile_idx<G>(M - k, g)], -(C)si[tile_idx<G>(M - k, g)]};
+    cplx<C> xa{(C)sr[tile_idx<G, T>(k, g)], (C)si[tile_idx<G, T>(k, g)]};
+    cplx<C> xb{(C)sr[tile_idx<G, T>(M - k, g)], -(C)si[tile_idx<G, T>(M - k, g)]};
     if (k == 0) {
       xa.y = (C)0;
       xb.y = (C)0;
@@ -535,13 +547,13 @@ __device__ __forceinline__ void t1_store_rows(const cplx<T> (&v)[E], cplx<T> xm,
     T *tr = lds, *ti = lds + TL * (G + 1);
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-      tr[tile_idx<G>(t, g)] = v[r].x;
-      ti[tile_idx<G>(t, g)] = v[r].y;
+      tr[tile_idx<G, T>(t, g)] = v[r].x;
+      ti[tile_idx<G, T>(t, g)] = v[r].y;
       __syncthreads();
       const int j = threadIdx.x / G, gg = threadIdx.x % G;
       if (y0 + gg < ny)
         out[((long)z * W + TL * r + j) * ny + y0 + gg] =
-            cplx<T>{tr[tile_idx<G>(j, gg)], ti[tile_idx<G>(j, gg)]};
+            cplx<T>{tr[tile_idx<G, T>(j, gg)], ti[tile_idx<G, T>(j, gg)]};
       __syncthreads();
     }
   }
@@ -557,19 +569,19 @@ __device__ __forceinline__ void t1_store_rows_wide(const cplx<T> (&v)[E], cplx<T
   __syncthreads(); // previous users of the LDS are done
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    sr[tile_idx<G>(t + TL * e, g)] = v[e].x;
-    si[tile_idx<G>(t + TL * e, g)] = v[e].y;
+    sr[tile_idx<G, T>(t + TL * e, g)] = v[e].x;
+    si[tile_idx<G, T>(t + TL * e, g)] = v[e].y;
   }
   if (t == 0) {
-    sr[tile_idx<G>(M, g)] = xm.x;
-    si[tile_idx<G>(M, g)] = xm.y;
+    sr[tile_idx<G, T>(M, g)] = xm.x;
+    si[tile_idx<G, T>(M, g)] = xm.y;
   }
   __syncthreads();
   for (int i = threadIdx.x; i < WR * G; i += G * TL) {
     const int kx = i / G, gg = i % G;
     if (y0 + gg < ny)
       out[((long)z * W + kx) * ny + y0 + gg] =
-          cplx<T>{sr[tile_idx<G>(kx, gg)], si[tile_idx<G>(kx, gg)]};
+          cplx<T>{sr[tile_idx<G, T>(kx, gg)], si[tile_idx<G, T>(kx, gg)]};
   }
 }
 
@@ -655,8 +667,8 @@ __device__ __forceinline__ void tx_tung_body(const TStepArgs<T> &a, T *lds) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = t + TL * e;
-      const cplx<T> xa{sr[tile_idx<G>(k, g)], si[tile_idx<G>(k, g)]};
-      const cplx<T> xb{sr[tile_idx<G>(M - k, g)], si[tile_idx<G>(M - k, g)]};
+      const cplx<T> xa{sr[tile_idx<G, T>(k, g)], si[tile_idx<G, T>(k, g)]};
+      const cplx<T> xb{sr[tile_idx<G, T>(M - k, g)], si[tile_idx<G, T>(M - k, g)]};
       v[e] = c2r_merge(xa, xb, a.wn[k], k == 0);
     }
     __syncthreads(); // the staging area becomes the FFT exchange area
@@ -761,13 +773,13 @@ __device__ __forceinline__ void fwd_split_store_wide(cplx<T> (&v)[Plan::E], T *l
 #pragma unroll
     for (int e = 0; e < E / 2; ++e) {
       const cplx<T> w = f ? pz[e] : v[e];
-      sr[tile_idx<G>(t + TL * e, g)] = w.x;
-      si[tile_idx<G>(t + TL * e, g)] = w.y;
+      sr[tile_idx<G, T>(t + TL * e, g)] = w.x;
+      si[tile_idx<G, T>(t + TL * e, g)] = w.y;
     }
     if (t == 0) { // k = N/2, element E/2 of thread 0
       const cplx<T> w = f ? pz[E / 2] : v[E / 2];
-      sr[tile_idx<G>(N / 2, g)] = w.x;
-      si[tile_idx<G>(N / 2, g)] = w.y;
+      sr[tile_idx<G, T>(N / 2, g)] = w.x;
+      si[tile_idx<G, T>(N / 2, g)] = w.y;
     }
     __syncthreads();
     cplx<T> *__restrict__ out = f ? a.B1 : a.A1;
@@ -775,7 +787,7 @@ __device__ __forceinline__ void fwd_split_store_wide(cplx<T> (&v)[Plan::E], T *l
       const int kx = i / G, gg = i % G;
       if (y0 + gg < a.ny)
         st_line<NT>(out + (((long)z * a.W + kx) * a.ny + y0 + gg),
-                    cplx<T>{sr[tile_idx<G>(kx, gg)], si[tile_idx<G>(kx, gg)]});
+                    cplx<T>{sr[tile_idx<G, T>(kx, gg)], si[tile_idx<G, T>(kx, gg)]});
     }
   }
 }
@@ -856,15 +868,15 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd_w(TStepArgs<T> a) {
 #pragma unroll
     for (int q = 0; q < EPR; ++q) {
       const int row = TL * q + t; // ky - h*ROWS
-      tr[tile_idx<G>(row, g)] = v[h * EPR + q].x;
-      ti[tile_idx<G>(row, g)] = v[h * EPR + q].y;
+      tr[tile_idx<G, T>(row, g)] = v[h * EPR + q].x;
+      ti[tile_idx<G, T>(row, g)] = v[h * EPR + q].y;
     }
     __syncthreads();
     for (int i = threadIdx.x; i < ROWS * G; i += G * TL) {
       const int row = i / G, gg = i % G;
       if (z0 + gg < a.nz)
         st_line<NT>(out + (((long)kx * N + h * ROWS + row) * a.nz + z0 + gg),
-                    cplx<T>{tr[tile_idx<G>(row, gg)], ti[tile_idx<G>(row, gg)]});
+                    cplx<T>{tr[tile_idx<G, T>(row, gg)], ti[tile_idx<G, T>(row, gg)]});
     }
   }
 }
@@ -895,7 +907,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv_w(TStepArgs<T> a) {
 #pragma unroll
     for (int q = 0; q < EPR; ++q) {
       const int row = TL * q + t;
-      v[h * EPR + q] = cplx<T>{tr[tile_idx<G>(row, g)], ti[tile_idx<G>(row, g)]};
+      v[h * EPR + q] = cplx<T>{tr[tile_idx<G, T>(row, g)], ti[tile_idx<G, T>(row, g)]};
     }
     __syncthreads(); // the tile is refilled (or becomes the FFT exchange area)
   }
@@ -951,8 +963,8 @@ __global__ void __launch_bounds__(G *PF::TL) k_tx_fuse(TStepArgs<T> a) {
 #pragma unroll
     for (int e = 0; e < EH; ++e) {
       const int k = t + TL * e;
-      v[e] = c2r_merge(cplx<T>{sr[tile_idx<G>(k, g)], si[tile_idx<G>(k, g)]},
-                       cplx<T>{sr[tile_idx<G>(M - k, g)], si[tile_idx<G>(M - k, g)]},
+      v[e] = c2r_merge(cplx<T>{sr[tile_idx<G, T>(k, g)], si[tile_idx<G, T>(k, g)]},
+                       cplx<T>{sr[tile_idx<G, T>(M - k, g)], si[tile_idx<G, T>(M - k, g)]},
                        a.wn[k], k == 0);
     }
     __syncthreads(); // the staging area becomes the FFT exchange area
