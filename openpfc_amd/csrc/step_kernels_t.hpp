@@ -38,7 +38,7 @@ template <typename Plan, int G> constexpr int t_plane() {
   return G * row_padded(Plan::N);
 }
 
-// Transpose tile: positions x G lines (callers size it (G + 1) per position).
+// Transpose tile: positions x G lines, tile_pitch<G, T>() slots per position.
 // A line is written position-major (consecutive positions of one line
 // across the lanes) and the tile is read row-major (S / G positions x G
 // lines per group of S lanes).  The LDS serves a wavefront's 4-byte accesses
@@ -47,10 +47,16 @@ template <typename Plan, int G> constexpr int t_plane() {
 // distinct slots.  For power-of-two G <= S the line index is XORed with
 // (pos / (S / G)) mod G within a pitch of G, which does that for both
 // patterns (a padding column, pitch G + 1, spreads the writes, but the reads
-// of S / G rows then wrap onto the first).
+// of S / G rows then wrap onto the first); other G keep the padding column.
+template <int G, typename T> constexpr bool tile_swizzled() {
+  return G <= 128 / (int)sizeof(T) && (G & (G - 1)) == 0;
+}
+template <int G, typename T> constexpr int tile_pitch() {
+  return tile_swizzled<G, T>() ? G : G + 1;
+}
 template <int G, typename T> __device__ __forceinline__ int tile_idx(int pos, int g) {
   constexpr int S = 128 / (int)sizeof(T);
-  if constexpr (G <= S && (G & (G - 1)) == 0)
+  if constexpr (tile_swizzled<G, T>())
     return pos * G + (g ^ ((pos / (S / G)) & (G - 1)));
   else
     return pos * (G + 1) + g;
@@ -172,7 +178,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd(TStepArgs<T> a) {
   }
   // transposed store of kx = TL*r + j, j < TL, r < E/2, through a TL x G tile
   __syncthreads(); // every line's transform is done with the LDS
-  T *tr = lds, *ti = lds + TL * (G + 1);
+  T *tr = lds, *ti = lds + TL * tile_pitch<G, T>();
 #pragma unroll
   for (int f = 0; f < 2; ++f) {
     cplx<T> *__restrict__ out = f ? a.B1 : a.A1;
@@ -224,7 +230,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd(TStepArgs<T> a) {
   }
   // transposed store: T2[(kx*ny + ky)*nz + z], ky = TL*r + j
   __syncthreads(); // every line's transform is done with the LDS
-  T *tr = lds, *ti = lds + TL * (G + 1);
+  T *tr = lds, *ti = lds + TL * tile_pitch<G, T>();
 #pragma unroll
   for (int r = 0; r < E; ++r) {
     tr[tile_idx<G, T>(t, g)] = v[r].x;
@@ -423,7 +429,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv(TStepArgs<T> a) {
 #pragma unroll
     for (int r = 0; r < E; ++r) v[r] = a.A2[((long)kx * N + TL * r + t) * a.nz + zz];
   }
-  T *tr = lds, *ti = lds + TL * (G + 1);
+  T *tr = lds, *ti = lds + TL * tile_pitch<G, T>();
 #pragma unroll
   for (int r = 0; r < (TILE ? E : 0); ++r) {
     const int j = threadIdx.x / G, gg = threadIdx.x % G;
@@ -451,7 +457,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv(TStepArgs<T> a) {
 // Plan is the M = nx/2 point plan.
 // ----------------------------------------------------------------------------
 template <typename T, typename Plan, int G, bool TILE> constexpr int tx_inv_lds() {
-  constexpr int stage = TILE ? 2 * (Plan::N + 1) * (G + 1) : 0;
+  constexpr int stage = TILE ? 2 * (Plan::N + 1) * tile_pitch<G, T>() : 0;
   constexpr int fft = 2 * t_plane<Plan, G>();
   return stage > fft ? stage : fft;
 }
@@ -491,7 +497,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_inv(TStepArgs<T> a) {
     }
   } else {
   // stage: sr/si[kx*(G+1) + line]
-  T *sr = lds, *si = lds + W * (G + 1);
+  T *sr = lds, *si = lds + W * tile_pitch<G, T>();
   const cplx<T> *__restrict__ src = a.A1 + (long)z * a.W * a.ny + y0;
   tile_load<W, G, G * TL, NT>(sr, si, a.ny - y0,
                               [&](int kx, int gg) { return src + ((long)kx * a.ny + gg); });
@@ -544,7 +550,7 @@ __device__ __forceinline__ void t1_store_rows(const cplx<T> (&v)[E], cplx<T> xm,
     }
   } else {
     __syncthreads(); // every line's transform is done with the LDS
-    T *tr = lds, *ti = lds + TL * (G + 1);
+    T *tr = lds, *ti = lds + TL * tile_pitch<G, T>();
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       tr[tile_idx<G, T>(t, g)] = v[r].x;
@@ -565,7 +571,7 @@ __device__ __forceinline__ void t1_store_rows_wide(const cplx<T> (&v)[E], cplx<T
                                                    cplx<T> *__restrict__ out, T *lds, int t,
                                                    int g, int z, int y0, int ny, int W) {
   constexpr int M = E * TL, WR = M + 1;
-  T *sr = lds, *si = lds + WR * (G + 1);
+  T *sr = lds, *si = lds + WR * tile_pitch<G, T>();
   __syncthreads(); // previous users of the LDS are done
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -659,7 +665,7 @@ __device__ __forceinline__ void tx_tung_body(const TStepArgs<T> &a, T *lds) {
       v[e] = c2r_merge(col[(long)k * a.ny], col[(long)(M - k) * a.ny], a.wn[k], k == 0);
     }
   } else {
-    T *sr = lds, *si = lds + W * (G + 1);
+    T *sr = lds, *si = lds + W * tile_pitch<G, T>();
     const cplx<T> *__restrict__ src = spec + (long)z * a.W * a.ny + y0;
     tile_load<W, G, G * TL, false>(sr, si, a.ny - y0,
                                    [&](int kx, int gg) { return src + ((long)kx * a.ny + gg); });
@@ -702,7 +708,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_tung(TStepArgs<T> a) {
   tx_tung_body<T, Plan, G, TILE, WIDE>(a, lds);
 }
 // The same pass with registers capped so that WPE waves fit on a SIMD: with
-// the 2 x (nx/2 + 1) x (G + 1) staging tile, G = 8 lines of an E = 8 plan
+// the 2 x (nx/2 + 1) x G staging tile, G = 8 lines of an E = 8 plan
 // fit two blocks on a CU only at <= 128 VGPRs, which the compiler reaches
 // without spilling when asked to.
 template <typename T, typename Plan, int G, bool TILE, bool WIDE, int WPE>
@@ -766,7 +772,7 @@ __device__ __forceinline__ void fwd_split_store_wide(cplx<T> (&v)[Plan::E], T *l
     v[e] = c_scale(sum, (T)0.5);
     pz[e] = cplx<T>{(T)0.5 * d.y, (T)-0.5 * d.x};
   }
-  T *sr = lds, *si = lds + W * (G + 1);
+  T *sr = lds, *si = lds + W * tile_pitch<G, T>();
 #pragma unroll
   for (int f = 0; f < 2; ++f) {
     __syncthreads(); // the previous users of the LDS are done
@@ -799,7 +805,7 @@ __device__ __forceinline__ void fwd_split_store_wide(cplx<T> (&v)[Plan::E], T *l
 // of two per E/2 rounds.
 // ----------------------------------------------------------------------------
 template <typename T, typename Plan, int G, bool SPLIT> constexpr int tx_fwd_w_lds() {
-  constexpr int stage = 2 * (Plan::N / 2 + 1) * (G + 1);
+  constexpr int stage = 2 * (Plan::N / 2 + 1) * tile_pitch<G, T>();
   constexpr int fft = (SPLIT ? 1 : 2) * t_plane<Plan, G>();
   return stage > fft ? stage : fft;
 }
@@ -833,7 +839,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd_w(TStepArgs<T> a) {
 // value of E.
 // ----------------------------------------------------------------------------
 template <typename T, typename Plan, int G, bool SPLIT, int NR> constexpr int ty_w_lds() {
-  constexpr int stage = 2 * (Plan::N / NR) * (G + 1);
+  constexpr int stage = 2 * (Plan::N / NR) * tile_pitch<G, T>();
   constexpr int fft = (SPLIT ? 1 : 2) * t_plane<Plan, G>();
   return stage > fft ? stage : fft;
 }
@@ -861,7 +867,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd_w(TStepArgs<T> a) {
   const LdsLine<T> ll{lds, g * RP, 1, PL};
   LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_y);
   // ky = TL*e + t; round h holds e in [h*EPR, (h+1)*EPR), i.e. ky in [h*ROWS, (h+1)*ROWS)
-  T *tr = lds, *ti = lds + ROWS * (G + 1);
+  T *tr = lds, *ti = lds + ROWS * tile_pitch<G, T>();
 #pragma unroll
   for (int h = 0; h < NR; ++h) {
     __syncthreads(); // previous users of the LDS are done
@@ -897,7 +903,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_inv_w(TStepArgs<T> a) {
   const int z = z0 + g;
 
   cplx<T> v[E];
-  T *tr = lds, *ti = lds + ROWS * (G + 1);
+  T *tr = lds, *ti = lds + ROWS * tile_pitch<G, T>();
 #pragma unroll
   for (int h = 0; h < NR; ++h) {
     const cplx<T> *__restrict__ src = a.A2 + ((long)kx * N + h * ROWS) * a.nz + z0;
@@ -939,8 +945,9 @@ template <typename T, typename PH, typename PF, int G, bool SPLIT> constexpr int
   return inv > xch ? (inv > fwd ? inv : fwd) : (xch > fwd ? xch : fwd);
 }
 
-template <typename T, typename PH, typename PF, int G, bool SPLIT>
-__global__ void __launch_bounds__(G *PF::TL) k_tx_fuse(TStepArgs<T> a) {
+// WPE > 1 caps the registers so that WPE waves fit on a SIMD.
+template <typename T, typename PH, typename PF, int G, bool SPLIT, int WPE = 1>
+__global__ void __launch_bounds__(G *PF::TL, WPE) k_tx_fuse(TStepArgs<T> a) {
   static_assert(PH::TL == PF::TL && PF::N == 2 * PH::N, "paired plans");
   constexpr int TL = PH::TL, EH = PH::E, EF = PF::E, M = PH::N;
   constexpr int RPH = row_padded(M), RPF = row_padded(PF::N);
@@ -955,7 +962,7 @@ __global__ void __launch_bounds__(G *PF::TL) k_tx_fuse(TStepArgs<T> a) {
   // inverse x: transposed load of the G rows, C2R merge, inverse transform
   cplx<T> v[EH];
   {
-    T *sr = lds, *si = lds + W * (G + 1);
+    T *sr = lds, *si = lds + W * tile_pitch<G, T>();
     const cplx<T> *__restrict__ src = a.A1 + (long)z * a.W * a.ny + y0;
     tile_load<W, G, G * TL, false>(sr, si, a.ny - y0,
                                    [&](int kx, int gg) { return src + ((long)kx * a.ny + gg); });

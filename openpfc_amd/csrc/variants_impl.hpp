@@ -465,10 +465,10 @@ hipError_t tx_tung_wide_o_launch(TStepArgs<T> a, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <typename T, typename PH, typename PF, int G, bool S>
+template <typename T, typename PH, typename PF, int G, bool S, int WPE = 1>
 hipError_t tx_fuse_launch(TStepArgs<T> a, hipStream_t s) {
   dim3 grid((unsigned)(a.nz * ((a.ny + G - 1) / G))), block(G * PF::TL);
-  hipLaunchKernelGGL((k_tx_fuse<T, PH, PF, G, S>), grid, block, 0, s, a);
+  hipLaunchKernelGGL((k_tx_fuse<T, PH, PF, G, S, WPE>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 
@@ -499,7 +499,7 @@ constexpr bool t_fits() {
   constexpr int fft = (S ? 1 : 2) * G * row_padded(Plan::N) * (int)sizeof(T);
   // row kernels on the nx/2-point plan use the full exchange only
   if (KIND == TK_X_INV || KIND == TK_X_TUNG) {
-    constexpr int stage = TILE ? 2 * (Plan::N + 1) * (G + 1) * (int)sizeof(T) : 0;
+    constexpr int stage = TILE ? 2 * (Plan::N + 1) * tile_pitch<G, T>() * (int)sizeof(T) : 0;
     constexpr int full = 2 * G * row_padded(Plan::N) * (int)sizeof(T);
     return !S && (stage > full ? stage : full) <= kMaxLds;
   }
@@ -606,14 +606,15 @@ void add_t_z_nt(std::vector<TV<T>> &v, int pid) {
                  pid, &tz_nt_launch<T, Plan, G, S, PRE>});
 }
 // Fused step boundary (k_tx_fuse): an nx/2-point plan PH paired with an
-// nx-point plan PF of the same threads per line
-template <typename T, typename PH, typename PF, int G, bool S>
+// nx-point plan PF of the same threads per line; WPE > 1 caps the registers
+// for WPE waves per SIMD (named /wN).
+template <typename T, typename PH, typename PF, int G, bool S, int WPE = 1>
 void add_t_fuse(std::vector<TV<T>> &v, int pid, int pid2) {
   if constexpr (PH::TL == PF::TL && G * PF::TL <= kMaxThreads && G * PF::TL >= 64 &&
                 tx_fuse_lds<T, PH, PF, G, S>() * (int)sizeof(T) <= kMaxLds) {
     TV<T> e{plan_name<PH>() + "+" + plan_name<PF>() + "/G" + std::to_string(G) +
-                (S ? "/split" : "/full"),
-            pid, &tx_fuse_launch<T, PH, PF, G, S>};
+                (S ? "/split" : "/full") + (WPE > 1 ? "/w" + std::to_string(WPE) : ""),
+            pid, &tx_fuse_launch<T, PH, PF, G, S, WPE>};
     e.plan_id2 = pid2;
     v.push_back(e);
   }
@@ -717,6 +718,11 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
             add_t_fuse<T, PH, PF, G0 / 2, false>(v, pid, pid2);
             add_t_fuse<T, PH, PF, G0, true>(v, pid, pid2);
             add_t_fuse<T, PH, PF, G0, false>(v, pid, pid2);
+            // four blocks of G0 / 2 lines per CU when the LDS allows it
+            if constexpr (4 * tx_fuse_lds<T, PH, PF, G0 / 2, true>() * (int)sizeof(T) <=
+                              160 * 1024 &&
+                          (G0 / 2) * PF::TL / 64 <= 8)
+              add_t_fuse<T, PH, PF, G0 / 2, true, 4 * (G0 / 2) * PF::TL / 256>(v, pid, pid2);
           });
         });
       }
