@@ -87,8 +87,12 @@ int pfc_hip_plan_create(int nx, int ny, int nz, int precision, int flags,
 /*
  * A plan over `batch` independent fields of one nx*ny*nz grid, stored one
  * after another (field b starts at b*nx*ny*nz values), stepped together by
- * pfc_hip_pfc_step / _step_timed / _autotune in the natural layout: one
- * launch per pass covers every field.  Each field has its own eps and mean
+ * pfc_hip_pfc_step / _step_timed / _autotune.  In the natural layout one
+ * launch per pass covers every field; in the transposed layout (the
+ * default where the extents have it) the fields are stepped one after
+ * another, every pass of a field before the next field, so that a field's
+ * spectra stay in the last-level cache between its passes; _step_timed then
+ * reports the passes of field 0.  Each field has its own eps and mean
  * (pfc_hip_pfc_set_members).  Entry points that act on one field (r2c, c2r,
  * initial and boundary conditions, operator tables, the adjoint, the
  * Tungsten model, set_mean) return PFC_HIP_EUNSUPPORTED on it; sizes are
@@ -229,8 +233,9 @@ int pfc_hip_pfc_set_mean(pfc_hip_plan_t plan, int enable, double mean);
  * values, or NULL to keep the operators' eps for every field.  mean_split
  * != 0 selects the mean-split step of pfc_hip_pfc_set_mean for every field,
  * with mean[b] the conserved average of field b (then not NULL).  On a plan
- * with batch > 1 eps and the means live in device memory read by the step
- * kernels; on a plan with batch 1 they are the plan's scalars.  An ensemble
+ * with batch > 1 eps and the means live in device memory read by the
+ * natural-layout kernels and are launch arguments of the transposed ones;
+ * on a plan with batch 1 they are the plan's scalars.  An ensemble
  * (pfc_hip_ensemble_step) captures its graph again when a group plan changed
  * anything its launches take as an argument (scalars, the mean-split mode,
  * variants), so later replays always use the values set here.  No reference

@@ -139,11 +139,17 @@ struct pfc_hip_plan_s {
   int var[ROLE_COUNT] = {};
   size_t bytes = 0;
   // Batched plans step `batch` fields of these extents stored one after
-  // another (natural layout only); per-field eps and (3m, 3m^2) live in
-  // m_eps (double) and m_nl (real type) once `members` is set.
+  // another.  The natural layout steps them all in each launch, reading
+  // per-field eps and (3m, 3m^2) from m_eps (double) and m_nl (real type)
+  // once `members` is set; the transposed layout steps them one after
+  // another (every pass of field 0, then of field 1, ...: a field's spectra
+  // stay in the last-level cache between its passes), through one set of
+  // T1/T2 buffers, with each field's eps and mean (h_eps, h_mean) as launch
+  // arguments.
   int batch = 1;
   void *m_eps = nullptr, *m_nl = nullptr;
   bool members = false;
+  std::vector<double> h_eps, h_mean;
   // Bumped by every call that changes a value the step's kernels receive as
   // a launch argument (operators, eps or mean held as scalars, the
   // mean-split mode, variants): a graph captured before such a change
@@ -261,14 +267,17 @@ int t_kind_of_role(int role) {
   return -1;
 }
 
-template <typename T> Nonlin<T> nonlin(const pfc_hip_plan_s *p) {
+template <typename T> Nonlin<T> nonlin_of(bool split, double mean) {
   Nonlin<T> n{};
-  n.split = p->mean_split ? 1 : 0;
-  n.c1 = (T)(3.0 * p->mean);
-  n.c2 = (T)(3.0 * p->mean * p->mean);
-  n.c1d = 3.0 * p->mean;
-  n.c2d = 3.0 * p->mean * p->mean;
+  n.split = split ? 1 : 0;
+  n.c1 = (T)(3.0 * mean);
+  n.c2 = (T)(3.0 * mean * mean);
+  n.c1d = 3.0 * mean;
+  n.c2d = 3.0 * mean * mean;
   return n;
+}
+template <typename T> Nonlin<T> nonlin(const pfc_hip_plan_s *p) {
+  return nonlin_of<T>(p->mean_split, p->mean);
 }
 
 void free_all(pfc_hip_plan_s *p) {
@@ -505,10 +514,13 @@ int step_buffers(pfc_hip_plan_s *p) {
   return rc;
 }
 
-template <typename T> TStepArgs<T> pfc_t_args(pfc_hip_plan_s *p, void *psi) {
+// Launch arguments of the transposed PFC step of field m of the plan (its
+// own eps and mean when the members are set).
+template <typename T> TStepArgs<T> pfc_t_args(pfc_hip_plan_s *p, void *psi, int m = 0) {
   TStepArgs<T> a{};
-  a.psi_in = (const T *)psi;
-  a.psi_out = (T *)psi;
+  T *field = (T *)psi + (size_t)m * p->inbox();
+  a.psi_in = field;
+  a.psi_out = field;
   a.A1 = (cplx<T> *)p->A1;
   a.B1 = (cplx<T> *)p->B1;
   a.A2 = (cplx<T> *)p->A2;
@@ -517,6 +529,10 @@ template <typename T> TStepArgs<T> pfc_t_args(pfc_hip_plan_s *p, void *psi) {
   a.wn = (const cplx<T> *)(p->xp ? p->wn64 : p->wn);
   a.op = p->op;
   a.nl = nonlin<T>(p);
+  if (p->members && p->batch > 1) {
+    a.op.eps = p->h_eps[(size_t)m];
+    a.nl = nonlin_of<T>(p->mean_split, p->h_mean[(size_t)m]);
+  }
   a.zero_k0 = p->mean_split ? 1 : 0;
   a.mode = ZM_PFC;
   a.nfields = 2;
@@ -546,16 +562,21 @@ int t_pass(pfc_hip_plan_s *p, TStepArgs<T> a, int k, hipStream_t s) {
   return 0;
 }
 
+// One transposed PFC step of every field of the plan, field after field.
+// `ev` (optional) brackets the passes of field 0: the pass times of one
+// field, as for a single plan.
 template <typename T>
 int step_transposed(pfc_hip_plan_s *p, void *psi, hipStream_t s, hipEvent_t *ev) {
-  auto mark = [&](int i) {
-    if (ev) (void)hipEventRecord(ev[i], s);
-  };
-  const TStepArgs<T> a = pfc_t_args<T>(p, psi);
-  mark(0);
-  for (int k = 0; k < 5; ++k) {
-    if (int rc = t_pass<T>(p, a, k, s)) return rc;
-    mark(k + 1);
+  for (int m = 0; m < p->batch; ++m) {
+    auto mark = [&](int i) {
+      if (ev && m == 0) (void)hipEventRecord(ev[i], s);
+    };
+    const TStepArgs<T> a = pfc_t_args<T>(p, psi, m);
+    mark(0);
+    for (int k = 0; k < 5; ++k) {
+      if (int rc = t_pass<T>(p, a, k, s)) return rc;
+      mark(k + 1);
+    }
   }
   return 0;
 }
@@ -570,24 +591,28 @@ template <typename T> bool fused_steps(pfc_hip_plan_s *p) {
 // step the y, z and inverse y passes, with the fused boundary pass between
 // steps, and the inverse x once at the end (4*nsteps + 1 launches).  `ev`
 // (optional) receives 4*nsteps + 2 events around the launches, in order.
+// (Batched plans: the nsteps steps of field 0, then of field 1, ...; `ev`
+// brackets those of field 0.)
 template <typename T>
 int steps_fused(pfc_hip_plan_s *p, void *psi, int nsteps, hipStream_t s, hipEvent_t *ev) {
-  int e = 0;
-  auto mark = [&]() {
-    if (ev) (void)hipEventRecord(ev[e++], s);
-  };
-  const TStepArgs<T> a = pfc_t_args<T>(p, psi);
-  mark();
-  for (int i = 0; i < nsteps; ++i) {
-    if (int rc = t_pass<T>(p, a, i == 0 ? 0 : 5, s)) return rc;
+  for (int m = 0; m < p->batch; ++m) {
+    int e = 0;
+    auto mark = [&]() {
+      if (ev && m == 0) (void)hipEventRecord(ev[e++], s);
+    };
+    const TStepArgs<T> a = pfc_t_args<T>(p, psi, m);
     mark();
-    for (int k = 1; k <= 3; ++k) {
-      if (int rc = t_pass<T>(p, a, k, s)) return rc;
+    for (int i = 0; i < nsteps; ++i) {
+      if (int rc = t_pass<T>(p, a, i == 0 ? 0 : 5, s)) return rc;
       mark();
+      for (int k = 1; k <= 3; ++k) {
+        if (int rc = t_pass<T>(p, a, k, s)) return rc;
+        mark();
+      }
     }
+    if (int rc = t_pass<T>(p, a, 4, s)) return rc;
+    mark();
   }
-  if (int rc = t_pass<T>(p, a, 4, s)) return rc;
-  mark();
   return 0;
 }
 
@@ -999,7 +1024,6 @@ int pfc_hip_plan_create_batched(int nx, int ny, int nz, int batch, int precision
   if (int rc = pfc_hip_plan_create(nx, ny, nz, precision, flags, plan)) return rc;
   pfc_hip_plan_s *p = *plan;
   p->batch = batch;
-  if (batch > 1) p->t_ok = false; // the transposed layouts step one field
   return 0;
 }
 
@@ -1839,7 +1863,9 @@ int pfc_hip_pfc_autotune(pfc_hip_plan_t plan, void *psi, int reps, void *stream,
       rc = run_step(pm);
       if (!rc && r > 0) fastest = std::min(fastest, span_ms(pm, ~0u));
     }
-    layout_ms[layout] = fastest;
+    // a batched plan's transposed step times its field 0 (the fields run
+    // one after another); the natural one times every field at once
+    layout_ms[layout] = layout == 0 ? fastest * plan->batch : fastest;
   }
   if (!rc && two) {
     rc = pfc_hip_variant_set(plan, ROLE_STEP_LAYOUT,
@@ -1917,9 +1943,11 @@ int pfc_hip_pfc_set_members(pfc_hip_plan_t plan, const double *eps, const double
     PFC_HIP_TRY(hipMemcpy(plan->m_nl, c2f.data(), c2f.size() * sizeof(float),
                           hipMemcpyHostToDevice),
                 "pfc_hip_pfc_set_members: mean upload");
-  // eps and the means are read from device memory; the mean-split mode
-  // (and the switch to the member arrays) is a launch argument
-  if (!plan->members || plan->mean_split != (mean_split != 0)) ++plan->gen;
+  plan->h_eps = e;
+  plan->h_mean = m;
+  // the natural layout reads eps and the means from device memory; the
+  // transposed one takes them, and the mean-split mode, as launch arguments
+  ++plan->gen;
   plan->members = true;
   plan->mean_split = mean_split != 0;
   plan->mean = 0.0;

@@ -2,9 +2,12 @@
 
 ``Ensemble`` wraps ``pfc_hip_ensemble_*`` (``include/pfc_hip.h``): runs
 with equal extents, spacing and dt form a group stepped by one batched plan
-(one launch per pass for all of the group's fields, per-run eps and mean in
-device memory), and one step of every group is captured once into a
-hipGraph, so each step of the whole ensemble is a single graph launch.
+(transposed layout where the extents have it: the group's fields one after
+another, each with its own eps and mean as launch arguments; else the
+natural layout: one launch per pass for all of the group's fields, per-run
+eps and mean in device memory), and one step of every group is captured
+once into a hipGraph, so each step of the whole ensemble is a single graph
+launch.
 The fields of all runs live in one device tensor owned by this object;
 ``field(r)`` is a (nz, ny, nx) view of run r.
 
@@ -21,6 +24,9 @@ from ._lib import F64, check, load
 
 PARAMS_PER_RUN = 6  # dx, dy, dz, eps, dt, mean
 _ROLES = {"x_c2r": 1, "col_y": 2, "pfc_x": 4, "pfc_z": 5}
+_T_ROLES = {"t_x_fwd": 7, "t_y_fwd": 8, "t_z": 9, "t_y_inv": 10, "t_x_inv": 11,
+            "t_x_fuse": 15, "step_fuse": 16}
+_LAYOUT = 6
 
 
 @dataclass(frozen=True)
@@ -120,8 +126,8 @@ class Ensemble:
         check(self._lib.pfc_hip_ensemble_autotune(self._h, int(reps), _stream_ptr(stream)))
 
     def groups(self) -> list[dict]:
-        """Per group: extents, number of runs, first value and the chosen
-        variant of every pass of the natural-layout step."""
+        """Per group: extents, number of runs, first value, the step layout
+        and the chosen variant of every pass of that layout's step."""
         n = ctypes.c_int()
         check(self._lib.pfc_hip_ensemble_groups(self._h, ctypes.byref(n)))
         out = []
@@ -129,11 +135,17 @@ class Ensemble:
             plan, batch, off = ctypes.c_void_p(), ctypes.c_int(), ctypes.c_size_t()
             check(self._lib.pfc_hip_ensemble_group(self._h, g, ctypes.byref(plan),
                                                    ctypes.byref(batch), ctypes.byref(off)))
-            chosen = {}
-            for name, role in _ROLES.items():
+            def name_of(role):
                 idx = ctypes.c_int()
                 check(self._lib.pfc_hip_variant_get(plan, role, ctypes.byref(idx)))
-                chosen[name] = self._lib.pfc_hip_variant_name(plan, role, idx.value).decode()
+                return self._lib.pfc_hip_variant_name(plan, role, idx.value).decode()
+            layout = name_of(_LAYOUT)
+            chosen = {"step_layout": layout}
+            for name, role in (_T_ROLES if layout == "transposed" else _ROLES).items():
+                cnt = ctypes.c_int()
+                check(self._lib.pfc_hip_variant_count(plan, role, ctypes.byref(cnt)))
+                if cnt.value > 0:
+                    chosen[name] = name_of(role)
             first = next(r for r, o in enumerate(self.offsets) if o == off.value)
             out.append({"ext": tuple(self.runs[first].ext), "batch": batch.value,
                         "offset": off.value, "variants": chosen})

@@ -5,7 +5,9 @@
 * The fused PFC step at those lengths against the oracle
   (docs/api/examples/10_complete_pfc_model.cpp:121-144), max rel <= 1e-10.
 * A batched plan of B fields gives, field by field, the bits of B single
-  plans with the same eps / mean (plain FP64 and mean-split FP32).
+  plans with the same eps / mean (plain FP64 and mean-split FP32; on
+  extents with the transposed step in both layouts, with fused step
+  boundaries and with FP64 butterflies).
 * Ensembles of uneven sizes (one graph launch per step): every run matches
   the oracle with its own eps; graph replays equal the eagerly enqueued
   step bit for bit, with eager work between replays; per-run eps changed
@@ -110,6 +112,64 @@ def test_batched_plan_equals_single_plans(torch_gpu, prec, split):
                 st.set_mean(mean[b])
             one = torch.from_numpy(fields[b]).to(rdt).cuda()
             st.step(one, 6)
+            assert torch.equal(one, stacked[b]), f"field {b} differs"
+    finally:
+        lib.pfc_hip_plan_destroy(h)
+
+
+@pytest.mark.parametrize("prec,split,flags", [("f64", False, 0), ("f32", True, 0),
+                                               ("f32", True, 1)])
+@pytest.mark.parametrize("layout,fuse", [(0, 0), (0, 1), (1, 0)])
+def test_batched_transposed_equals_single_plans(torch_gpu, prec, split, flags, layout, fuse):
+    """Extents with the transposed step: a batched plan steps its fields one
+    after another there (layout 0, with or without fused step boundaries,
+    FP64, FP32 and FP32 with FP64 butterflies) and all at once in the natural
+    layout (1); either way each field gets the bits of a single plan with the
+    same eps, mean and variants."""
+    import openpfc_amd as P
+    torch = torch_gpu
+    lib = P.load()
+    if flags and (layout == 1 or fuse):
+        pytest.skip("plans with FP64 butterflies have the transposed layout and single steps "
+                    "only")
+    n = (256, 128, 128)
+    print(f"batched transposed n={n} {prec} split={split} flags={flags} layout={layout} "
+          f"fuse={fuse}", flush=True)
+    eps = [-0.20, -0.31, -0.25]
+    mean = [-0.18, -0.22, -0.27]
+    pr = P.F64 if prec == "f64" else P.F32
+    rdt = torch.float64 if prec == "f64" else torch.float32
+    fields = [O.uniform_noise(n, 17 + b, -0.01, 0.01, base=0.0 if split else mean[b])
+              for b in range(3)]
+    h = ctypes.c_void_p()
+    P.check(lib.pfc_hip_plan_create_batched(*n, 3, pr, flags, ctypes.byref(h)))
+    try:
+        op = (ctypes.c_double * 4)(DX, DX, DX, eps[0])
+        P.check(lib.pfc_hip_pfc_set_operators(h, 0, op, 4, DT))
+        P.check(lib.pfc_hip_pfc_set_members(h, (ctypes.c_double * 3)(*eps),
+                                            (ctypes.c_double * 3)(*mean), int(split)))
+        cnt = ctypes.c_int()
+        P.check(lib.pfc_hip_variant_count(h, 6, ctypes.byref(cnt)))
+        assert cnt.value == (1 if flags else 2)
+        if not flags:
+            P.check(lib.pfc_hip_variant_set(h, 6, layout))
+        if fuse:
+            P.check(lib.pfc_hip_variant_set(h, 16, 1))
+        stacked = torch.from_numpy(np.stack(fields)).to(rdt).cuda().contiguous()
+        s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        P.check(lib.pfc_hip_pfc_step(h, ctypes.c_void_p(stacked.data_ptr()), 4, s))
+        for b in range(3):
+            f = P.HipFft(*n, precision=pr, flags=flags)
+            st = P.PfcStepper(f, (DX, DX, DX), eps[b])
+            st.initialize(DT)
+            if split:
+                st.set_mean(mean[b])
+            if not flags:
+                f.set_variant("step_layout", layout)
+            if fuse:
+                f.set_variant("step_fuse", 1)
+            one = torch.from_numpy(fields[b]).to(rdt).cuda()
+            st.step(one, 4)
             assert torch.equal(one, stacked[b]), f"field {b} differs"
     finally:
         lib.pfc_hip_plan_destroy(h)
@@ -285,8 +345,9 @@ def test_ensemble_autotune_keeps_fields_and_parity(torch_gpu):
     ens.autotune(reps=1)
     torch.cuda.synchronize()
     assert torch.equal(before, ens.buffer)
-    for g in ens.groups():
-        assert set(g["variants"]) == {"x_c2r", "col_y", "pfc_x", "pfc_z"}
+    for g in ens.groups():  # every size here is below the transposed step's 128
+        assert g["variants"]["step_layout"] == "natural"
+        assert set(g["variants"]) == {"step_layout", "x_c2r", "col_y", "pfc_x", "pfc_z"}
     ens.step(5)
     torch.cuda.synchronize()
     for i, r in enumerate(runs):
