@@ -644,7 +644,7 @@ def ensemble_bytes(sizes, esize):
     return sum(2 * esize * n ** 3 + 12 * 2 * esize * (n // 2 + 1) * n * n for n in sizes)
 
 
-def ensemble_run(P, torch, steps, warmup, nruns=64, seed=4, acc_steps=100):
+def ensemble_run(P, torch, steps, warmup, nruns=64, seed=4, acc_steps=100, trace_steps=0):
     """Ensemble of nruns independent plain-PFC runs of uneven cubic
     sizes (dx = pi/4, dt = 0.5), psi0 = mean + U(-0.01, 0.01) (torch's
     device generator, seed 1000*seed + run), FP32-mixed (each field holds
@@ -674,6 +674,21 @@ def ensemble_run(P, torch, steps, warmup, nruns=64, seed=4, acc_steps=100):
     torch.cuda.synchronize()
     tune_s = time.perf_counter() - t0
     log(f"ensemble: autotune {tune_s:.1f} s")
+    if trace_steps:
+        # for a kernel trace: the GPU idles 0.2 s, then trace_steps graph
+        # steps run back to back (the window profile_summary --gap-window
+        # finds after the last long gap)
+        ens.step(1, graph=True)
+        torch.cuda.synchronize()
+        time.sleep(0.2)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        ens.step(trace_steps, graph=True)
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / trace_steps
+        ens.close()
+        return {"trace_steps": trace_steps, "graph_ms_per_step": ms, "dof": dof}
 
     def timed(graph):
         ens.buffer.copy_(d0)
@@ -781,6 +796,9 @@ def main(argv=None):
     ap.add_argument("--ensemble", action="store_true",
                     help="64 uneven-size PFC runs (FP32-mixed) advanced "
                          "together, one hipGraph launch per step; --steps K are timed")
+    ap.add_argument("--ens-trace-steps", type=int, default=0, metavar="K",
+                    help="with --ensemble: after the autotune, idle 0.2 s, run K graph "
+                         "steps and stop (a kernel trace's window for the idle share)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also time a device-to-device copy of 2 x 4 GiB (the plain "
                          "copy kernel whose PMC byte counts scale the traffic figure)")
@@ -802,7 +820,8 @@ def main(argv=None):
         print(json.dumps(adjoint_run(P, torch)), flush=True)
         return 0
     if args.ensemble:
-        print(json.dumps(ensemble_run(P, torch, args.steps, args.warmup)), flush=True)
+        print(json.dumps(ensemble_run(P, torch, args.steps, args.warmup,
+                                      trace_steps=args.ens_trace_steps)), flush=True)
         return 0
     n, dx, dt, r = args.n, math.pi / 4, 0.5, -0.25
     if args.accuracy:

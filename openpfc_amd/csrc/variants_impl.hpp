@@ -718,11 +718,16 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
             add_t_fuse<T, PH, PF, G0 / 2, false>(v, pid, pid2);
             add_t_fuse<T, PH, PF, G0, true>(v, pid, pid2);
             add_t_fuse<T, PH, PF, G0, false>(v, pid, pid2);
-            // four blocks of G0 / 2 lines per CU when the LDS allows it
+            // registers capped so that the blocks the LDS allows all fit: four of
+            // G0 / 2 lines or two of G0 lines (128-byte stores) per CU
             if constexpr (4 * tx_fuse_lds<T, PH, PF, G0 / 2, true>() * (int)sizeof(T) <=
                               160 * 1024 &&
                           (G0 / 2) * PF::TL / 64 <= 8)
               add_t_fuse<T, PH, PF, G0 / 2, true, 4 * (G0 / 2) * PF::TL / 256>(v, pid, pid2);
+            if constexpr (2 * tx_fuse_lds<T, PH, PF, G0, true>() * (int)sizeof(T) <=
+                              160 * 1024 &&
+                          G0 * PF::TL / 128 <= 8 && G0 * PF::TL / 128 > 1)
+              add_t_fuse<T, PH, PF, G0, true, 2 * G0 * PF::TL / 256>(v, pid, pid2);
           });
         });
       }

@@ -7,6 +7,9 @@
 --trace   a `rocprofv3 --kernel-trace --stats -f csv` output directory
 --fetch   a `rocprofv3 --pmc FETCH_SIZE -f csv` output directory
 --write   a `rocprofv3 --pmc WRITE_SIZE -f csv` output directory
+--gap-window MS [--window-steps K]   the idle share of the kernel activity
+          after the last idle gap longer than MS in the trace (the graph steps
+          of `bench.py --ensemble --ens-trace-steps K`)
 
 FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by what the same
 counter reads for the plain device-to-device copy of `bench.py
@@ -95,9 +98,109 @@ def counter_means(pmc_dir: str):
     return {k: sum(v) / len(v) for k, v in per.items()}, cal
 
 
+def busy_window(trace_dir: str, gap_ms: float):
+    """Kernel activity after the last idle gap longer than gap_ms in a
+    `rocprofv3 --kernel-trace` run: the window's span, the time covered by
+    at least one kernel (the union of the dispatch intervals; kernels on
+    parallel branches overlap) and the launches in it."""
+    rows = _csv(trace_dir, "kernel_trace.csv")
+    iv = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in rows)
+    start = 0
+    reach = iv[0][1]
+    for i in range(1, len(iv)):
+        if iv[i][0] - reach > gap_ms * 1e6:
+            start = i
+        reach = max(reach, iv[i][1])
+    win = iv[start:]
+    covered, cur_s, cur_e = 0, win[0][0], win[0][1]
+    for b, e in win[1:]:
+        if b > cur_e:
+            covered += cur_e - cur_s
+            cur_s, cur_e = b, e
+        else:
+            cur_e = max(cur_e, e)
+    covered += cur_e - cur_s
+    span = max(e for _, e in win) - win[0][0]
+    return {"kernels": len(win), "span_ms": span / 1e6, "busy_ms": covered / 1e6,
+            "idle_frac": 1.0 - covered / span}
+
+
+def kernel_vgprs(lib: str):
+    """VGPRs per kernel (short names) from the gfx950 code objects of a
+    library built here: the objects are unbundled with llvm-objdump
+    --offloading into a scratch directory and their notes read with
+    llvm-readelf."""
+    import shutil
+    import subprocess
+    import tempfile
+    llvm = "/opt/rocm/llvm/bin"
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        copy = os.path.join(d, "lib.so")
+        shutil.copy(lib, copy)
+        subprocess.run([f"{llvm}/llvm-objdump", "--offloading", copy], check=True,
+                       capture_output=True)
+        for obj in glob.glob(os.path.join(d, "lib.so.*gfx*")):
+            notes = subprocess.run([f"{llvm}/llvm-readelf", "--notes", obj], check=True,
+                                   capture_output=True, text=True).stdout
+            for blk in notes.split(".agpr_count:")[1:]:
+                nm = re.search(r"\.name:\s+(\S+)", blk)
+                vg = re.search(r"\.vgpr_count:\s+(\d+)", blk)
+                if nm and vg:
+                    out[nm.group(1)] = int(vg.group(1))
+    # short names of the demangled symbols (one c++filt run for all)
+    names = sorted(out)
+    dem = subprocess.run(["c++filt"], input="\n".join(names), capture_output=True,
+                         text=True, check=True).stdout.splitlines()
+    short = {}
+    for n, d in zip(names, dem):
+        d = d.strip()
+        short[short_name(d if d.startswith("void ") else "void " + d)] = out[n]
+    return short
+
+
+def sq_table(dir_a: str, dir_b: str, lib: str | None):
+    """Per-kernel SQ counter table from two `rocprofv3 --pmc` runs of the same
+    command (their counters joined per kernel; means over dispatches)."""
+    vg = kernel_vgprs(lib) if lib else {}
+    per = defaultdict(lambda: defaultdict(list))
+    meta = {}
+    for d in (dir_a, dir_b):
+        for r in _csv(d, "counter_collection.csv"):
+            k = short_name(r["Kernel_Name"])
+            if not k.startswith("k_t"):
+                continue
+            per[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+            mangled = re.sub(r"\(.*", "", r["Kernel_Name"])
+            meta[k] = (int(r["Workgroup_Size"]), int(r["LDS_Block_Size"]), mangled)
+    rows = []
+    for k, c in per.items():
+        x = {n: sum(v) / len(v) for n, v in c.items()}
+        need = ("SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT",
+                "SQ_ACTIVE_INST_LDS")
+        if any(n not in x for n in need):
+            continue
+        wg, lds, _ = meta[k]
+        rows.append((k, wg, lds, vg.get(k), x))
+    return rows
+
+
+def waves_per_simd(vgpr, lds, wg):
+    """Waves a SIMD holds for this kernel: 512 VGPRs per lane in granules of
+    8, 160 KiB of LDS per CU, at most 8 waves, whole blocks."""
+    if not vgpr:
+        return None
+    by_vgpr = min(8, 512 // (8 * ((vgpr + 7) // 8)))
+    wpb = max(1, wg // 64)
+    blocks = (4 * by_vgpr) // wpb
+    if lds:
+        blocks = min(blocks, (160 * 1024) // lds)
+    return blocks * wpb / 4
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--trace", required=True)
+    ap.add_argument("--trace")
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     ap.add_argument("--cmd", required=True)
@@ -106,7 +209,21 @@ def main(argv=None):
     ap.add_argument("--title", default="rocprofv3 kernel summary")
     ap.add_argument("--min-pct", type=float, default=0.5,
                     help="list kernels with at least this share of GPU time")
+    ap.add_argument("--gap-window", type=float, metavar="MS",
+                    help="also report the idle share of the kernel activity after the last "
+                         "idle gap longer than MS (bench.py --ensemble --ens-trace-steps)")
+    ap.add_argument("--window-steps", type=int, default=0,
+                    help="steps in that window (for the per-step figures)")
+    ap.add_argument("--sq", nargs=2, metavar="DIR",
+                    help="two rocprofv3 --pmc output directories with the SQ counters "
+                         "(SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY "
+                         "SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_LDS SQ_ACTIVE_INST_VMEM; "
+                         "SQ_INSTS_VALU SQ_INSTS_LDS SQ_LDS_BANK_CONFLICT SQ_WAIT_INST_LDS "
+                         "SQ_INSTS_SALU SQ_INSTS_VMEM_RD SQ_INST_LEVEL_VMEM SQ_LEVEL_WAVES)")
+    ap.add_argument("--lib", help="the built library, for the kernels' VGPR counts")
     a = ap.parse_args(argv)
+    if a.sq:
+        return sq_main(a)
 
     stats = kernel_stats(a.trace)
     lines = [f"# {a.title}", "", "Command (run from the repository root on one MI355X):", "",
@@ -120,6 +237,16 @@ def main(argv=None):
         shown.add(s["name"])
         lines.append(f"| `{s['name']}` | {s['calls']} | {s['avg_ms']:.4f} | {s['min_ms']:.4f} "
                      f"| {s['max_ms']:.4f} | {s['pct']:.2f} |")
+    if a.gap_window:
+        w = busy_window(a.trace, a.gap_window)
+        lines += ["", "## Kernel activity of the traced steps", "",
+                  f"After the last idle gap longer than {a.gap_window:g} ms: {w['kernels']} "
+                  f"kernel launches over {w['span_ms']:.3f} ms, of which {w['busy_ms']:.3f} ms "
+                  f"have at least one kernel running; idle share {100 * w['idle_frac']:.2f} %."]
+        if a.window_steps:
+            lines.append(f"Per step ({a.window_steps} steps): {w['kernels'] / a.window_steps:.0f} "
+                         f"launches, {w['span_ms'] / a.window_steps:.3f} ms span, "
+                         f"{(w['span_ms'] - w['busy_ms']) / a.window_steps:.4f} ms idle.")
     traffic = {}
     if a.fetch and a.write:
         fm, fcopy = counter_means(a.fetch)
@@ -149,6 +276,38 @@ def main(argv=None):
                 json.dump({"command": a.cmd, "scale": {"fetch": sf, "write": sw},
                            "units": "GB read, GB written per dispatch",
                            "kernels": table}, fh, indent=0)
+    with open(a.out, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def sq_main(a):
+    rows = sq_table(a.sq[0], a.sq[1], a.lib)
+    lines = [f"# {a.title}", "", "Commands (run from the repository root on one MI355X):", "",
+             "```", a.cmd, "```", "",
+             "Per kernel, means over its dispatches.  Percentages are of SQ_WAVE_CYCLES "
+             "(wave-cycles) for SQ_ACTIVE_INST_VALU, SQ_ACTIVE_INST_LDS, SQ_WAIT_INST_ANY, "
+             "SQ_WAIT_ANY and SQ_WAIT_INST_LDS.  conflict/LDS: SQ_LDS_BANK_CONFLICT over "
+             "SQ_ACTIVE_INST_LDS (cycles lost to bank conflicts per LDS-active cycle).  "
+             "VGPRs from the built code object; waves/SIMD from VGPRs, LDS and the block "
+             "size (512 VGPRs per lane in granules of 8, 160 KiB LDS per CU).", "",
+             "| kernel | block | LDS B | VGPRs | waves/SIMD | VALU/wave | LDS inst/wave | "
+             "VALU % | LDS % | WAIT_INST_ANY % | WAIT_ANY % | WAIT_INST_LDS % | conflict/LDS |",
+             "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    for k, wg, lds, vgpr, x in sorted(rows, key=lambda r: r[0]):
+        wc = x["SQ_WAVE_CYCLES"]
+        w = x["SQ_WAVES"]
+        occ = waves_per_simd(vgpr, lds, wg)
+
+        def pct(n):
+            return f"{100 * x[n] / wc:.1f}" if n in x and wc else "-"
+        lines.append(
+            f"| `{k}` | {wg} | {lds} | {vgpr if vgpr else '-'} | "
+            f"{occ if occ is not None else '-'} | "
+            f"{x.get('SQ_INSTS_VALU', 0) / w:.0f} | {x['SQ_INSTS_LDS'] / w:.0f} | "
+            f"{pct('SQ_ACTIVE_INST_VALU')} | {pct('SQ_ACTIVE_INST_LDS')} | "
+            f"{pct('SQ_WAIT_INST_ANY')} | {pct('SQ_WAIT_ANY')} | {pct('SQ_WAIT_INST_LDS')} | "
+            f"{x['SQ_LDS_BANK_CONFLICT'] / max(1.0, x['SQ_ACTIVE_INST_LDS']):.2f} |")
     with open(a.out, "w") as fh:
         fh.write("\n".join(lines) + "\n")
     print("\n".join(lines))
