@@ -345,10 +345,38 @@ struct Pass {
       if constexpr (NS > 1) {
         // XP plans read a table of double entries through the same pointer
         const cplx<C> *w = reinterpret_cast<const cplx<C> *>(tw) + TWOFF + k * (R - 1);
+        if constexpr (std::is_same_v<T, double> && R == 16) {
+          // FP64 fields, radix 16: only w^(2^i) come from the table (4 loads
+          // instead of 15); w^q and w^(q + 8), q < 8, are products of those,
+          // formed as they are used, so that few twiddles are live at once
+          // (each within a few roundings of the table's entry, far below the
+          // step's tolerance)
+          constexpr int H = R / 2;
+          cplx<C> p[4]; // p[i] = w^(2^i)
 #pragma unroll
-        for (int q = 1; q < R; ++q) {
-          cplx<C> wq = w[q - 1];
-          a[q] = (DIR < 0) ? c_mul(a[q], wq) : c_mulc(a[q], wq);
+          for (int i = 0; (1 << i) < R; ++i) p[i] = w[(1 << i) - 1];
+          const cplx<C> wh = p[H == 2 ? 1 : (H == 4 ? 2 : 3)];
+#pragma unroll
+          for (int q = 1; q < H; ++q) {
+            cplx<C> wq = p[0];
+            bool first = true;
+#pragma unroll
+            for (int i = 0; (1 << i) < H; ++i)
+              if (q & (1 << i)) {
+                wq = first ? p[i] : c_mul(wq, p[i]);
+                first = false;
+              }
+            const cplx<C> wq2 = c_mul(wq, wh);
+            a[q] = (DIR < 0) ? c_mul(a[q], wq) : c_mulc(a[q], wq);
+            a[q + H] = (DIR < 0) ? c_mul(a[q + H], wq2) : c_mulc(a[q + H], wq2);
+          }
+          a[H] = (DIR < 0) ? c_mul(a[H], wh) : c_mulc(a[H], wh);
+        } else {
+#pragma unroll
+          for (int q = 1; q < R; ++q) {
+            cplx<C> wq = w[q - 1];
+            a[q] = (DIR < 0) ? c_mul(a[q], wq) : c_mulc(a[q], wq);
+          }
         }
       }
       Dft<R, DIR, C>::run(a);
