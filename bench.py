@@ -87,12 +87,13 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
     `mode` is the z-pass kind (0 PFC, 1 Tungsten mean field, 2 Tungsten combine)."""
     import re
     t = "double" if prec == "f64" else "float"
-    if kernel == "k_tx_fuse":  # "rA/EA+rB/EB/G8/split": nx/2- and nx-point plans
+    if kernel == "k_tx_fuse":  # "rA/EA+rB/EB/G8/split[/wN]": nx/2- and nx-point plans
         half, full = variant.split("+")
         ra, ea = half.split("/")
         pb = full.split("/")
+        wpe = pb[4][1:] if len(pb) > 4 and pb[4][:1] == "w" else "1"  # waves per SIMD cap
         return (f"k_tx_fuse<{t},N{n // 2}/E{ea[1:]},r{ra[1:]}+N{n}/E{pb[1][1:]},r{pb[0][1:]},"
-                f"{pb[2][1:]},{'true' if pb[3] == 'split' else 'false'}>")
+                f"{pb[2][1:]},{'true' if pb[3] == 'split' else 'false'},{wpe}>")
     parts = variant.split("/")
     nt = "nt" in parts  # non-temporal lines: the kernel's last template argument
     xp = "xp" in parts  # a plan with double butterflies

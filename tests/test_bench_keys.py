@@ -53,7 +53,40 @@ def demangled(kernel, t, plan, rest):
      demangled("k_tx_inv", "float", HALF8XP, "16, true, true")),
     ("k_tz", "r8x8x4x4/E8/xp/G1/split/nt", 0,
      demangled("k_tz", "float", PLAN8XP, "1, true, 0, false, true")),
+    ("k_tx_fuse", "r8x8x8/E8+r16x16x4/E16/G8/split/w4", 0,
+     demangled("k_tx_fuse", "double", HALF8 + ", " + PLAN16, "8, true, 4")),
+    ("k_tx_fuse", "r4x4x4x4x2/E4+r8x8x4x4/E8/G4/full", 0,
+     demangled("k_tx_fuse", "double", HALF4 + ", " + PLAN8, "4, false, 1")),
 ])
 def test_variant_key_matches_profile_name(kernel, variant, mode, name):
     prec = "f32" if "<float" in name else "f64"
     assert bench.variant_key(kernel, prec, 1024, variant, mode=mode) == short_name(name)
+
+
+def _trace_dir(tmp_path, intervals):
+    """A kernel_trace.csv with the given (start, end) nanosecond intervals."""
+    import csv
+    d = tmp_path / "trace"
+    d.mkdir()
+    with open(d / "x_kernel_trace.csv", "w", newline="") as fh:
+        w = csv.writer(fh)
+        w.writerow(["Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        for b, e in intervals:
+            w.writerow(["k", b, e])
+    return str(d)
+
+
+def test_busy_window_after_last_gap(tmp_path):
+    """Idle share after the last long gap: overlapping kernels count once,
+    short gaps count as idle, the activity before the long gap is left out."""
+    from openpfc_amd.profile_summary import busy_window
+    ms = 1_000_000
+    iv = [(0, 5 * ms),                     # before the gap (ignored)
+          (300 * ms, 302 * ms), (301 * ms, 304 * ms),  # overlap: 4 ms busy
+          (305 * ms, 306 * ms)]            # 1 ms idle before, 1 ms busy
+    w = busy_window(_trace_dir(tmp_path, iv), 100.0)
+    assert w["kernels"] == 3
+    assert abs(w["span_ms"] - 6.0) < 1e-9
+    assert abs(w["busy_ms"] - 5.0) < 1e-9
+    assert abs(w["idle_frac"] - 1 / 6) < 1e-12
+
