@@ -431,7 +431,9 @@ struct Pass {
   // a multiple of 32 where SWZ holds), so that the reads of all e share one
   // address register and take the rest as an immediate offset.
   __device__ __forceinline__ static int read_slot(int t, int e) {
-    if constexpr (!SWZ || SWZ_KIND == 2) {
+    if constexpr (ROWPAD && TL % 16 == 0 && (!SWZ || SWZ_KIND == 2)) {
+      return t + (t >> 4) + e * (TL + TL / 16); // pad_pos(t + TL*e), split
+    } else if constexpr (!SWZ || SWZ_KIND == 2) {
       return pad_pos<ROWPAD>(t + TL * e);
     } else if constexpr (SWZ_KIND == 0) {
       return t + (t >> 5) + e * (TL + TL / 32);
@@ -446,7 +448,13 @@ struct Pass {
 
   // slot(dst_pos(t, b, q)) as a per-(lane, b) part plus q times a constant.
   __device__ __forceinline__ static int write_slot(int t, int b, int q) {
-    if constexpr (!SWZ || SWZ_KIND == 2) {
+    if constexpr (ROWPAD && (!SWZ || SWZ_KIND == 2) &&
+                  ((NS == 1 && R <= 16 && (R & (R - 1)) == 0) || (NS > 1 && NS % 16 == 0))) {
+      // pad_pos(d0 + q*NS), split: d0 = R*j with R dividing 16 (NS == 1; so
+      // d0 mod 16 + q < 16), or q*NS a multiple of 16
+      const int d0 = dst_pos(t, b, 0);
+      return d0 + (d0 >> 4) + q * (NS == 1 ? 1 : NS + NS / 16);
+    } else if constexpr (!SWZ || SWZ_KIND == 2) {
       return pad_pos<ROWPAD>(dst_pos(t, b, q));
     } else {
       const int d0 = dst_pos(t, b, 0);

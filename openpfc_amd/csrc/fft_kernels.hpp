@@ -96,24 +96,47 @@ __global__ void __launch_bounds__(L *Plan::TL)
 }
 
 // ----------------------------------------------------------------------------
-// Row helpers: fetch the value held at line position `pp(e)` for every owned
-// element, through LDS (row layout with padding).
+// Row helpers.  Padded LDS slot of line position t + TL*e (0 <= t < TL) and of
+// its mirror (L - k for k = t + TL*e != 0, 0 for k = 0; L = E*TL the line
+// length), each as a per-lane part plus a per-e constant when TL is a
+// multiple of 16, so that the accesses of all e share one address register:
+// pad_pos(u - t) = u + u/16 - t - ceil(t/16) for u = L - TL*e, a multiple of
+// 16 and larger than t.
 // ----------------------------------------------------------------------------
-template <typename T, int E, int TL, bool WAVE = false, typename PartnerPos>
+template <int TL> __device__ __forceinline__ int row_slot(int t, int e) {
+  if constexpr (TL % 16 == 0)
+    return t + (t >> 4) + e * (TL + TL / 16);
+  else
+    return pad_pos<true>(t + TL * e);
+}
+template <int E, int TL> __device__ __forceinline__ int row_mirror_slot(int t, int e) {
+  constexpr int L = E * TL;
+  if constexpr (TL % 16 == 0) {
+    const int u = L - TL * e;
+    const int i = u + u / 16 - t - ((t + 15) >> 4);
+    return e == 0 ? (t == 0 ? 0 : i) : i;
+  } else {
+    const int k = t + TL * e;
+    return pad_pos<true>(k == 0 ? 0 : L - k);
+  }
+}
+
+// Fetch, for every owned element k = t + TL*e, the value at the mirror
+// position (L - k, or 0 for k = 0) of the same line, through LDS.
+template <typename T, int E, int TL, bool WAVE = false>
 __device__ __forceinline__ void row_fetch_partner(const cplx<T> (&v)[E],
                                                   cplx<T> (&p)[E], int t,
-                                                  T *lds_re, T *lds_im,
-                                                  PartnerPos pp) {
+                                                  T *lds_re, T *lds_im) {
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int i = pad_pos<true>(t + TL * e);
+    const int i = row_slot<TL>(t, e);
     lds_re[i] = v[e].x;
     lds_im[i] = v[e].y;
   }
   lds_sync<WAVE>();
 #pragma unroll
   for (int e = 0; e < E; ++e) {
-    const int i = pad_pos<true>(pp(t + TL * e));
+    const int i = row_mirror_slot<E, TL>(t, e);
     p[e] = cplx<T>{lds_re[i], lds_im[i]};
   }
   lds_sync<WAVE>();
@@ -188,8 +211,7 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_x_r2c(XArgs<T> a) {
   LineFft<T, Plan, -1, false, true, row_wave<Plan>()>::run(v, t, ll, a.tw);
 
   cplx<T> pz[E];
-  row_fetch_partner<T, E, TL, row_wave<Plan>()>(v, pz, t, re, im,
-                              [](int k) { return k == 0 ? 0 : M - k; });
+  row_fetch_partner<T, E, TL, row_wave<Plan>()>(v, pz, t, re, im);
   if (!active) return;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
@@ -298,8 +320,7 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_pfc_x(PfcXArgs<T> a) {
   LineFft<T, Plan, -1, false, true, row_wave<Plan>()>::run(v, t, ll, a.tw);
 
   cplx<T> pz[E];
-  row_fetch_partner<T, E, TL, row_wave<Plan>()>(v, pz, t, re, im,
-                              [](int k) { return k == 0 ? 0 : N - k; });
+  row_fetch_partner<T, E, TL, row_wave<Plan>()>(v, pz, t, re, im);
   if (!active) return;
   cplx<T> *__restrict__ oa = a.outA + r * a.pitch;
   cplx<T> *__restrict__ ob = a.outB + r * a.pitch;
@@ -671,8 +692,7 @@ __global__ void __launch_bounds__(LPB *Plan::TL) k_x_tung(TungXArgs<T> a) {
   }
   LineFft<T, Plan, -1, false, true, row_wave<Plan>()>::run(v, t, ll, a.tw);
   cplx<T> pz[E];
-  row_fetch_partner<T, E, TL, row_wave<Plan>()>(v, pz, t, lds + c * RP, lds + LPB * RP + c * RP,
-                              [](int k) { return k == 0 ? 0 : M - k; });
+  row_fetch_partner<T, E, TL, row_wave<Plan>()>(v, pz, t, lds + c * RP, lds + LPB * RP + c * RP);
   if (!active) return;
 #pragma unroll
   for (int e = 0; e < E; ++e) {

@@ -62,28 +62,27 @@ template <int G, typename T> __device__ __forceinline__ int tile_idx(int pos, in
     return pos * (G + 1) + g;
 }
 
-// Fetch, for every owned element, the value at line position pp(k) of the
-// same line through LDS; one plane at a time when SPLIT (half the LDS).
-template <typename T, int E, int TL, bool SPLIT, bool WAVE = false,
-          typename PartnerPos>
+// Fetch, for every owned element k, the value at the mirror position (L - k,
+// 0 for k = 0) of the same line through LDS; one plane at a time when SPLIT
+// (half the LDS).
+template <typename T, int E, int TL, bool SPLIT, bool WAVE = false>
 __device__ __forceinline__ void fetch_partner(const cplx<T> (&v)[E], cplx<T> (&p)[E],
-                                              int t, T *re, T *im,
-                                              PartnerPos pp) {
+                                              int t, T *re, T *im) {
   if constexpr (SPLIT) {
 #pragma unroll
-    for (int e = 0; e < E; ++e) re[pad_pos<true>(t + TL * e)] = v[e].x;
+    for (int e = 0; e < E; ++e) re[row_slot<TL>(t, e)] = v[e].x;
     lds_sync<WAVE>();
 #pragma unroll
-    for (int e = 0; e < E; ++e) p[e].x = re[pad_pos<true>(pp(t + TL * e))];
+    for (int e = 0; e < E; ++e) p[e].x = re[row_mirror_slot<E, TL>(t, e)];
     lds_sync<WAVE>();
 #pragma unroll
-    for (int e = 0; e < E; ++e) re[pad_pos<true>(t + TL * e)] = v[e].y;
+    for (int e = 0; e < E; ++e) re[row_slot<TL>(t, e)] = v[e].y;
     lds_sync<WAVE>();
 #pragma unroll
-    for (int e = 0; e < E; ++e) p[e].y = re[pad_pos<true>(pp(t + TL * e))];
+    for (int e = 0; e < E; ++e) p[e].y = re[row_mirror_slot<E, TL>(t, e)];
     lds_sync<WAVE>();
   } else {
-    row_fetch_partner<T, E, TL, WAVE>(v, p, t, re, im, pp);
+    row_fetch_partner<T, E, TL, WAVE>(v, p, t, re, im);
   }
 }
 
@@ -148,8 +147,7 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tx_fwd(TStepArgs<T> a) {
   const LdsLine<T> ll{lds, g * RP, 1, PL};
   LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_x);
   cplx<T> pz[E];
-  fetch_partner<T, E, TL, SPLIT, row_wave<Plan>()>(v, pz, t, re, im,
-                                 [](int k) { return k == 0 ? 0 : N - k; });
+  fetch_partner<T, E, TL, SPLIT, row_wave<Plan>()>(v, pz, t, re, im);
   // split Z into Psi (v) and N (pz) for k = t + TL*e, e < E/2
 #pragma unroll
   for (int e = 0; e <= E / 2 && e < E; ++e) {
@@ -603,8 +601,7 @@ __device__ __forceinline__ void row_r2c(cplx<T> (&v)[Plan::E], cplx<T> &xm, T *l
   const LdsLine<T> ll{lds, g * RP, 1, PL};
   LineFft<T, Plan, -1, false, true, row_wave<Plan>()>::run(v, t, ll, tw);
   cplx<T> pz[E];
-  row_fetch_partner<T, E, TL, row_wave<Plan>()>(v, pz, t, lds + g * RP, lds + PL + g * RP,
-                              [](int k) { return k == 0 ? 0 : M - k; });
+  row_fetch_partner<T, E, TL, row_wave<Plan>()>(v, pz, t, lds + g * RP, lds + PL + g * RP);
   xm = cplx<T>{v[0].x - v[0].y, (T)0};
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = r2c_split(v[e], pz[e], wn[t + TL * e]);
@@ -763,8 +760,7 @@ __device__ __forceinline__ void fwd_split_store_wide(cplx<T> (&v)[Plan::E], T *l
   const LdsLine<T> ll{lds, g * RP, 1, PL};
   LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_x);
   cplx<T> pz[E];
-  fetch_partner<T, E, TL, SPLIT, row_wave<Plan>()>(v, pz, t, re, im,
-                                 [](int k) { return k == 0 ? 0 : N - k; });
+  fetch_partner<T, E, TL, SPLIT, row_wave<Plan>()>(v, pz, t, re, im);
 #pragma unroll
   for (int e = 0; e <= E / 2 && e < E; ++e) {
     const cplx<T> zk = v[e], zc = c_conj(pz[e]);
@@ -984,15 +980,17 @@ __global__ void __launch_bounds__(G *PF::TL, WPE) k_tx_fuse(TStepArgs<T> a) {
   T *row = lds + g * RPF;
 #pragma unroll
   for (int e = 0; e < EH; ++e) {
-    const int m = t + TL * e;
-    row[pad_pos<true>(2 * m)] = v[e].x;
-    row[pad_pos<true>(2 * m + 1)] = v[e].y;
+    // positions 2m and 2m + 1 of m = t + TL*e: pad_pos(2m) = row_slot<2TL>(2t, e)
+    // (2m mod 16 <= 14, so 2m + 1 lands in the next slot)
+    const int i = row_slot<2 * TL>(2 * t, e);
+    row[i] = v[e].x;
+    row[i + 1] = v[e].y;
   }
   __syncthreads();
   cplx<T> w[EF];
 #pragma unroll
   for (int e = 0; e < EF; ++e) {
-    const T u = row[pad_pos<true>(t + TL * e)];
+    const T u = row[row_slot<TL>(t, e)];
     w[e] = cplx<T>{u, a.nl(u)};
   }
   __syncthreads(); // the LDS becomes the forward transform's exchange area
