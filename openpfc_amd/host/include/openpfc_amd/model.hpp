@@ -41,6 +41,12 @@ public:
 
   virtual void initialize(double dt) = 0;
   virtual void step(double t) = 0;
+  // n steps ending at times[0..n-1] with nothing in between (no modifier,
+  // no writer): a device model may keep the field on the device and fuse
+  // the step boundaries.  Default: n calls of step().
+  virtual void step_many(const double *times, int n) {
+    for (int i = 0; i < n; ++i) step(times[i]);
+  }
   // Device-resident models copy device -> host here so modifiers and
   // writers see current data, and host -> device in finalize.
   virtual void prepare_for_field_modifiers() {}

@@ -51,6 +51,9 @@ public:
     m_host_stale = true;
   }
 
+  // the step does not depend on t: one call, fused step boundaries
+  void step_many(const double *, int n) override { step_n(n); }
+
   // advance n steps without returning to the host in between
   void step_n(int n) {
     pfc::detail::check_abi(pfc_hip_pfc_step(m_hfft.plan(), m_psi.data(), n, m_hfft.stream()),

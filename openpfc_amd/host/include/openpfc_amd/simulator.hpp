@@ -56,6 +56,30 @@ public:
     if (m_time.do_save()) write();
   }
 
+  // Up to max_steps steps with the effect of that many step() calls (no
+  // reference counterpart: the reference loops over step()).  Without
+  // boundary conditions nothing happens between two steps up to the next
+  // save point, so those steps go to Model::step_many in one call; returns
+  // the number of steps taken.
+  int step_block(int max_steps) {
+    if (max_steps <= 1 || m_time.get_increment() == 0 || !m_bcs.empty()) {
+      step();
+      return 1;
+    }
+    Time look = m_time;
+    std::vector<double> times;
+    while ((int)times.size() < max_steps && !look.done()) {
+      look.next();
+      times.push_back(look.get_current());
+      if (look.do_save()) break;
+    }
+    if (times.empty()) return 0;
+    m_model.step_many(times.data(), (int)times.size());
+    for (size_t i = 0; i < times.size(); ++i) m_time.next();
+    if (m_time.do_save()) write();
+    return (int)times.size();
+  }
+
   bool done() const { return m_time.done(); }
   Model &get_model() { return m_model; }
   Time &get_time() { return m_time; }

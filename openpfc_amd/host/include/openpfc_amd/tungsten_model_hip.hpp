@@ -127,6 +127,7 @@ public:
   gpu::TuneOutcome autotune_outcome() const noexcept { return m_tuned; }
 
   void step(double) override { step_n(1); }
+  void step_many(const double *, int n) override { step_n(n); }
 
   // advance n steps without returning to the host in between
   void step_n(int n) {

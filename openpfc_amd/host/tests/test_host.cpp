@@ -115,6 +115,38 @@ static int run_cpu() {
     CHECK(m.psi[0] == -0.2 && m.psi.back() == -0.2);
     CHECK(m.prepares >= 1 && m.finalizes == 1); // ICs once, no BCs
   }
+  // step_block: the same model steps, times and saves as step() calls, in
+  // blocks that end at save points; with a boundary condition, one step at
+  // a time
+  {
+    World w = domain::create(GridSize({8, 4, 2}), PhysicalOrigin({0, 0, 0}), GridSpacing({1, 1, 1}));
+    BoxOnlyFft f(w.size);
+    CountingModel a(f, w), b(f, w);
+    a.initialize(0.5);
+    b.initialize(0.5);
+    Time ta({0.0, 100.0, 0.5}, 5.0), tb({0.0, 100.0, 0.5}, 5.0);
+    Simulator sa(a, ta), sb(b, tb);
+    std::vector<double> save_a, save_b;
+    sa.add_results_writer([&](Model &, const Time &t) { save_a.push_back(t.get_current()); });
+    sb.add_results_writer([&](Model &, const Time &t) { save_b.push_back(t.get_current()); });
+    while (!done(sa)) pfc::step(sa);
+    int calls = 0;
+    while (!done(sb)) {
+      const int k = sb.step_block(1 << 30);
+      CHECK(k >= 1 && k <= 10);
+      ++calls;
+    }
+    CHECK(b.steps == 200 && b.times == a.times && save_b == save_a);
+    CHECK(calls == 21); // the first step, then 20 blocks of 10 (saveat 5, dt 0.5)
+    CountingModel c(f, w);
+    c.initialize(0.5);
+    Time tc({0.0, 10.0, 0.5}, 5.0);
+    Simulator sc(c, tc);
+    sc.add_boundary_conditions(std::make_unique<Constant>(-0.2));
+    int one = 0;
+    while (!done(sc)) one += sc.step_block(1 << 30) == 1;
+    CHECK(c.steps == 20 && one == 20);
+  }
   // Time validation (time.hpp:366-395)
   CHECK(throws<std::invalid_argument>([] { Time t({-1.0, 1.0, 0.1}, 0.0); }));
   CHECK(throws<std::invalid_argument>([] { Time t({1.0, 1.0, 0.1}, 0.0); }));
