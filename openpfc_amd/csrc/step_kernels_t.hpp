@@ -406,6 +406,104 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
   }
 }
 
+// ----------------------------------------------------------------------------
+// T2 z pass as k_tz (PRE = true ordering, non-temporal lines), with the
+// multipliers of each line tabulated in LDS.  kz enters every mode's
+// multipliers only as kz*kz, and pfc_kc(k) = -pfc_kc(nz - k) exactly, so a
+// line has nz/2 + 1 distinct pairs: the block evaluates those once (k_tz
+// evaluates all nz), while its loads are in flight, and each transform's
+// output reads its pair back from slot k or nz - k.  No multiplier stays in
+// registers across a transform.  The values and the arithmetic that applies
+// them are k_tz's, so both give the same bits.
+// ----------------------------------------------------------------------------
+template <typename T, typename Plan, int G> constexpr int tz_table_bytes() {
+  return G * 2 * (Plan::N / 2 + 1) * (int)sizeof(std::conditional_t<Plan::XP, double, T>);
+}
+
+template <typename T, typename Plan, int G, bool SPLIT, int MODE>
+__global__ void __launch_bounds__(G *Plan::TL) k_tz_mt(TStepArgs<T> a) {
+  constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
+  constexpr int PL = t_plane<Plan, G>(), NH = N / 2 + 1;
+  using M = std::conditional_t<Plan::XP, double, T>;
+  __shared__ T lds[(SPLIT ? 1 : 2) * PL];
+  __shared__ M tab[G * 2 * NH];
+  const int t = threadIdx.x % TL, g = threadIdx.x / TL;
+  const long nlines = (long)a.W * a.ny;
+  const long l0 = (long)blockIdx.x * G;
+  const long li = l0 + g;
+  const bool active = li < nlines;
+  const long l = active ? li : l0;
+  const int kx = (int)(l / a.ny), ky = (int)(l % a.ny);
+  const LdsLine<T> ll{lds, g * RP, 1, PL};
+  cplx<T> *__restrict__ la = a.A2 + l * N;
+  cplx<T> *__restrict__ lb = a.B2 + l * N;
+  M *__restrict__ ta = tab + g * 2 * NH;
+  M *__restrict__ tb = ta + NH;
+  auto slot = [&](int e) {
+    const int k = t + TL * e;
+    return k <= N / 2 ? k : N - k;
+  };
+
+  cplx<T> acc[E], v[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    acc[e] = ld_line<true>(la + (t + TL * e));
+    if constexpr (MODE != ZM_TMF) v[e] = ld_line<true>(lb + (t + TL * e));
+  }
+  {
+    const double kxy = pfc_kxy(a.op, kx, ky);
+    for (int k = t; k < NH; k += TL) {
+      double ma, mb;
+      z_mode_ops<MODE>(a.op, a.top, kxy, pfc_kc(k, a.op.nz, a.op.fz), ma, mb);
+      ta[k] = (M)ma;
+      if constexpr (MODE != ZM_TMF) tb[k] = (M)mb;
+    }
+  }
+  __syncthreads(); // the table is complete
+  if constexpr (MODE == ZM_TMF) {
+    // A = Fz(A) (kept for the combine); B = Fz^-1(fMF * A)
+    LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(acc, t, ll, a.tw_z);
+    if (active) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) st_line<true>(la + (t + TL * e), acc[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc[e] = c_scale(acc[e], (T)ta[slot(e)]);
+  } else if constexpr (MODE == ZM_PFC) {
+    // A = Fz^-1(opL*Fz(A) + opN*Fz(B))
+    LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(acc, t, ll, a.tw_z);
+    if constexpr (!Plan::XP) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) acc[e] = c_scale(acc[e], ta[slot(e)]);
+    }
+    LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_z);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const M opn = tb[slot(e)];
+      if constexpr (Plan::XP) {
+        const M opl = ta[slot(e)];
+        acc[e] = cplx<T>{(T)((double)acc[e].x * opl + (double)v[e].x * opn),
+                         (T)((double)acc[e].y * opl + (double)v[e].y * opn)};
+      } else {
+        acc[e] = c_add(acc[e], c_scale(v[e], opn));
+      }
+    }
+    if (a.zero_k0 && l == 0 && t == 0) acc[0] = cplx<T>{0, 0};
+  } else {
+    // A = Fz^-1(expL*A + nW*Fz(B)), A already a full spectrum
+    LineFft<T, Plan, -1, SPLIT, true, row_wave<Plan>()>::run(v, t, ll, a.tw_z);
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      acc[e] = c_add(c_scale(acc[e], (T)ta[slot(e)]), c_scale(v[e], (T)tb[slot(e)]));
+  }
+  LineFft<T, Plan, +1, SPLIT, true, row_wave<Plan>()>::run(acc, t, ll, a.tw_z);
+  if (active) {
+    cplx<T> *__restrict__ out = MODE == ZM_TMF ? lb : la;
+#pragma unroll
+    for (int e = 0; e < E; ++e) st_line<true>(out + (t + TL * e), acc[e]);
+  }
+}
+
 
 // ----------------------------------------------------------------------------
 // T2 -> T1 inverse y: block = (kx, z0..z0+G-1); transposed load.
