@@ -734,9 +734,11 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
         add_t_z_nt<T, P, 1, false, false>(v, pid);
         add_t_z_nt<T, P, 1, true, true>(v, pid);
         add_t_z_nt<T, P, 1, false, true>(v, pid);
-        add_t_z_mt<T, P, 1, true>(v, pid);
-        add_t_z_mt<T, P, 1, false>(v, pid);
-        add_t_z_mt<T, P, 2, true>(v, pid);
+        // tabulated multipliers: one wavefront of lines per block at least
+        constexpr int GM = P::TL >= 64 ? 1 : 64 / P::TL;
+        add_t_z_mt<T, P, GM, true>(v, pid);
+        add_t_z_mt<T, P, GM, false>(v, pid);
+        add_t_z_mt<T, P, 2 * GM, true>(v, pid);
       });
     }
     if constexpr (KIND == TK_X_FUSE) {

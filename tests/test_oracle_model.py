@@ -128,3 +128,22 @@ def test_trajectory_64_200_matches_numpy_fixture():
         st = d[f"stats_{s}"]
         assert abs(flat.sum() - st[0]) <= 1e-9 * abs(st[0])
         assert abs(flat.min() - st[2]) <= 1e-11 and abs(flat.max() - st[3]) <= 1e-11
+
+
+@pytest.mark.parametrize("n", [(12, 6, 8), (6, 10, 7), (8, 4, 2)])
+def test_pfc_operators_same_bits_on_mirrored_kz_planes(n):
+    """kz enters the operators only as kz*kz, and k_component(k) =
+    -k_component(nz - k) exactly for 0 < k < nz, k != nz/2, so the planes k and
+    nz - k of opL and opN hold the same bits (even and odd nz).  The z pass
+    that tabulates nz/2 + 1 multiplier pairs per line rests on this."""
+    spacing = (DX, 0.9, 1.3)
+    opL, opN = O.pfc_operators(n, spacing, EPS, DT)
+    nz = n[2]
+    f = 2 * math.pi / (spacing[2] * nz)
+    for k in range(1, nz):
+        if 2 * k != nz:
+            assert O.k_component(k, nz, f) == -O.k_component(nz - k, nz, f), k
+    mirror = (-np.arange(nz)) % nz
+    assert np.array_equal(opL, opL[mirror]) and np.array_equal(opN, opN[mirror])
+    # the planes are not all alike: the check can fail
+    assert not np.array_equal(opL[1], opL[0])

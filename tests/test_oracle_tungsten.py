@@ -110,3 +110,17 @@ def test_host_tungsten_params_match_golden_derivation():
         {k: v for k, v in PARAMS.items() if k != "lambda_"} | {"lambda": 0.22})
     with pytest.raises(KeyError):
         TungstenParams.from_dict({"no_such": 1.0})
+
+
+@pytest.mark.parametrize("n", [(16, 8, 12), (8, 6, 9)])
+def test_tungsten_operators_same_bits_on_mirrored_kz_planes(n):
+    """As for the PFC operators: the mean-field filter, exp(L dt) and the ETD
+    weight depend on kz only through kz*kz, so the planes k and nz - k agree
+    bit for bit (the premise of the tabulated z pass)."""
+    dx = 1.1107207345395915
+    op, _ = derived(PARAMS)
+    fMF, expL, nW = O.tungsten_operators(n, (dx, 1.2 * dx, 0.9 * dx), op, 1.0)
+    mirror = (-np.arange(n[2])) % n[2]
+    for a in (fMF, expL, nW):
+        assert np.array_equal(a, a[mirror])
+        assert not np.array_equal(a[1], a[0])
