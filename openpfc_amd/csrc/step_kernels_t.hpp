@@ -303,6 +303,32 @@ __device__ __forceinline__ void tile_load(T *sr, T *si, int nvalid, Src src) {
   }
 }
 
+// Block-wide store of a ROWS x G transpose tile (sr/si at tile_idx<G, T>(row,
+// line)) to HBM, NTH threads, element i = row*G + line: every LDS read of a
+// thread is issued before its first store, so the stores leave back to back
+// instead of one LDS round trip apart.  Lines >= nvalid are not stored.  It
+// holds ceil(ROWS*G / NTH) values per thread, which the wide forward y pass
+// has room for with 8-byte planes (122 -> 126 VGPRs, four waves per SIMD
+// either way) but not with 4-byte ones (61 -> 66: one of its 16-wave blocks
+// per CU instead of two), nor do the x passes (their stores follow the
+// transforms with more values live).
+template <int ROWS, int G, int NTH, bool NT, typename T, typename Dst>
+__device__ __forceinline__ void tile_store(const T *sr, const T *si, int nvalid, Dst dst) {
+  constexpr int TOT = ROWS * G, IT = (TOT + NTH - 1) / NTH;
+  cplx<T> w[IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int i = (int)threadIdx.x + it * NTH;
+    if (TOT % NTH == 0 || i < TOT)
+      w[it] = cplx<T>{sr[tile_idx<G, T>(i / G, i % G)], si[tile_idx<G, T>(i / G, i % G)]};
+  }
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    const int i = (int)threadIdx.x + it * NTH;
+    if ((TOT % NTH == 0 || i < TOT) && i % G < nvalid) st_line<NT>(dst(i / G, i % G), w[it]);
+  }
+}
+
 template <typename T, typename Plan, int G, bool SPLIT, int MODE, bool PRE = false,
           bool NT = false>
 __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
@@ -972,11 +998,17 @@ __global__ void __launch_bounds__(G *Plan::TL) k_ty_fwd_w(TStepArgs<T> a) {
       ti[tile_idx<G, T>(row, g)] = v[h * EPR + q].y;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < ROWS * G; i += G * TL) {
-      const int row = i / G, gg = i % G;
-      if (z0 + gg < a.nz)
-        st_line<NT>(out + (((long)kx * N + h * ROWS + row) * a.nz + z0 + gg),
-                    cplx<T>{tr[tile_idx<G, T>(row, gg)], ti[tile_idx<G, T>(row, gg)]});
+    if constexpr (sizeof(T) == 8) {
+      tile_store<ROWS, G, G * TL, NT>(tr, ti, a.nz - z0, [&](int row, int gg) {
+        return out + (((long)kx * N + h * ROWS + row) * a.nz + z0 + gg);
+      });
+    } else {
+      for (int i = threadIdx.x; i < ROWS * G; i += G * TL) {
+        const int row = i / G, gg = i % G;
+        if (z0 + gg < a.nz)
+          st_line<NT>(out + (((long)kx * N + h * ROWS + row) * a.nz + z0 + gg),
+                      cplx<T>{tr[tile_idx<G, T>(row, gg)], ti[tile_idx<G, T>(row, gg)]});
+      }
     }
   }
 }
