@@ -118,10 +118,13 @@ def variant_key(kernel, prec, n, variant, direction=None, mode=0):
             rest.append("true" if p == "split" else "false")
     if kernel == "k_col":
         rest = [str(direction)] + rest
-    if kernel == "k_tz" and "mt" in parts:  # multipliers tabulated per line
+    pair = "mtp" in parts  # lines ky and ny - ky sharing one table
+    if kernel == "k_tz" and ("mt" in parts or pair):  # multipliers tabulated per line
         kernel = "k_tz_mt"
     if kernel in ("k_tz", "k_tz_mt", "k_pfc_z"):
         rest.append(str(mode))
+        if kernel == "k_tz_mt":
+            rest.append("true" if pair else "false")
         if kernel == "k_tz":
             rest.append("true" if "pre" in parts else "false")
     elif kernel in ("k_ty_fwd_w", "k_ty_inv_w"):

@@ -441,29 +441,48 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz(TStepArgs<T> a) {
 // output reads its pair back from slot k or nz - k.  No multiplier stays in
 // registers across a transform.  The values and the arithmetic that applies
 // them are k_tz's, so both give the same bits.
+//
+// PAIR (G = 2): the block's lines are (kx, ky) and (kx, ny - ky), whose
+// kx*kx + ky*ky agree bit for bit by the same argument in y, so one table
+// serves both and the two lines' threads fill it together (half the
+// evaluations again).  Block 0 of each kx holds the lines that are their own
+// mirror, ky = 0 and (ny even) ky = ny/2, with a table each; for odd ny its
+// second line is idle.  Grid: W * ((ny + 1) / 2) blocks.
 // ----------------------------------------------------------------------------
 template <typename T, typename Plan, int G> constexpr int tz_table_bytes() {
   return G * 2 * (Plan::N / 2 + 1) * (int)sizeof(std::conditional_t<Plan::XP, double, T>);
 }
 
-template <typename T, typename Plan, int G, bool SPLIT, int MODE>
+template <typename T, typename Plan, int G, bool SPLIT, int MODE, bool PAIR = false>
 __global__ void __launch_bounds__(G *Plan::TL) k_tz_mt(TStepArgs<T> a) {
+  static_assert(!PAIR || G == 2, "a paired block holds two lines");
   constexpr int TL = Plan::TL, E = Plan::E, N = Plan::N, RP = row_padded(N);
   constexpr int PL = t_plane<Plan, G>(), NH = N / 2 + 1;
   using M = std::conditional_t<Plan::XP, double, T>;
   __shared__ T lds[(SPLIT ? 1 : 2) * PL];
   __shared__ M tab[G * 2 * NH];
   const int t = threadIdx.x % TL, g = threadIdx.x / TL;
-  const long nlines = (long)a.W * a.ny;
-  const long l0 = (long)blockIdx.x * G;
-  const long li = l0 + g;
-  const bool active = li < nlines;
-  const long l = active ? li : l0;
+  long l;
+  bool active, shared = false;
+  if constexpr (PAIR) {
+    const int nb = (a.ny + 1) / 2; // blocks per kx
+    const int kxb = (int)(blockIdx.x / nb), j = (int)(blockIdx.x % nb);
+    shared = j != 0;
+    const int kyp = j != 0 ? (g == 0 ? j : a.ny - j) : (g == 0 ? 0 : a.ny / 2);
+    active = j != 0 || g == 0 || a.ny % 2 == 0;
+    l = (long)kxb * a.ny + (active ? kyp : 0);
+  } else {
+    const long nlines = (long)a.W * a.ny;
+    const long l0 = (long)blockIdx.x * G;
+    const long li = l0 + g;
+    active = li < nlines;
+    l = active ? li : l0;
+  }
   const int kx = (int)(l / a.ny), ky = (int)(l % a.ny);
   const LdsLine<T> ll{lds, g * RP, 1, PL};
   cplx<T> *__restrict__ la = a.A2 + l * N;
   cplx<T> *__restrict__ lb = a.B2 + l * N;
-  M *__restrict__ ta = tab + g * 2 * NH;
+  M *__restrict__ ta = tab + (shared ? 0 : g) * 2 * NH;
   M *__restrict__ tb = ta + NH;
   auto slot = [&](int e) {
     const int k = t + TL * e;
@@ -478,7 +497,8 @@ __global__ void __launch_bounds__(G *Plan::TL) k_tz_mt(TStepArgs<T> a) {
   }
   {
     const double kxy = pfc_kxy(a.op, kx, ky);
-    for (int k = t; k < NH; k += TL) {
+    const int k0 = shared ? (int)threadIdx.x : t, dk = shared ? G * TL : TL;
+    for (int k = k0; k < NH; k += dk) {
       double ma, mb;
       z_mode_ops<MODE>(a.op, a.top, kxy, pfc_kc(k, a.op.nz, a.op.fz), ma, mb);
       ta[k] = (M)ma;

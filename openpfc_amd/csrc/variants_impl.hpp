@@ -447,19 +447,21 @@ hipError_t tz_nt_launch(TStepArgs<T> a, hipStream_t s) {
   return hipGetLastError();
 }
 // z pass with the multipliers tabulated per line in LDS (k_tz_mt)
-template <typename T, typename Plan, int G, bool S>
+// (PAIR: lines ky and ny - ky share one table, G = 2)
+template <typename T, typename Plan, int G, bool S, bool PAIR = false>
 hipError_t tz_mt_launch(TStepArgs<T> a, hipStream_t s) {
   const long lines = (long)a.W * a.ny;
-  dim3 grid((unsigned)((lines + G - 1) / G)), block(G * Plan::TL);
+  const long blocks = PAIR ? (long)a.W * ((a.ny + 1) / 2) : (lines + G - 1) / G;
+  dim3 grid((unsigned)blocks), block(G * Plan::TL);
   switch (a.mode) {
   case ZM_PFC:
-    hipLaunchKernelGGL((k_tz_mt<T, Plan, G, S, ZM_PFC>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_tz_mt<T, Plan, G, S, ZM_PFC, PAIR>), grid, block, 0, s, a);
     break;
   case ZM_TMF:
-    hipLaunchKernelGGL((k_tz_mt<T, Plan, G, S, ZM_TMF>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_tz_mt<T, Plan, G, S, ZM_TMF, PAIR>), grid, block, 0, s, a);
     break;
   case ZM_TCOMB:
-    hipLaunchKernelGGL((k_tz_mt<T, Plan, G, S, ZM_TCOMB>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((k_tz_mt<T, Plan, G, S, ZM_TCOMB, PAIR>), grid, block, 0, s, a);
     break;
   default:
     return hipErrorInvalidValue;
@@ -625,15 +627,15 @@ void add_t_z_nt(std::vector<TV<T>> &v, int pid) {
                      (PRE ? "/pre" : "") + "/nt",
                  pid, &tz_nt_launch<T, Plan, G, S, PRE>});
 }
-template <typename T, typename Plan, int G, bool S>
+template <typename T, typename Plan, int G, bool S, bool PAIR = false>
 void add_t_z_mt(std::vector<TV<T>> &v, int pid) {
-  if constexpr (t_fits<T, Plan, G, S, true, TK_Z>() &&
+  if constexpr (t_fits<T, Plan, G, S, true, TK_Z>() && (!PAIR || G == 2) &&
                 (S ? 1 : 2) * G * row_padded(Plan::N) * (int)sizeof(T) +
                         tz_table_bytes<T, Plan, G>() <=
                     kMaxLds)
     v.push_back({plan_name<Plan>() + "/G" + std::to_string(G) + (S ? "/split" : "/full") +
-                     "/mt",
-                 pid, &tz_mt_launch<T, Plan, G, S>});
+                     (PAIR ? "/mtp" : "/mt"),
+                 pid, &tz_mt_launch<T, Plan, G, S, PAIR>});
 }
 // Fused step boundary (k_tx_fuse): an nx/2-point plan PH paired with an
 // nx-point plan PF of the same threads per line; WPE > 1 caps the registers
@@ -739,6 +741,8 @@ template <typename T, int N, int KIND> std::vector<TV<T>> make_t_kind() {
         add_t_z_mt<T, P, GM, true>(v, pid);
         add_t_z_mt<T, P, GM, false>(v, pid);
         add_t_z_mt<T, P, 2 * GM, true>(v, pid);
+        add_t_z_mt<T, P, 2, true, true>(v, pid);
+        add_t_z_mt<T, P, 2, false, true>(v, pid);
       });
     }
     if constexpr (KIND == TK_X_FUSE) {
@@ -827,6 +831,7 @@ template <int N, int KIND> std::vector<TV<float>> make_t_kind_xp() {
       add_t<T, P, 2 * G, true, true, KIND>(v, id);
       add_t_z_mt<T, P, G, false>(v, id);
       add_t_z_mt<T, P, G, true>(v, id);
+      add_t_z_mt<T, P, 2, true, true>(v, id);
     });
   }
   if constexpr (KIND == TK_X_INV) {

@@ -54,9 +54,11 @@ def demangled(kernel, t, plan, rest):
     ("k_tz", "r8x8x4x4/E8/xp/G1/split/nt", 0,
      demangled("k_tz", "float", PLAN8XP, "1, true, 0, false, true")),
     ("k_tz", "r8x8x4x4/E8/G1/full/mt", 0,
-     demangled("k_tz_mt", "double", PLAN8, "1, false, 0")),
+     demangled("k_tz_mt", "double", PLAN8, "1, false, 0, false")),
     ("k_tz", "r8x8x4x4/E8/xp/G2/split/mt", 2,
-     demangled("k_tz_mt", "float", PLAN8XP, "2, true, 2")),
+     demangled("k_tz_mt", "float", PLAN8XP, "2, true, 2, false")),
+    ("k_tz", "r16x16x4/E16/G2/split/mtp", 0,
+     demangled("k_tz_mt", "double", PLAN16, "2, true, 0, true")),
     ("k_tx_fuse", "r8x8x8/E8+r16x16x4/E16/G8/split/w4", 0,
      demangled("k_tx_fuse", "double", HALF8 + ", " + PLAN16, "8, true, 4")),
     ("k_tx_fuse", "r4x4x4x4x2/E4+r8x8x4x4/E8/G4/full", 0,

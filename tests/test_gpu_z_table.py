@@ -3,7 +3,8 @@ t_z) against the per-element form (k_tz) of the same plan, lines per block
 and exchange mode.
 
 k_tz_mt evaluates each line's multipliers once per distinct kz*kz (slots
-0..nz/2) and reads element k's pair from slot k or nz - k.  The oracle's
+0..nz/2) and reads element k's pair from slot k or nz - k; its paired form
+shares one table between the lines ky and ny - ky of a block.  The oracle's
 operator tables hold the same bits on the planes k and nz - k
 (test_oracle_model.py, test_oracle_tungsten.py), and the arithmetic that
 applies the pairs is k_tz's, so the two forms must agree bit for bit:
@@ -20,14 +21,20 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 DX, DT, EPS = math.pi / 4, 0.5, -0.25
-N = (256, 128, 128)  # smallest grid with the transposed layout (nx/2, ny, nz >= 128)
+# a grid with the transposed layout (nx/2, ny, nz >= 128) whose 256-point z
+# lines fill a wavefront with two lines per block (the paired-table forms)
+N = (256, 128, 256)
 
 
 def _pairs(names):
-    """(plain, tabulated) variant indices of the same plan, G and exchange."""
-    out = [(names.index(m[:-len("/mt")]), i) for i, m in enumerate(names)
-           if m.endswith("/mt") and m[:-len("/mt")] in names]
-    assert out, names
+    """(plain, tabulated) variant indices of the same plan, G and exchange;
+    "/mtp" forms hold lines ky and ny - ky in one block with a shared table."""
+    out = []
+    for i, m in enumerate(names):
+        base = m.rsplit("/", 1)[0]
+        if m.rsplit("/", 1)[-1] in ("mt", "mtp") and base in names:
+            out.append((names.index(base), i))
+    assert any(names[i].endswith("/mtp") for _, i in out), names
     return out
 
 
