@@ -6,9 +6,9 @@
   steps; the relative L2 and max error against FP64 are checked every 100
   steps against the bound each mode reaches (ERR_BOUND, XP_BOUND), the
   FP64-butterfly error is at most a fifth of the FP32-butterfly error, and
-  the mean is conserved.  The north star's bar of 1e-4 relative L2 after 1000
-  steps is met to about 400 steps with FP32 butterflies and to about 950
-  steps with FP64 butterflies.
+  the mean is conserved.  The FP32-mixed accuracy target, 1e-4 relative L2
+  after 1000 steps, is met to about 400 steps with FP32 butterflies and to
+  about 950 steps with FP64 butterflies.
 * Ensemble: 64 runs of cubic sizes drawn from {32, 48, 64, 96, 128, 192,
   256} (seed 4; r in U(-0.35, -0.15), mean in U(-0.30, -0.15)), FP32-mixed,
   one graph launch per step: every run within FP32 error of the FP64 oracle
