@@ -140,7 +140,7 @@ inline void check_backend(const Value &j) {
     throw std::invalid_argument("plan_options.backend must be a string, got " + b->dump());
   const fft::Backend be = fft::backend_from_string(b->as_string());
   if (be == fft::Backend::CUDA)
-    throw std::runtime_error("CUDA FFT backend requested, but this build provides the HIP "
+    throw std::runtime_error("CUDA FFT backend requested, but this library provides the HIP "
                              "stack only");
 }
 

@@ -31,7 +31,7 @@ inline int comm_size(Comm c) {
 } // namespace pfc
 #else
 namespace pfc {
-// One-rank communicator stand-in (no MPI in this build).
+// One-rank communicator stand-in (no MPI in this library).
 struct Comm {
   int id = 0;
   friend bool operator==(Comm a, Comm b) { return a.id == b.id; }
@@ -50,7 +50,7 @@ namespace pfc::detail {
 inline void require_one_rank(Comm c, const char *who) {
   const int n = comm_size(c);
   if (n != 1)
-    throw std::logic_error(std::string(who) + ": this build runs one rank per GPU; the "
+    throw std::logic_error(std::string(who) + ": this library runs one rank per GPU; the "
                            "communicator has " + std::to_string(n) + " ranks");
 }
 } // namespace pfc::detail

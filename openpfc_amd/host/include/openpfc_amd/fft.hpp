@@ -225,7 +225,7 @@ inline void require_single_part(const Decomposition &d, int rank_id, Comm comm,
   if (decomposition::get_num_domains(d) != 1)
     throw std::logic_error(std::string(who) + ": the decomposition has " +
                            std::to_string(decomposition::get_num_domains(d)) +
-                           " parts; this build plans one part per GPU");
+                           " parts; this library plans one part per GPU");
   if (rank_id != 0)
     throw std::invalid_argument(std::string(who) + ": rank_id " + std::to_string(rank_id) +
                                 " outside a one-part decomposition");
@@ -246,7 +246,7 @@ inline void require_single_part(const Decomposition &d, int rank_id, Comm comm,
   return create_hip(d, 0, comm);
 }
 
-// create_with_backend (src/openpfc/runtime/cpu/fft.cpp:131-191): this build
+// create_with_backend (src/openpfc/runtime/cpu/fft.cpp:131-191): this library
 // provides the HIP backend; FFTW and CUDA requests are rejected as the
 // reference rejects backends it was not built with.
 [[nodiscard]] inline std::unique_ptr<IFFT> create_with_backend(const World &w, Backend b) {
